@@ -1,0 +1,1109 @@
+// mmh_oracle.cpp -- CPU restatement of the moving-mesh reconstruction solve.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: a plain C++ restatement of the reference
+// algorithm (pandaPQY/Moving-Mesh-reconstruction, mmh/*.f90) used by tests/, by the smoke check
+// and by bench.py's cpu_baseline leg as the checker.  The product (the HIP library in
+// moving-mesh-reconstruction_amd/) never links, loads or calls it.
+//
+// Every routine is templated on the working precision T (float = the reference as shipped,
+// double = the reference built with -fdefault-real-8).  Each arithmetic expression keeps the
+// evaluation order of the Fortran statement it restates (left-to-right association, x**2 as
+// x*x, literals converted to T), so that with -ffp-contract=off the results are bit-identical
+// to the flang build of the reference where the algorithm is the same.  Global sums are FP64
+// accumulators as in the reference; their association order is selectable:
+//   sumorder 0 = sequential (k,j,i) order, as the reference runs with one OpenMP thread;
+//   sumorder 1 = the blocked tree order the HIP reduction kernels use (see canon_sum below).
+// Arrays are Fortran column-major, x fastest: offset = i + n*(j + n*k), indices 0-based here.
+//
+// Parity status of each function is listed in the header of tests/test_oracle_ref.py.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+typedef int64_t i64;
+
+inline i64 IX(int i, int j, int k, int n) { return (i64)i + (i64)n * ((i64)j + (i64)n * (i64)k); }
+inline int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+template <class T> inline T fmax_(T a, T b) { return a > b ? a : b; }
+template <class T> inline T fmin_(T a, T b) { return a < b ? a : b; }
+template <class T> inline T sign1(T a) { return std::signbit(a) ? T(-1) : T(1); }
+inline float cosT(float x) { return cosf(x); }
+inline double cosT(double x) { return cos(x); }
+inline float acosT(float x) { return acosf(x); }
+inline double acosT(double x) { return acos(x); }
+inline float sqrtT(float x) { return sqrtf(x); }
+inline double sqrtT(double x) { return sqrt(x); }
+inline float powT(float a, float b) { return powf(a, b); }
+inline double powT(double a, double b) { return pow(a, b); }
+
+// ------------------------------------------------------------------------------------------------
+// Canonical FP64 sums.  Blocked order (sumorder 1): blocks of RB*RPT consecutive elements; thread t
+// of a block adds elements t, t+RB, ..., t+(RPT-1)*RB of its block sequentially from 0.0, then the
+// RB thread sums are combined by a halving tree (s[t] += s[t+h], h = RB/2 ... 1).  Block results
+// are reduced again the same way until one value remains.
+const int RB = 256, RPT = 8;
+
+double blocked_sum(const double* x, i64 n) {
+  if (n <= 0) return 0.0;
+  std::vector<double> cur(x, x + n);
+  while (true) {
+    i64 ch = (i64)RB * RPT;
+    i64 nb = (i64(cur.size()) + ch - 1) / ch;
+    std::vector<double> nxt(nb);
+    for (i64 b = 0; b < nb; ++b) {
+      double s[RB];
+      for (int t = 0; t < RB; ++t) {
+        double a = 0.0;
+        for (int m = 0; m < RPT; ++m) {
+          i64 q = b * ch + (i64)m * RB + t;
+          if (q < (i64)cur.size()) a += cur[q];
+        }
+        s[t] = a;
+      }
+      for (int h = RB / 2; h > 0; h >>= 1)
+        for (int t = 0; t < h; ++t) s[t] = s[t] + s[t + h];
+      nxt[b] = s[0];
+    }
+    if (nb == 1) return nxt[0];
+    cur.swap(nxt);
+  }
+}
+
+template <class T> double canon_sum(const T* x, i64 n, i64 stride, int sumorder) {
+  if (sumorder == 0) {
+    double s = 0.0;
+    for (i64 q = 0; q < n; ++q) s = s + (double)x[q * stride];
+    return s;
+  }
+  std::vector<double> v(n);
+  for (i64 q = 0; q < n; ++q) v[q] = (double)x[q * stride];
+  return blocked_sum(v.data(), n);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Metric (gauss1.f90:1027-1215, cbajraw).  G[c*n3 + p] for c = 0..5 = G11,G12,G13,G22,G23,G33.
+// lsquare = 0: triad branch G = rho_box * adj(A) / det (gauss1.f90:1182-1205), used by the
+// potential-flow solver (iopt/8 mod 8 == 2).  lsquare = 1: adj(A) adj(A)^T / det
+// (gauss1.f90:1173-1181).  Tangled cells (det <= 1e-5) get the identity (gauss1.f90:1162-1171).
+template <class T>
+void metric(T* G, const T* def, const T* u, int nu, int n, T dx, int lsquare) {
+  const i64 n3 = (i64)n * n * n;
+  const T di2 = T(1) / (dx * dx);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k) {
+    int kp = wrap(k + 1, n), km = wrap(k - 1, n);
+    for (int j = 0; j < n; ++j) {
+      int jp = wrap(j + 1, n), jm = wrap(j - 1, n);
+      for (int i = 0; i < n; ++i) {
+        int ip = wrap(i + 1, n), im = wrap(i - 1, n);
+        const T d0 = def[IX(i, j, k, n)];
+        T phixx = (def[IX(ip, j, k, n)] - T(2) * d0 + def[IX(im, j, k, n)]) * di2;
+        T phiyy = (def[IX(i, jp, k, n)] - T(2) * d0 + def[IX(i, jm, k, n)]) * di2;
+        T phizz = (def[IX(i, j, kp, n)] - T(2) * d0 + def[IX(i, j, km, n)]) * di2;
+        T phixy = (def[IX(ip, jp, k, n)] - def[IX(im, jp, k, n)] - def[IX(ip, jm, k, n)] + def[IX(im, jm, k, n)]) * di2 / T(4);
+        T phiyz = (def[IX(i, jp, kp, n)] - def[IX(i, jp, km, n)] - def[IX(i, jm, kp, n)] + def[IX(i, jm, km, n)]) * di2 / T(4);
+        T phixz = (def[IX(ip, j, kp, n)] - def[IX(im, j, kp, n)] - def[IX(ip, j, km, n)] + def[IX(im, j, km, n)]) * di2 / T(4);
+        T a11 = T(1) + phixx, a12 = phixy, a13 = phixz, a22 = T(1) + phiyy, a23 = phiyz, a33 = T(1) + phizz;
+        T det = a11 * a22 * a33 + T(2) * a12 * a13 * a23 - (a13 * a13) * a22 - a11 * (a23 * a23) - (a12 * a12) * a33;
+        T b11 = a22 * a33 - a23 * a23;
+        T b12 = a13 * a23 - a12 * a33;
+        T b13 = a12 * a23 - a13 * a22;
+        T b22 = a11 * a33 - a13 * a13;
+        T b23 = a12 * a13 - a11 * a23;
+        T b33 = a11 * a22 - a12 * a12;
+        if (det <= T(1.e-5)) { b11 = 1; b12 = 0; b13 = 0; b22 = 1; b23 = 0; b33 = 1; det = 1; }
+        const i64 p = IX(i, j, k, n);
+        if (lsquare) {
+          G[0 * n3 + p] = (b11 * b11 + b12 * b12 + b13 * b13) / det;
+          G[1 * n3 + p] = (b11 * b12 + b12 * b22 + b13 * b23) / det;
+          G[2 * n3 + p] = (b11 * b13 + b12 * b23 + b13 * b33) / det;
+          G[3 * n3 + p] = (b12 * b12 + b22 * b22 + b23 * b23) / det;
+          G[4 * n3 + p] = (b12 * b13 + b22 * b23 + b23 * b33) / det;
+          G[5 * n3 + p] = (b13 * b13 + b23 * b23 + b33 * b33) / det;
+        } else {
+          const int is[3] = {im, i, ip}, js[3] = {jm, j, jp}, ks[3] = {km, k, kp};
+          T urg = 0;
+          bool first = true;
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+              for (int c = 0; c < 3; ++c) {
+                T v = u[(i64)nu * IX(is[a], js[b], ks[c], n)];
+                if (first) { urg = v; first = false; } else urg = urg + v;
+              }
+          urg = urg / T(27);
+          G[0 * n3 + p] = urg * b11 / det;
+          G[1 * n3 + p] = urg * b12 / det;
+          G[2 * n3 + p] = urg * b13 / det;
+          G[3 * n3 + p] = urg * b22 / det;
+          G[4 * n3 + p] = urg * b23 / det;
+          G[5 * n3 + p] = urg * b33 / det;
+        }
+      }
+    }
+  }
+}
+
+// The symmetric 19-point curvilinear operator at one point: returns t1, t2 and tdiag
+// (gauss1.f90:954-978 relaxplane, identical to calcresid 683-707).
+template <class T>
+inline void stencil(const T* phi, const T* G, i64 n3, int n, int i, int j, int k, T& t1, T& t2, T& tdiag) {
+  const T* G1 = G;
+  const T* G2 = G + n3;
+  const T* G3 = G + 2 * n3;
+  const T* G4 = G + 3 * n3;
+  const T* G5 = G + 4 * n3;
+  const T* G6 = G + 5 * n3;
+  int ip = wrap(i + 1, n), im = wrap(i - 1, n), jp = wrap(j + 1, n), jm = wrap(j - 1, n), kp = wrap(k + 1, n), km = wrap(k - 1, n);
+  const i64 c = IX(i, j, k, n);
+  T b11p = (G1[IX(ip, j, k, n)] + G1[c]) / T(2);
+  T b11m = (G1[IX(im, j, k, n)] + G1[c]) / T(2);
+  T b22p = (G4[IX(i, jp, k, n)] + G4[c]) / T(2);
+  T b22m = (G4[IX(i, jm, k, n)] + G4[c]) / T(2);
+  T b33p = (G6[IX(i, j, kp, n)] + G6[c]) / T(2);
+  T b33m = (G6[IX(i, j, km, n)] + G6[c]) / T(2);
+  tdiag = -(b11p + b11m + b22p + b22m + b33p + b33m);
+  t1 = (b11p * phi[IX(ip, j, k, n)] + b11m * phi[IX(im, j, k, n)] + b22p * phi[IX(i, jp, k, n)] +
+        b22m * phi[IX(i, jm, k, n)] + b33p * phi[IX(i, j, kp, n)] + b33m * phi[IX(i, j, km, n)]);
+  t2 = ((phi[IX(ip, jp, k, n)] - phi[IX(im, jp, k, n)]) * G2[IX(i, jp, k, n)]
+      - (phi[IX(ip, jm, k, n)] - phi[IX(im, jm, k, n)]) * G2[IX(i, jm, k, n)]
+      + (phi[IX(ip, jp, k, n)] - phi[IX(ip, jm, k, n)]) * G2[IX(ip, j, k, n)]
+      - (phi[IX(im, jp, k, n)] - phi[IX(im, jm, k, n)]) * G2[IX(im, j, k, n)]
+      + (phi[IX(i, jp, kp, n)] - phi[IX(i, jm, kp, n)]) * G5[IX(i, j, kp, n)]
+      - (phi[IX(i, jp, km, n)] - phi[IX(i, jm, km, n)]) * G5[IX(i, j, km, n)]
+      + (phi[IX(i, jp, kp, n)] - phi[IX(i, jp, km, n)]) * G5[IX(i, jp, k, n)]
+      - (phi[IX(i, jm, kp, n)] - phi[IX(i, jm, km, n)]) * G5[IX(i, jm, k, n)]
+      + (phi[IX(ip, j, kp, n)] - phi[IX(im, j, kp, n)]) * G3[IX(i, j, kp, n)]
+      - (phi[IX(ip, j, km, n)] - phi[IX(im, j, km, n)]) * G3[IX(i, j, km, n)]
+      + (phi[IX(ip, j, kp, n)] - phi[IX(ip, j, km, n)]) * G3[IX(ip, j, k, n)]
+      - (phi[IX(im, j, kp, n)] - phi[IX(im, j, km, n)]) * G3[IX(im, j, k, n)]) / T(4);
+}
+
+// One Gauss-Seidel sweep in the reference's colour order.  r1lax (gauss1.f90:265-499) pipelines
+// planes so that odd (1-based) planes see even neighbours from the previous sweep and even planes
+// see odd neighbours from the current one; relaxplane (910-913) updates the four in-plane colours
+// (i odd, j odd), (odd, even), (even, odd), (even, even).  Together this is an 8-colour sweep:
+// 0-based plane parity 0 first, then 1; in each, (i,j) parity (0,0),(0,1),(1,0),(1,1).
+template <class T>
+void sweep(T* phi, const T* rhs, const T* G, int n, T dx, int nsweep) {
+  const i64 n3 = (i64)n * n * n;
+  const T dx2 = dx * dx;
+  for (int s = 0; s < nsweep; ++s)
+    for (int kp = 0; kp < 2; ++kp)
+      for (int io = 0; io < 2; ++io)
+        for (int jo = 0; jo < 2; ++jo) {
+#pragma omp parallel for schedule(static)
+          for (int k = kp; k < n; k += 2)
+            for (int j = jo; j < n; j += 2)
+              for (int i = io; i < n; i += 2) {
+                T t1, t2, tdiag;
+                stencil(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+                const i64 c = IX(i, j, k, n);
+                phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
+              }
+        }
+}
+
+// Residual r = rhs - L[phi] (calcresid, gauss1.f90:683-709).  Writes r if rout != null; returns
+// max|r| (gauss1.f90:718) and, for mode 4, min|r| (gauss1.f90:792).
+template <class T>
+T residual(T* rout, const T* phi, const T* rhs, const T* G, int n, T dx, bool want_min) {
+  const i64 n3 = (i64)n * n * n;
+  const T di2 = T(1) / (dx * dx);
+  T rmax = 0, rmin = T(1.0e20);
+#pragma omp parallel for schedule(static) reduction(max : rmax) reduction(min : rmin)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        T t1, t2, tdiag;
+        stencil(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+        const i64 c = IX(i, j, k, n);
+        T r = rhs[c] - (t1 + t2 + tdiag * phi[c]) * di2;
+        T a = std::fabs(r);
+        rmax = fmax_(rmax, a);
+        rmin = fmin_(rmin, a);
+        if (rout) rout[c] = r;
+      }
+  return want_min ? rmin : rmax;
+}
+
+// Full-weighting restriction of the fine residual fused in calcresid (gauss1.f90:819-859).  Each
+// coarse plane ck receives a "main" contribution from fine planes 2ck, 2ck+1 and a "share" from
+// fine plane 2ck-1.  The r1lax schedule evaluates the fine plane pairs starting with the pair whose
+// coarse index is nrelax_last (the sweep count of the final r1lax call) and proceeding cyclically,
+// so that coarse plane receives main then share; every other coarse plane receives share then
+// main.  The two accumulation orders round differently, so both are reproduced.
+template <class T>
+void restrict_resid(T* rc, const T* r, int n, int nrelax_last) {
+  const int h = n / 2;
+  const int ck0 = nrelax_last % h;
+#pragma omp parallel for schedule(static)
+  for (int ck = 0; ck < h; ++ck) {
+    const int kb = 2 * ck, kbp = 2 * ck + 1, ks = wrap(2 * ck - 1, n);
+    for (int cj = 0; cj < h; ++cj)
+      for (int ci = 0; ci < h; ++ci) {
+        int ib = 2 * ci, ibp = wrap(ib + 1, n), ibm = wrap(ib - 1, n);
+        int jb = 2 * cj, jbp = wrap(jb + 1, n), jbm = wrap(jb - 1, n);
+        auto R = [&](int a, int b, int c) { return r[IX(a, b, c, n)]; };
+        // main: resid + rt(ib,jb,kb)/8 + (...)/16 + (...)/32 + (...)/64
+        auto add_main = [&](T acc) {
+          return acc + R(ib, jb, kb) / T(8)
+            + (R(ibp, jb, kb) + R(ibm, jb, kb) + R(ib, jbp, kb) + R(ib, jbm, kb) + R(ib, jb, kbp)) / T(16)
+            + (R(ibp, jbp, kb) + R(ibp, jbm, kb) + R(ibm, jbp, kb) + R(ibm, jbm, kb) + R(ibp, jb, kbp)
+               + R(ibm, jb, kbp) + R(ib, jbp, kbp) + R(ib, jbm, kbp)) / T(32)
+            + (R(ibp, jbp, kbp) + R(ibp, jbm, kbp) + R(ibm, jbp, kbp) + R(ibm, jbm, kbp)) / T(64);
+        };
+        // share from fine plane ks (the kbp plane of the previous pair), gauss1.f90:850-854
+        auto add_share = [&](T acc) {
+          return acc + R(ib, jb, ks) / T(16)
+            + (R(ibp, jb, ks) + R(ibm, jb, ks) + R(ib, jbp, ks) + R(ib, jbm, ks)) / T(32)
+            + (R(ibp, jbp, ks) + R(ibp, jbm, ks) + R(ibm, jbp, ks) + R(ibm, jbm, ks)) / T(64);
+        };
+        T acc = 0;
+        if (ck == ck0) acc = add_share(add_main(acc));
+        else acc = add_main(add_share(acc));
+        rc[IX(ci, cj, ck, h)] = acc;
+      }
+  }
+}
+
+// 27-point full weighting, vertex-centred (mgutil.f90:74-118 restrict, 121-167 restrictu).  The
+// same formula and term order is the nsmooth filter (limiter.f90:705-711) with stride 1.
+template <class T>
+inline T box27(const T* u, i64 st, int n, int i2, int j2, int k2) {
+  int ip = wrap(i2 + 1, n), im = wrap(i2 - 1, n), jp = wrap(j2 + 1, n), jm = wrap(j2 - 1, n), kp = wrap(k2 + 1, n), km = wrap(k2 - 1, n);
+  auto U = [&](int a, int b, int c) { return u[st * IX(a, b, c, n)]; };
+  return U(i2, j2, k2) / T(8)
+    + (U(ip, j2, k2) + U(im, j2, k2) + U(i2, jp, k2) + U(i2, jm, k2) + U(i2, j2, kp) + U(i2, j2, km)) / T(16)
+    + (U(ip, jp, k2) + U(ip, jm, k2) + U(im, jp, k2) + U(im, jm, k2) + U(ip, j2, kp) + U(ip, j2, km)
+       + U(im, j2, kp) + U(im, j2, km) + U(i2, jp, kp) + U(i2, jp, km) + U(i2, jm, kp) + U(i2, jm, km)) / T(32)
+    + (U(ip, jp, kp) + U(ip, jp, km) + U(ip, jm, kp) + U(ip, jm, km) + U(im, jp, kp) + U(im, jp, km)
+       + U(im, jm, kp) + U(im, jm, km)) / T(64);
+}
+
+template <class T> void restrict27(T* half, const T* full, int n, int stride) {
+  const int h = n / 2;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < h; ++k)
+    for (int j = 0; j < h; ++j)
+      for (int i = 0; i < h; ++i) half[IX(i, j, k, h)] = box27(full, stride, n, 2 * i, 2 * j, 2 * k);
+}
+
+template <class T> void nsmooth(T* a, int n) {
+  const i64 n3 = (i64)n * n * n;
+  std::vector<T> v(n3);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) v[IX(i, j, k, n)] = box27(a, 1, n, i, j, k);
+  std::memcpy(a, v.data(), sizeof(T) * n3);
+}
+
+// Trilinear prolongation (mgutil.f90:38-71 inject adds, 2-34 prolong overwrites).
+template <class T> void prolong_add(T* u, const T* uh, int n, bool add) {
+  const int h = n / 2;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        int ibl = i / 2, ibh = ((i + 1) / 2) % h;
+        int jbl = j / 2, jbh = ((j + 1) / 2) % h;
+        int kbl = k / 2, kbh = ((k + 1) / 2) % h;
+        auto H = [&](int a, int b, int c) { return uh[IX(a, b, c, h)]; };
+        T s = (H(ibl, jbl, kbl) + H(ibl, jbl, kbh) + H(ibl, jbh, kbl) + H(ibl, jbh, kbh)
+             + H(ibh, jbh, kbl) + H(ibh, jbh, kbh) + H(ibh, jbl, kbl) + H(ibh, jbl, kbh)) / T(8);
+        const i64 c = IX(i, j, k, n);
+        u[c] = add ? u[c] + s : s;
+      }
+}
+
+// zerosum (gauss1.f90:1420-1466): subtract the FP64 mean.
+template <class T> void zerosum(T* a, int n, int sumorder) {
+  const i64 n3 = (i64)n * n * n;
+  double s = canon_sum(a, n3, 1, sumorder);
+  s = s / (double)n3;
+  for (i64 q = 0; q < n3; ++q) a[q] = (T)((double)a[q] - s);
+}
+
+// rgzerosum (gauss1.f90:1219-1417): rhs -= (sum rhs / sum det) * det, det = sqrt(g) unclamped.
+template <class T> void det_field(T* det, const T* def, T dx, int n) {
+  const T di2 = T(1) / (dx * dx);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k) {
+    int kp = wrap(k + 1, n), km = wrap(k - 1, n);
+    for (int j = 0; j < n; ++j) {
+      int jp = wrap(j + 1, n), jm = wrap(j - 1, n);
+      for (int i = 0; i < n; ++i) {
+        int ip = wrap(i + 1, n), im = wrap(i - 1, n);
+        const T d0 = def[IX(i, j, k, n)];
+        T phixx = (def[IX(ip, j, k, n)] - T(2) * d0 + def[IX(im, j, k, n)]) * di2;
+        T phiyy = (def[IX(i, jp, k, n)] - T(2) * d0 + def[IX(i, jm, k, n)]) * di2;
+        T phizz = (def[IX(i, j, kp, n)] - T(2) * d0 + def[IX(i, j, km, n)]) * di2;
+        T phixy = (def[IX(ip, jp, k, n)] - def[IX(im, jp, k, n)] - def[IX(ip, jm, k, n)] + def[IX(im, jm, k, n)]) * di2 / T(4);
+        T phiyz = (def[IX(i, jp, kp, n)] - def[IX(i, jp, km, n)] - def[IX(i, jm, kp, n)] + def[IX(i, jm, km, n)]) * di2 / T(4);
+        T phixz = (def[IX(ip, j, kp, n)] - def[IX(im, j, kp, n)] - def[IX(ip, j, km, n)] + def[IX(im, j, km, n)]) * di2 / T(4);
+        T a11 = T(1) + phixx, a12 = phixy, a13 = phixz, a22 = T(1) + phiyy, a23 = phiyz, a33 = T(1) + phizz;
+        det[IX(i, j, k, n)] = a11 * a22 * a33 + T(2) * a12 * a13 * a23 - (a13 * a13) * a22 - a11 * (a23 * a23) - (a12 * a12) * a33;
+      }
+    }
+  }
+}
+
+template <class T> void rgzerosum(T* arr, const T* def, T dx, int n, int sumorder, double* dsum_out) {
+  const i64 n3 = (i64)n * n * n;
+  std::vector<T> det(n3);
+  det_field(det.data(), def, dx, n);
+  double dsum = canon_sum(det.data(), n3, 1, sumorder);
+  double asum = canon_sum(arr, n3, 1, sumorder);
+  T dfact = (T)(asum / dsum);
+  for (i64 q = 0; q < n3; ++q) arr[q] = arr[q] - dfact * det[q];
+  if (dsum_out) { dsum_out[0] = dsum; dsum_out[1] = asum; }
+}
+
+// relax (gauss1.f90:64-160) with r1lax's effect: nrelax sweeps, then (mod(iopt,8) > 0) the
+// residual of the final iterate, its norm and its restriction.
+template <class T>
+void relax(T* arr, T* resid, const T* rhs, const T* def, const T* u, int nu, T* rinf, T dx, int n, int nrelax, int iopt) {
+  const i64 n3 = (i64)n * n * n;
+  const int iresidopt = iopt % 8;
+  const int lsquare = ((iopt / 8) % 8) == 1;
+  std::vector<T> G(6 * n3);
+  metric(G.data(), def, u, nu, n, dx, lsquare);
+  sweep(arr, rhs, G.data(), n, dx, nrelax);
+  const int nrmax = std::min(4, std::max(1, n / 4));
+  int nrl;
+  if (nrelax > nrmax) nrl = (nrelax % nrmax) ? nrelax % nrmax : nrmax;
+  else nrl = nrelax;
+  *rinf = 0;
+  if (iresidopt > 0) {
+    std::vector<T> r(n3);
+    T rn = residual(r.data(), arr, rhs, G.data(), n, dx, iresidopt == 4);
+    if (iresidopt == 2 || iresidopt == 3) *rinf = rn;
+    if (iresidopt == 4) *rinf = rn;
+    if (iresidopt == 1 || iresidopt == 3) restrict_resid(resid, r.data(), n, nrl);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// V-cycle (mgtemplate.f90 / mg8..mg1024.f90 / multigrid.f90:94-121 mg4) and entry (multigrid.f90:16-90)
+struct MgStats {
+  double r1[16], relaxerr[16];
+  int guard[16];
+  int nlev;
+  int zerosum_entry;
+};
+
+template <class T>
+void mg_level(T* u, T* rhs, const T* defpot, const T* ubig, int nu, int n, T dx, int iopt, int ng, int sumorder, MgStats* st, int lev) {
+  const int isolver = iopt % 8;
+  if (n == 4) {
+    T r1, relaxerr;
+    std::vector<T> dummy(8);
+    relax(u, dummy.data(), rhs, defpot, ubig, nu, &r1, dx, 4, 32, isolver * 8 + 2);
+    relax(u, dummy.data(), rhs, defpot, ubig, nu, &relaxerr, dx, 4, 32, isolver * 8 + 2);
+    bool g = (4 != ng && relaxerr > r1 * T(1.001) && r1 > T(1.e-5));
+    if (st && lev < 16) { st->r1[lev] = r1; st->relaxerr[lev] = relaxerr; st->guard[lev] = g; st->nlev = lev + 1; }
+    if (g) std::fill(u, u + 64, T(0));
+    else zerosum(u, 4, sumorder);
+    return;
+  }
+  const int h = n / 2;
+  const i64 h3 = (i64)h * h * h;
+  std::vector<T> uhalf(h3), rhshalf(h3), defhalf(h3), usmall(h3);
+  const int nrelax = 4 * ng / n;
+  T r1, relaxerr;
+  relax(u, rhshalf.data(), rhs, defpot, ubig, nu, &r1, dx, n, nrelax, isolver * 8 + 3);
+  restrict27(defhalf.data(), defpot, n, 1);
+  restrict27(usmall.data(), ubig, n, nu);
+  std::fill(uhalf.begin(), uhalf.end(), T(0));
+  mg_level(uhalf.data(), rhshalf.data(), defhalf.data(), usmall.data(), 1, h, dx * T(2), iopt, ng, sumorder, st, lev + 1);
+  prolong_add(u, uhalf.data(), n, true);
+  relax(u, rhshalf.data(), rhs, defpot, ubig, nu, &relaxerr, dx, n, nrelax, isolver * 8 + 2);
+  bool g = (n != ng && relaxerr > r1);
+  if (st && lev < 16) { st->r1[lev] = r1; st->relaxerr[lev] = relaxerr; st->guard[lev] = g; }
+  if (g) std::fill(u, u + (i64)n * n * n, T(0));
+  else zerosum(u, n, sumorder);
+}
+
+inline bool pow2(int n) { return n >= 4 && (n & (n - 1)) == 0; }
+
+template <class T>
+int multigrid(T* u, T* rhs, const T* defpot, const T* ubig, int nu, T dx, int n, int iopt, int ng, int sumorder, MgStats* st) {
+  if (!pow2(n) || n > ng) return -1;
+  if (st) std::memset(st, 0, sizeof(MgStats));
+  rgzerosum(rhs, defpot, dx, n, sumorder, nullptr);
+  const i64 n3 = (i64)n * n * n;
+  T asum = (T)canon_sum(u, n3, 1, sumorder);
+  if (asum > T(1.e-2)) {
+    zerosum(u, n, sumorder);
+    if (st) st->zerosum_entry = 1;
+  }
+  mg_level(u, rhs, defpot, ubig, nu, n, dx, iopt, ng, sumorder, st, 0);
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// FFT.  Radix-2 Stockham (decimation in time) per line, twiddles W[q] = exp(s*2*pi*i*q/L) from a
+// double table rounded to T; axes transformed x, then y, then z.  X[k] = sum_x x[m] exp(s*2*pi*i*m*k/L).
+template <class T> struct Cx { T r, i; };
+
+template <class T> void fft_line(Cx<T>* x, Cx<T>* y, int L, const Cx<T>* W) {
+  int p = 1;
+  Cx<T>* a = x;
+  Cx<T>* b = y;
+  while (p < L) {
+    const int stride = L / (2 * p);
+    for (int i = 0; i < L / 2; ++i) {
+      int k = i & (p - 1);
+      Cx<T> w = W[k * stride];
+      Cx<T> u0 = a[i], u1 = a[i + L / 2];
+      T tr = w.r * u1.r - w.i * u1.i;
+      T ti = w.r * u1.i + w.i * u1.r;
+      int jj = 2 * i - k;
+      b[jj].r = u0.r + tr; b[jj].i = u0.i + ti;
+      b[jj + p].r = u0.r - tr; b[jj + p].i = u0.i - ti;
+    }
+    std::swap(a, b);
+    p <<= 1;
+  }
+  if (a != x) std::memcpy(x, a, sizeof(Cx<T>) * L);
+}
+
+template <class T> std::vector<Cx<T>> twiddles(int L, int s) {
+  std::vector<Cx<T>> W(std::max(1, L / 2));
+  for (int q = 0; q < L / 2; ++q) {
+    double ang = 2.0 * M_PI * (double)q / (double)L;
+    W[q].r = (T)cos(ang);
+    W[q].i = (T)((double)s * sin(ang));
+  }
+  return W;
+}
+
+// 3-D complex FFT of z (dims d0 x d1 x d2, d0 fastest), sign s.
+template <class T> void fft3(Cx<T>* z, int d0, int d1, int d2, int s) {
+  const int dims[3] = {d0, d1, d2};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int L = dims[ax];
+    if (L == 1) continue;
+    auto W = twiddles<T>(L, s);
+    const i64 st = ax == 0 ? 1 : (ax == 1 ? (i64)d0 : (i64)d0 * d1);
+    const i64 nl = (i64)d0 * d1 * d2 / L;
+#pragma omp parallel
+    {
+      std::vector<Cx<T>> buf(L), tmp(L);
+#pragma omp for schedule(static)
+      for (i64 l = 0; l < nl; ++l) {
+        i64 base;
+        if (ax == 0) base = l * d0;
+        else if (ax == 1) base = (l % d0) + (l / d0) * (i64)d0 * d1;
+        else base = l;
+        for (int q = 0; q < L; ++q) buf[q] = z[base + q * st];
+        fft_line(buf.data(), tmp.data(), L, W.data());
+        for (int q = 0; q < L; ++q) z[base + q * st] = buf[q];
+      }
+    }
+  }
+}
+
+// rlft3 semantics (recipes12.f90:77-135): 3-D real FFT with the Numerical Recipes packing.  data
+// holds n1*n2*n3 reals (x fastest) = complex z(n1/2, n2, n3); speq holds the k1 = n1/2 plane as
+// complex(n2, n3).  isign = +1 forward (kernel exp(+i...)), -1 inverse (unnormalised).  The
+// half-length complex transform is the Stockham FFT above; the pack/unpack recombination follows
+// rlft3's formulas with exact twiddles; the k1 = n1/4 column, which rlft3 maps to itself, is left
+// unchanged.
+template <class T> void rlft3(T* data, T* speq, int n1, int n2, int n3, int isign) {
+  Cx<T>* z = reinterpret_cast<Cx<T>*>(data);
+  Cx<T>* sq = reinterpret_cast<Cx<T>*>(speq);
+  const int h1 = n1 / 2;
+  if (isign == 1) {
+    fft3(z, h1, n2, n3, 1);
+    for (int i3 = 0; i3 < n3; ++i3)
+      for (int i2 = 0; i2 < n2; ++i2) sq[i2 + (i64)n2 * i3] = z[IX(0, i2, i3, 0) + (i64)h1 * (i2 + (i64)n2 * i3)];
+  }
+  const T c2i = T(-0.5) * T(isign);  // c2 = (0, -0.5*isign)
+  auto zat = [&](int a, int b, int c) -> Cx<T>& { return z[(i64)a + (i64)h1 * ((i64)b + (i64)n2 * c)]; };
+  // i1 = 0: pairs with speq
+  std::vector<Cx<T>> sold(sq, sq + (i64)n2 * n3);
+  for (int i3 = 0; i3 < n3; ++i3) {
+    int j3 = i3 == 0 ? 0 : n3 - i3;
+    for (int i2 = 0; i2 < n2; ++i2) {
+      int j2 = i2 == 0 ? 0 : n2 - i2;
+      Cx<T> X = zat(0, i2, i3), Y = sold[j2 + (i64)n2 * j3];
+      Cx<T> s{X.r + Y.r, X.i - Y.i}, d{X.r - Y.r, X.i + Y.i};
+      Cx<T> H1{T(0.5) * s.r, T(0.5) * s.i};
+      Cx<T> H2{-c2i * d.i, c2i * d.r};
+      zat(0, i2, i3) = Cx<T>{H1.r + H2.r, H1.i + H2.i};
+      sq[j2 + (i64)n2 * j3] = Cx<T>{H1.r - H2.r, -(H1.i - H2.i)};
+    }
+  }
+  for (int i1 = 1; i1 < h1 - i1; ++i1) {
+    const int j1 = h1 - i1;
+    double ang = 2.0 * M_PI * (double)i1 / (double)n1;
+    const T wr = (T)cos(ang), wi = (T)((double)isign * sin(ang));
+    for (int i3 = 0; i3 < n3; ++i3) {
+      int j3 = i3 == 0 ? 0 : n3 - i3;
+      for (int i2 = 0; i2 < n2; ++i2) {
+        int j2 = i2 == 0 ? 0 : n2 - i2;
+        Cx<T> X = zat(i1, i2, i3), Y = zat(j1, j2, j3);
+        Cx<T> s{X.r + Y.r, X.i - Y.i}, d{X.r - Y.r, X.i + Y.i};
+        Cx<T> H1{T(0.5) * s.r, T(0.5) * s.i};
+        Cx<T> H2{-c2i * d.i, c2i * d.r};
+        Cx<T> WH{wr * H2.r - wi * H2.i, wr * H2.i + wi * H2.r};
+        zat(i1, i2, i3) = Cx<T>{H1.r + WH.r, H1.i + WH.i};
+        zat(j1, j2, j3) = Cx<T>{H1.r - WH.r, -(H1.i - WH.i)};
+      }
+    }
+  }
+  if (isign == -1) fft3(z, h1, n2, n3, -1);
+}
+
+// ------------------------------------------------------------------------------------------------
+// calcdefp (limiter.f90:17-673).  Eigenvalues of the symmetric 3x3 C = I + H by the trigonometric
+// method (limiter.f90:346-359).
+template <class T>
+inline void eigen_limits(T c11, T c12, T c13, T c22, T c23, T c33, T pi, T& compression) {
+  T trace = c11 + c22 + c33;
+  T d11 = c11 - trace / T(3);
+  T d22 = c22 - trace / T(3);
+  T a = -(d11 * d11 + c12 * c12 + c13 * c13 + d11 * d22 + d22 * d22 + c23 * c23);
+  T b = -(d11 * (c12 * c12)) + (d11 * d11) * d22 - (c12 * c12) * d22 + (c13 * c13) * d22 + d11 * (d22 * d22)
+        - T(2) * c12 * c13 * c23 + d11 * (c23 * c23);
+  if (std::fabs(a) < T(1.e-20)) a = T(1.e-20);
+  T am = T(2) * sqrtT(std::fabs(-a) / T(3));
+  T arg = T(3) * b / a / am;
+  arg = fmax_(T(-1.), fmin_(T(1.), arg));
+  T theta = acosT(arg) / T(3);
+  T r1 = am * cosT(theta);
+  T r2 = am * cosT(theta + T(2) * pi / T(3.));
+  T r3 = am * cosT(theta + T(4) * pi / T(3.));
+  T t3 = trace / T(3);
+  T l1 = r1 + t3, l2 = r2 + t3, l3 = r3 + t3;
+  compression = T(1) / fmin_(fmin_(l1, l2), l3);
+}
+
+struct LimStats {
+  double cmax, detmin, detmax, fluxmin, fluxmax, rho2;
+  int nexpand, ncompress;
+  double rhomin, rhomax, rms;
+  MgStats mg;
+};
+
+template <class T> struct Hess {
+  T xx, yy, zz, xy, yz, xz;
+};
+template <class T> inline Hess<T> hess(const T* f, int n, int i, int j, int k) {
+  int ip = wrap(i + 1, n), im = wrap(i - 1, n), jp = wrap(j + 1, n), jm = wrap(j - 1, n), kp = wrap(k + 1, n), km = wrap(k - 1, n);
+  const T f0 = f[IX(i, j, k, n)];
+  Hess<T> h;
+  h.xx = (f[IX(ip, j, k, n)] - T(2) * f0 + f[IX(im, j, k, n)]);
+  h.yy = (f[IX(i, jp, k, n)] - T(2) * f0 + f[IX(i, jm, k, n)]);
+  h.zz = (f[IX(i, j, kp, n)] - T(2) * f0 + f[IX(i, j, km, n)]);
+  h.xy = (f[IX(ip, jp, k, n)] - f[IX(im, jp, k, n)] - f[IX(ip, jm, k, n)] + f[IX(im, jm, k, n)]) / T(4);
+  h.yz = (f[IX(i, jp, kp, n)] - f[IX(i, jp, km, n)] - f[IX(i, jm, kp, n)] + f[IX(i, jm, km, n)]) / T(4);
+  h.xz = (f[IX(ip, j, kp, n)] - f[IX(im, j, kp, n)] - f[IX(ip, j, km, n)] + f[IX(im, j, km, n)]) / T(4);
+  return h;
+}
+
+// unclamped inverse metric of plane data: b = adj(A)/det, det (limiter.f90:268-286, relaxing.f90:876-901)
+template <class T> inline void binv(const Hess<T>& h, T* b, T& det) {
+  T a11 = T(1) + h.xx, a12 = h.xy, a13 = h.xz, a22 = T(1) + h.yy, a23 = h.yz, a33 = T(1) + h.zz;
+  det = a11 * a22 * a33 + T(2) * a12 * a13 * a23 - (a13 * a13) * a22 - a11 * (a23 * a23) - (a12 * a12) * a33;
+  b[0] = (a22 * a33 - a23 * a23) / det;
+  b[1] = (a13 * a23 - a12 * a33) / det;
+  b[2] = (a12 * a23 - a13 * a22) / det;
+  b[3] = (a11 * a33 - a13 * a13) / det;
+  b[4] = (a12 * a13 - a11 * a23) / det;
+  b[5] = (a11 * a22 - a12 * a12) / det;
+}
+
+template <class T>
+int calcdefp(T* defp, T* tmp, T* tmp2, const T* def, const T* u, T dtold1, T dtaumesh, int nfluid, int n, T compressmax,
+             int ng, int sumorder, LimStats* st) {
+  if (!pow2(n) || nfluid < 1) return -1;
+  const i64 n3 = (i64)n * n * n;
+  const T pi = T(3.14159265358979323846264338328);
+  const float densinit = 1.0f;  // real*4 densinit = 1 (limiter.f90:64,79-88)
+  T dtold = dtold1;
+  T cmax = 0;
+  const T cmpmaxglobal = compressmax;
+  if (st) std::memset(st, 0, sizeof(LimStats));
+  if (cmpmaxglobal < T(0)) return -2;  // diagnostic-only branch limiter.f90:95-108 is not restated
+  if (dtold < T(1.e-8)) dtold = 1;
+  auto U = [&](int c, i64 p) { return u[(i64)nfluid * p + c]; };
+  // 1. clipped mismatch (limiter.f90:134-169)
+  for (i64 p = 0; p < n3; ++p) {
+    T t1 = (T)densinit - U(0, p);
+    T xlow = std::fabs(U(0, p)) * T(0.2);
+    xlow = 0;
+    T xhigh = 2;
+    if (std::fabs(t1) < xlow) t1 = 0;
+    if (std::fabs(t1) > xhigh) t1 = std::copysign(xhigh, t1);
+    tmp[p] = t1 / T(30) / dtold;
+  }
+  for (int r = 0; r < 3; ++r) nsmooth(tmp, n);
+  // 2. per-plane inverse metric of def (no dx scaling) for all planes
+  std::vector<T> B(7 * n3);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        Hess<T> h = hess(def, n, i, j, k);
+        T b[6], det;
+        binv(h, b, det);
+        const i64 p = IX(i, j, k, n);
+        for (int c = 0; c < 6; ++c) B[c * n3 + p] = b[c];
+        B[6 * n3 + p] = det;
+      }
+  // 3. eflux at plane km = k-1 gated by the metric and eigenvalues of plane k (limiter.f90:238-381)
+  std::vector<T> eflux(n3);
+  int nexpand = 0;
+  T detmax = 0, detmin = 1, fluxmax = 0, fluxmin = 0, rho2 = 0;
+  for (int k = 0; k < n; ++k) {
+    const int km = wrap(k - 1, n), kmm = wrap(k - 2, n);
+    for (int j = 0; j < n; ++j) {
+      const int jp = wrap(j + 1, n), jm = wrap(j - 1, n);
+      for (int i = 0; i < n; ++i) {
+        const int ip = wrap(i + 1, n), im = wrap(i - 1, n);
+        const i64 pk = IX(i, j, k, n), pkm = IX(i, j, km, n);
+        const T det = B[6 * n3 + pk];
+        const T b13 = B[2 * n3 + pk], b23 = B[4 * n3 + pk], b33 = B[5 * n3 + pk];
+        auto Bm = [&](int c, int a, int b2, int kk) { return B[c * n3 + IX(a, b2, kk, n)]; };
+        rho2 = rho2 + U(0, pkm) * U(0, pkm) / det;
+        T s1p = (Bm(0, ip, j, km) * U(1, IX(ip, j, km, n)) + Bm(1, ip, j, km) * U(2, IX(ip, j, km, n)) + Bm(2, ip, j, km) * U(3, IX(ip, j, km, n))) / U(0, IX(ip, j, km, n));
+        T s2p = (Bm(1, i, jp, km) * U(1, IX(i, jp, km, n)) + Bm(3, i, jp, km) * U(2, IX(i, jp, km, n)) + Bm(4, i, jp, km) * U(3, IX(i, jp, km, n))) / U(0, IX(i, jp, km, n));
+        T s3p = (b13 * U(1, pk) + b23 * U(2, pk) + b33 * U(3, pk)) / U(0, pk);
+        T s1m = (Bm(0, im, j, km) * U(1, IX(im, j, km, n)) + Bm(1, im, j, km) * U(2, IX(im, j, km, n)) + Bm(2, im, j, km) * U(3, IX(im, j, km, n))) / U(0, IX(im, j, km, n));
+        T s2m = (Bm(1, i, jm, km) * U(1, IX(i, jm, km, n)) + Bm(3, i, jm, km) * U(2, IX(i, jm, km, n)) + Bm(4, i, jm, km) * U(3, IX(i, jm, km, n))) / U(0, IX(i, jm, km, n));
+        T s3m = (Bm(2, i, j, kmm) * U(1, IX(i, j, kmm, n)) + Bm(4, i, j, kmm) * U(2, IX(i, j, kmm, n)) + Bm(5, i, j, kmm) * U(3, IX(i, j, kmm, n))) / U(0, IX(i, j, kmm, n));
+        T ef = (s1p + s2p + s3p - s1m - s2m - s3m) / T(2);
+        ef = ef + tmp[pkm];
+        if (det > T(10.0f * densinit)) {
+          ef = -T(2) * std::fabs(ef);
+          nexpand++;
+        }
+        Hess<T> h = hess(def, n, i, j, k);
+        T compression;
+        eigen_limits(T(1) + h.xx, h.xy, h.xz, T(1) + h.yy, h.yz, T(1) + h.zz, pi, compression);
+        T cmpmax = cmpmaxglobal * powT((T)densinit, T(-1.) / T(3.));
+        T tlim = T(3) * (T(1) / cmpmax - T(1) / compression) / dtold;
+        if (tlim > T(0)) ef = fmax_(T(0.), ef);
+        detmax = fmax_(detmax, det);
+        detmin = fmin_(detmin, det);
+        ef = fmin_(T(1.) / dtaumesh, ef);
+        ef = fmax_(-T(1.) / dtaumesh, ef);
+        eflux[pkm] = ef;
+        fluxmax = fmax_(fluxmax, ef * dtold);
+        fluxmin = fmin_(fluxmin, ef * dtold);
+      }
+    }
+  }
+  std::memcpy(tmp, eflux.data(), sizeof(T) * n3);
+  nsmooth(tmp, n);
+  // 4. solve (limiter.f90:394-408)
+  std::fill(defp, defp + n3, T(0));
+  MgStats* mst = st ? &st->mg : nullptr;
+  int rc = multigrid(defp, tmp, def, u, nfluid, T(1.), n, 2, ng, sumorder, mst);
+  if (rc) return rc;
+  nsmooth(defp, n);
+  nsmooth(defp, n);
+  // 5. compression limiter (limiter.f90:425-533)
+  const int np2 = n + 2;
+  int ncompress = 0;
+  std::vector<T> lim(n3);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        Hess<T> hp = hess(defp, n, i, j, k), hd = hess(def, n, i, j, k);
+        T pxx = hp.xx * dtold, pyy = hp.yy * dtold, pzz = hp.zz * dtold;
+        T pxy = hp.xy * dtold, pyz = hp.yz * dtold, pxz = hp.xz * dtold;
+        pxx = pxx + hd.xx; pyy = pyy + hd.yy; pzz = pzz + hd.zz;
+        pxy = pxy + hd.xy; pyz = pyz + hd.yz; pxz = pxz + hd.xz;
+        T compression;
+        eigen_limits(T(1) + pxx, pxy, pxz, T(1) + pyy, pyz, T(1) + pzz, pi, compression);
+        T cmpmax = cmpmaxglobal * powT((T)densinit, T(-1.) / T(3.));
+        T tlim = T(3) * (T(1) / cmpmax - T(1) / compression) / dtold;
+        cmax = fmax_(cmax, compression);
+        T v = T(0.);
+        if (tlim > T(0)) { ncompress++; v = tlim; }
+        lim[IX(i, j, k, n)] = v;
+      }
+  // tmp2 (defplim, (n+2) x n x n) keeps the limiter source like the reference
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j) {
+      for (int i = 0; i < n; ++i) tmp2[(i64)i + np2 * ((i64)j + (i64)n * k)] = lim[IX(i, j, k, n)];
+    }
+  if (ncompress > 0) {
+    // FFT Poisson solve with the 7-point symbol (limiter.f90:535-596)
+    std::vector<T> data(lim), speq(2 * (i64)n * n, T(0));
+    rlft3(data.data(), speq.data(), n, n, n, 1);
+    for (int k = 0; k < n; ++k) {
+      T zk = T(2) * pi * (T(k + 1) - T(1.)) / T(n);
+      for (int j = 0; j < n; ++j) {
+        T yk = T(2) * pi * (T(j + 1) - T(1.)) / T(n);
+        for (int i = 0; i < n + 2; ++i) {
+          int ii = i / 2;
+          T xk = T(2) * pi * T(ii) / T(n);
+          T ak = T(2) * cosT(xk) + T(2) * cosT(yk) + T(2) * cosT(zk) - T(6);
+          if (std::fabs(ak) < T(1e-10)) ak = 1;
+          T& v = i < n ? data[IX(i, j, k, n)] : speq[(i64)(i - n) + 2 * ((i64)j + (i64)n * k)];
+          v = v / ak;
+        }
+      }
+    }
+    data[0] = 0;
+    rlft3(data.data(), speq.data(), n, n, n, -1);
+    for (i64 p = 0; p < n3; ++p) data[p] = data[p] * T(2.) / T(n) / T(n) / T(n);
+    for (i64 p = 0; p < n3; ++p) defp[p] = defp[p] + data[p];
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < n; ++i) tmp2[(i64)i + np2 * ((i64)j + (i64)n * k)] = data[IX(i, j, k, n)];
+        tmp2[(i64)n + np2 * ((i64)j + (i64)n * k)] = speq[2 * ((i64)j + (i64)n * k)];
+        tmp2[(i64)n + 1 + np2 * ((i64)j + (i64)n * k)] = speq[2 * ((i64)j + (i64)n * k) + 1];
+      }
+  }
+  // 6. diagnostics and the tmp convention on exit (limiter.f90:599-671): with the inert COLD
+  // flags every cell is "hot", so tmp is set to 1 and tmp(1:3,1,1) carry rho2, xlum, temp.
+  T rhomin = U(0, 0), rhomax = rhomin;
+  double rms = 0;
+  for (i64 p = 0; p < n3; ++p) {
+    rhomin = fmin_(rhomin, U(0, p));
+    rhomax = fmax_(rhomax, U(0, p));
+    rms += (double)((U(0, p) - T(1)) * (U(0, p) - T(1)));
+    tmp[p] = 1;
+  }
+  tmp[0] = rho2;
+  tmp[1] = 0;
+  tmp[2] = 0;
+  if (st) {
+    st->cmax = cmax; st->detmin = detmin; st->detmax = detmax; st->fluxmin = fluxmin; st->fluxmax = fluxmax;
+    st->rho2 = rho2; st->nexpand = nexpand; st->ncompress = ncompress;
+    st->rhomin = rhomin; st->rhomax = rhomax; st->rms = sqrt(rms / (double)n3);
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// relaxing-TVD transport (relaxing.f90).  The reference streams planes with rolling buffers; the
+// streamed order is equivalent to a whole-volume two-stage Runge-Kutta step with the freezing
+// speed c evaluated once from the old state:
+//   vt = flux(u); u1 = u - res(u, vt, c)*(dt/2); vtf = flux(u1); u = u - res(u1, vtf, c)*dt.
+template <class T> inline T minmod_(T a, T b) {
+  const T wslopelim = T(1.5), wbee = T(1);
+  auto mm1 = [&](T x, T y) { return fmin_(wslopelim * x, wbee * y + (T(1) - wbee) * x); };
+  T aa = std::fabs(a), ab = std::fabs(b);
+  T mm2 = mm1(fmin_(aa, ab), fmax_(aa, ab));
+  return (sign1(a) + sign1(b)) * mm2 / T(2);
+}
+
+// half-cell TVD flux between cell 0 and cell +1 from values at -1, 0, +1, +2 (wphalfx 532-540)
+template <class T> inline T wphalf(T um, T vm, T u0, T v0, T up, T vp, T upp, T vpp) {
+  T a = (u0 + v0) / T(2);
+  T b = (up - vp) / T(2);
+  T da = (up + vp) / T(2) - a;
+  T dam = a - (um + vm) / T(2);
+  T ap = a + minmod_(dam, da) / T(2);
+  T db = (upp - vpp) / T(2) - b;
+  T dbm = b - (u0 - v0) / T(2);
+  T bp = b - minmod_(db, dbm) / T(2);
+  return ap - bp;
+}
+
+// flux (relaxing.f90:910-996) with fluxeuler (999-1063): vt[(d*nf + l)*n3 + p]
+template <class T>
+void tvd_flux(T* vt, const T* u, const T* defp, const T* bq, int nf, int n) {
+  const i64 n3 = (i64)n * n * n;
+  const T gamma = T(5.) / T(3.);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const i64 p = IX(i, j, k, n);
+        const T rg = bq[6 * n3 + p];
+        const T* ul = u + (i64)nf * p;
+        T rho = ul[0] / rg, mx = ul[1] / rg, my = ul[2] / rg, mz = ul[3] / rg;
+        T vx = mx / rho, vy = my / rho, vz = mz / rho;
+        T engy = ul[4] / rg;
+        T kinetic = rho * (vx * vx + vy * vy + vz * vz) / T(2);
+        T pressure = (gamma - T(1)) * (engy - kinetic);
+        pressure = fmax_(T(0.), pressure);
+        T econv = engy + pressure;
+        T v1[3][5] = {{mx, mx * vx + pressure, my * vx, mz * vx, econv * vx},
+                      {my, mx * vy, my * vy + pressure, mz * vy, econv * vy},
+                      {mz, mx * vz, my * vz, mz * vz + pressure, econv * vz}};
+        int ip = wrap(i + 1, n), im = wrap(i - 1, n), jp = wrap(j + 1, n), jm = wrap(j - 1, n), kp = wrap(k + 1, n), km = wrap(k - 1, n);
+        T pdxg = (defp[IX(ip, j, k, n)] - defp[IX(im, j, k, n)]) / T(2) / rg;
+        T pdyg = (defp[IX(i, jp, k, n)] - defp[IX(i, jm, k, n)]) / T(2) / rg;
+        T pdzg = (defp[IX(i, j, kp, n)] - defp[IX(i, j, km, n)]) / T(2) / rg;
+        T b11 = bq[0 * n3 + p] * rg, b12 = bq[1 * n3 + p] * rg, b13 = bq[2 * n3 + p] * rg;
+        T b22 = bq[3 * n3 + p] * rg, b23 = bq[4 * n3 + p] * rg, b33 = bq[5 * n3 + p] * rg;
+        for (int l = 0; l < nf; ++l) {
+          T ulv = ul[l];
+          T e0 = l < 5 ? v1[0][l] : T(0), e1 = l < 5 ? v1[1][l] : T(0), e2 = l < 5 ? v1[2][l] : T(0);
+          T vtt1 = e0 - ulv * pdxg;
+          T vtt2 = e1 - ulv * pdyg;
+          T vtt3 = e2 - ulv * pdzg;
+          vt[(0 * (i64)nf + l) * n3 + p] = b11 * vtt1 + b12 * vtt2 + b13 * vtt3;
+          vt[(1 * (i64)nf + l) * n3 + p] = b12 * vtt1 + b22 * vtt2 + b23 * vtt3;
+          vt[(2 * (i64)nf + l) * n3 + p] = b13 * vtt1 + b23 * vtt2 + b33 * vtt3;
+        }
+      }
+}
+
+// dpv (relaxing.f90:269-496): flux divergence res[l*n3 + p]
+template <class T>
+void tvd_res(T* res, const T* u, const T* vt, const T* c, int nf, int n) {
+  const i64 n3 = (i64)n * n * n;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const i64 p = IX(i, j, k, n);
+        int ip = wrap(i + 1, n), im = wrap(i - 1, n), ipp = wrap(i + 2, n), imm = wrap(i - 2, n);
+        int jp = wrap(j + 1, n), jm = wrap(j - 1, n), jpp = wrap(j + 2, n), jmm = wrap(j - 2, n);
+        int kp = wrap(k + 1, n), km = wrap(k - 1, n), kpp = wrap(k + 2, n), kmm = wrap(k - 2, n);
+        for (int l = 0; l < nf; ++l) {
+          const T* vz = vt + (2 * (i64)nf + l) * n3;
+          const T* vx = vt + (0 * (i64)nf + l) * n3;
+          const T* vy = vt + (1 * (i64)nf + l) * n3;
+          auto UU = [&](int a, int b, int cc) { return u[(i64)nf * IX(a, b, cc, n) + l]; };
+          // z: vk = vt3/c at planes k-2..k+2
+          const int ksz[5] = {kmm, km, k, kp, kpp};
+          T uk[5], vk[5];
+          for (int q = 0; q < 5; ++q) {
+            i64 pq = IX(i, j, ksz[q], n);
+            vk[q] = vz[pq] / c[pq];
+            uk[q] = UU(i, j, ksz[q]);
+          }
+          T flz1 = wphalf(uk[0], vk[0], uk[1], vk[1], uk[2], vk[2], uk[3], vk[3]);
+          T flz2 = wphalf(uk[1], vk[1], uk[2], vk[2], uk[3], vk[3], uk[4], vk[4]);
+          T ck = c[p], ckp = c[IX(i, j, kp, n)], ckm = c[IX(i, j, km, n)];
+          T r = (ckp + ck) / T(2) * flz2 - (ck + ckm) / T(2) * flz1;
+          // x
+          auto vxc = [&](int a) { i64 q = IX(a, j, k, n); return vx[q] / c[q]; };
+          auto uxc = [&](int a) { return UU(a, j, k); };
+          T flxi = wphalf(uxc(im), vxc(im), uxc(i), vxc(i), uxc(ip), vxc(ip), uxc(ipp), vxc(ipp));
+          T flxim = wphalf(uxc(imm), vxc(imm), uxc(im), vxc(im), uxc(i), vxc(i), uxc(ip), vxc(ip));
+          T cip = c[IX(ip, j, k, n)], cim = c[IX(im, j, k, n)];
+          T fp = (cip + ck) / T(2) * flxi;
+          T fm = (ck + cim) / T(2) * flxim;
+          r = r + fp - fm;
+          // y
+          auto vyc = [&](int b) { i64 q = IX(i, b, k, n); return vy[q] / c[q]; };
+          auto uyc = [&](int b) { return UU(i, b, k); };
+          T flyj = wphalf(uyc(jm), vyc(jm), uyc(j), vyc(j), uyc(jp), vyc(jp), uyc(jpp), vyc(jpp));
+          T flyjm = wphalf(uyc(jmm), vyc(jmm), uyc(jm), vyc(jm), uyc(j), vyc(j), uyc(jp), vyc(jp));
+          T cjp = c[IX(i, jp, k, n)], cjm = c[IX(i, jm, k, n)];
+          r = r + (cjp + ck) / T(2) * flyj - (ck + cjm) / T(2) * flyjm;
+          res[(i64)l * n3 + p] = r;
+        }
+      }
+}
+
+template <class T> int relaxing(T* u, const T* def, const T* defp, T dt, int nf, int n) {
+  if (!pow2(n) || n < 16 || nf != 5) return -1;
+  const i64 n3 = (i64)n * n * n;
+  const T gamma = T(5.) / T(3.);
+  std::vector<T> bq(7 * n3), c(n3), vt(3 * (i64)nf * n3), res((i64)nf * n3), u1((i64)nf * n3);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        Hess<T> h = hess(def, n, i, j, k);
+        T b[6], det;
+        binv(h, b, det);
+        const i64 p = IX(i, j, k, n);
+        bq[6 * n3 + p] = det;
+        for (int q = 0; q < 6; ++q) bq[q * n3 + p] = b[q];
+      }
+  // cflkeuler (relaxing.f90:1067-1139)
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const i64 p = IX(i, j, k, n);
+        int ip = wrap(i + 1, n), im = wrap(i - 1, n), jp = wrap(j + 1, n), jm = wrap(j - 1, n), kp = wrap(k + 1, n), km = wrap(k - 1, n);
+        T pdx = (defp[IX(ip, j, k, n)] - defp[IX(im, j, k, n)]) / T(2);
+        T pdy = (defp[IX(i, jp, k, n)] - defp[IX(i, jm, k, n)]) / T(2);
+        T pdz = (defp[IX(i, j, kp, n)] - defp[IX(i, j, km, n)]) / T(2);
+        T b11 = bq[0 * n3 + p], b12 = bq[1 * n3 + p], b13 = bq[2 * n3 + p], b22 = bq[3 * n3 + p], b23 = bq[4 * n3 + p], b33 = bq[5 * n3 + p];
+        T det = bq[6 * n3 + p];
+        const T* ul = u + (i64)nf * p;
+        T vx = ul[1] / ul[0], vy = ul[2] / ul[0], vz = ul[3] / ul[0];
+        T rho = ul[0] / det;
+        T kin = rho * (vx * vx + vy * vy + vz * vz) / T(2);
+        T ed = ul[4] / det - kin;
+        T pressure = (gamma - T(1)) * ed;
+        T cs = sqrtT(gamma * std::fabs(pressure / rho));
+        T v1 = vx - pdx, v2 = vy - pdy, v3 = vz - pdz;
+        T vv1 = b11 * v1 + b12 * v2 + b13 * v3;
+        T vv2 = b12 * v1 + b22 * v2 + b23 * v3;
+        T vv3 = b13 * v1 + b23 * v2 + b33 * v3;
+        T blim1 = cs * sqrtT(b11 * b11 + b12 * b12 + b13 * b13) + std::fabs(vv1);
+        T blim2 = cs * sqrtT(b12 * b12 + b22 * b22 + b23 * b23) + std::fabs(vv2);
+        T blim3 = cs * sqrtT(b13 * b13 + b23 * b23 + b33 * b33) + std::fabs(vv3);
+        c[p] = fmax_(fmax_(blim1, blim2), blim3);
+      }
+  tvd_flux(vt.data(), u, defp, bq.data(), nf, n);
+  tvd_res(res.data(), u, vt.data(), c.data(), nf, n);
+  const T hdt = dt / T(2);
+  for (i64 p = 0; p < n3; ++p)
+    for (int l = 0; l < nf; ++l) u1[(i64)nf * p + l] = u[(i64)nf * p + l] - res[(i64)l * n3 + p] * hdt;
+  tvd_flux(vt.data(), u1.data(), defp, bq.data(), nf, n);
+  tvd_res(res.data(), u1.data(), vt.data(), c.data(), nf, n);
+  for (i64 p = 0; p < n3; ++p)
+    for (int l = 0; l < nf; ++l) u[(i64)nf * p + l] = u[(i64)nf * p + l] - res[(i64)l * n3 + p] * dt;
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Output stage: scalar2gradient (gradient.f90:1-74) and defp2delta (reconstruct.f90:1-41), on the
+// rlft3 semantics above.  k_d = 2*pi*m_d/n with m_x = 0..n/2 on the half axis and m_y, m_z signed,
+// the n/2 plane counted positive.
+template <class T> void scalar2gradient(const T* in, T* out, int n) {
+  const i64 n3 = (i64)n * n * n;
+  const T pi = (T)(atan(1.0) * 4.0);
+  std::vector<T> d(n3), sq(2 * (i64)n * n);
+  for (int dir = 0; dir < 3; ++dir) {
+    std::memcpy(d.data(), in, sizeof(T) * n3);
+    std::fill(sq.begin(), sq.end(), T(0));
+    rlft3(d.data(), sq.data(), n, n, n, 1);
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int ic = 0; ic <= n / 2; ++ic) {
+          T km;
+          if (dir == 0) km = T(ic);
+          else if (dir == 1) km = (j + 1 <= n / 2 + 1) ? T(j) : T(j - n);
+          else km = (k + 1 <= n / 2 + 1) ? T(k) : T(k - n);
+          T* re = ic < n / 2 ? &d[IX(2 * ic, j, k, n)] : &sq[2 * ((i64)j + (i64)n * k)];
+          T* im = re + 1;
+          T A = *re, Bv = *im;
+          T cr = A * T(0) - Bv * km, ci = A * km + Bv * T(0);
+          cr = cr / T(n) * T(2.) * pi;
+          ci = ci / T(n) * T(2.) * pi;
+          *re = cr;
+          *im = ci;
+        }
+    rlft3(d.data(), sq.data(), n, n, n, -1);
+    for (i64 p = 0; p < n3; ++p) out[(i64)dir * n3 + p] = d[p] * T(2.) / T(n) / T(n) / T(n);
+  }
+}
+
+template <class T> void defp2delta(const T* in, T* out, int n) {
+  const i64 n3 = (i64)n * n * n;
+  const T pi = (T)(atan(1.0) * 4.0);
+  std::vector<T> d(in, in + n3), sq(2 * (i64)n * n, T(0));
+  rlft3(d.data(), sq.data(), n, n, n, 1);
+  for (int k = 0; k < n; ++k) {
+    T k3 = (k + 1 <= n / 2 + 1) ? T(k) : T(k - n);
+    k3 = k3 / T(n) * T(2.) * pi;
+    for (int j = 0; j < n; ++j) {
+      T k2 = (j + 1 <= n / 2 + 1) ? T(j) : T(j - n);
+      k2 = k2 / T(n) * T(2.) * pi;
+      for (int ic = 0; ic <= n / 2; ++ic) {
+        T k1 = T(ic);
+        k1 = k1 / T(n) * T(2.) * pi;
+        T f = k1 * k1 + k2 * k2 + k3 * k3;
+        T* re = ic < n / 2 ? &d[IX(2 * ic, j, k, n)] : &sq[2 * ((i64)j + (i64)n * k)];
+        re[0] = re[0] * f;
+        re[1] = re[1] * f;
+      }
+    }
+  }
+  rlft3(d.data(), sq.data(), n, n, n, -1);
+  for (i64 p = 0; p < n3; ++p) out[p] = d[p] * T(2.) / T(n) / T(n) / T(n);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Synthetic lognormal density (builder-defined input; the reference reads N-body density files).
+// White N(0,1) from a counter-based SplitMix64 hash + Box-Muller, coloured by sqrt(P(k)) with
+// P = |k|^slope exp(-(|k|/ks)^2), P(0) = 0, via the complex FFT above; G scaled to variance
+// sigma^2; rho = exp(G - sigma^2/2) normalised to mean 1.
+inline uint64_t smix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+void lognormal(double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed) {
+  const i64 n3 = (i64)n * n * n;
+  std::vector<Cx<double>> z(n3);
+  const uint64_t key = smix(seed ^ 0x6A09E667F3BCC909ULL);
+  for (i64 q = 0; q < n3; ++q) {
+    uint64_t h1 = smix(key + 2 * (uint64_t)q), h2 = smix(key + 2 * (uint64_t)q + 1);
+    double u1 = ((double)(h1 >> 11) + 1.0) * 0x1.0p-53;
+    double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+    z[q].r = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+    z[q].i = 0.0;
+  }
+  fft3(z.data(), n, n, n, -1);
+  const double ks = ks_frac * M_PI;
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        double mi = i <= n / 2 ? i : i - n, mj = j <= n / 2 ? j : j - n, mk = k <= n / 2 ? k : k - n;
+        double kx = 2.0 * M_PI * mi / n, ky = 2.0 * M_PI * mj / n, kz = 2.0 * M_PI * mk / n;
+        double kk = sqrt(kx * kx + ky * ky + kz * kz);
+        double a = kk > 0 ? sqrt(pow(kk, slope) * exp(-(kk / ks) * (kk / ks))) : 0.0;
+        i64 p = IX(i, j, k, n);
+        z[p].r = z[p].r * a;
+        z[p].i = z[p].i * a;
+      }
+  fft3(z.data(), n, n, n, 1);
+  std::vector<double> g(n3), g2(n3);
+  for (i64 q = 0; q < n3; ++q) { g[q] = z[q].r; g2[q] = g[q] * g[q]; }
+  double var = blocked_sum(g2.data(), n3) / (double)n3;
+  double sc = sigma / sqrt(var);
+  for (i64 q = 0; q < n3; ++q) g[q] = exp(g[q] * sc - 0.5 * sigma * sigma);
+  double mean = blocked_sum(g.data(), n3) / (double)n3;
+  for (i64 q = 0; q < n3; ++q) rho[q] = g[q] / mean;
+}
+
+}  // namespace
+
+// ================================================================================================
+// C entry points (ctypes).  Suffix _d = double, _f = float.
+extern "C" {
+
+#define OR_BOTH(X) X(double, d) X(float, f)
+
+#define DEF_SIMPLE(T, S)                                                                                       \
+  void or_metric_##S(T* G, const T* def, const T* u, int nu, int n, T dx, int lsq) { metric(G, def, u, nu, n, dx, lsq); } \
+  void or_sweep_##S(T* phi, const T* rhs, const T* G, int n, T dx, int ns) { sweep(phi, rhs, G, n, dx, ns); }             \
+  T or_residual_##S(T* r, const T* phi, const T* rhs, const T* G, int n, T dx) { return residual(r, phi, rhs, G, n, dx, false); } \
+  void or_restrict_resid_##S(T* rc, const T* r, int n, int nrl) { restrict_resid(rc, r, n, nrl); }                       \
+  void or_restrict_##S(T* half, const T* full, int n, int stride) { restrict27(half, full, n, stride); }                  \
+  void or_inject_##S(T* u, const T* uh, int n) { prolong_add(u, uh, n, true); }                                           \
+  void or_prolong_##S(T* u, const T* uh, int n) { prolong_add(u, uh, n, false); }                                         \
+  void or_nsmooth_##S(T* a, int n) { nsmooth(a, n); }                                                                     \
+  void or_zerosum_##S(T* a, int n, int so) { zerosum(a, n, so); }                                                          \
+  void or_det_##S(T* det, const T* def, T dx, int n) { det_field(det, def, dx, n); }                                       \
+  void or_rgzerosum_##S(T* a, const T* def, T dx, int n, int so, double* sums) { rgzerosum(a, def, dx, n, so, sums); }     \
+  double or_sum_##S(const T* x, int64_t n, int so) { return canon_sum(x, n, 1, so); }                                      \
+  void or_relax_##S(T* arr, T* resid, const T* rhs, const T* def, const T* u, int nu, T* rinf, T dx, int n, int nrelax,   \
+                    int iopt) { relax(arr, resid, rhs, def, u, nu, rinf, dx, n, nrelax, iopt); }                            \
+  int or_multigrid_##S(T* u, T* rhs, const T* defpot, const T* ubig, int nu, T dx, int n, int iopt, int ng, int so,       \
+                       void* st) { return multigrid(u, rhs, defpot, ubig, nu, dx, n, iopt, ng, so, (MgStats*)st); }        \
+  int or_calcdefp_##S(T* defp, T* tmp, T* tmp2, const T* def, const T* u, T dtold, T dtaumesh, int nfluid, int n,        \
+                      T compressmax, int ng, int so, void* st) {                                                          \
+    return calcdefp(defp, tmp, tmp2, def, u, dtold, dtaumesh, nfluid, n, compressmax, ng, so, (LimStats*)st);            \
+  }                                                                                                                        \
+  int or_relaxing_##S(T* u, const T* def, const T* defp, T dt, int nf, int n) { return relaxing(u, def, defp, dt, nf, n); } \
+  void or_rlft3_##S(T* data, T* speq, int n1, int n2, int n3, int isign) { rlft3(data, speq, n1, n2, n3, isign); }        \
+  void or_fft3_##S(T* z, int d0, int d1, int d2, int s) { fft3(reinterpret_cast<Cx<T>*>(z), d0, d1, d2, s); }               \
+  void or_gradient_##S(const T* in, T* out, int n) { scalar2gradient(in, out, n); }                                        \
+  void or_delta_##S(const T* in, T* out, int n) { defp2delta(in, out, n); }
+
+OR_BOTH(DEF_SIMPLE)
+
+void or_lognormal(double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed) {
+  lognormal(rho, n, sigma, slope, ks_frac, seed);
+}
+int or_mgstats_size(void) { return (int)sizeof(MgStats); }
+int or_limstats_size(void) { return (int)sizeof(LimStats); }
+}
