@@ -1,0 +1,168 @@
+/* mmh.h -- C ABI of the MI355X moving-mesh reconstruction library (libmmh_amd.so).
+ *
+ * Drop-in boundary for the reconstruction solve of Pen's MMH code
+ * (pandaPQY/Moving-Mesh-reconstruction).  The reference exposes Fortran externals with implicit
+ * interfaces, every argument by reference; this header gives the same operations with explicit
+ * C types.  Each entry point names the reference routine it replaces (file:line in the reference).
+ *
+ * Conventions
+ *  - Fields are N^3 arrays in Fortran order (x fastest), periodic; u is the reference's AoS
+ *    u(nu, N, N, N) whose component 1 is the density.
+ *  - _d entry points compute in FP64, _f in FP32.  Pointers of the device entry points are HIP
+ *    device pointers; work is queued on the context stream and the call returns after it has
+ *    completed.  _host entry points take host pointers and copy through context buffers.
+ *  - Every entry point returns MMH_OK or an error code; nothing exits the process (the reference
+ *    calls stop / pause, mmh/multigrid.f90:28-35,83-86, mmh/gauss1.f90:243-254).
+ *  - N must be a power of two, 4 <= N <= 1024 for the solver (16 <= N for relaxing and the
+ *    limiter, which the reference also requires, mmh/relaxing.f90:18).
+ */
+#ifndef MMH_H
+#define MMH_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mmh_ctx mmh_ctx;
+
+#define MMH_OK 0
+#define MMH_EINVAL 1   /* invalid size or argument */
+#define MMH_EDEVICE 2  /* HIP runtime error */
+#define MMH_ENOMEM 3   /* device allocation failed */
+#define MMH_ENOCONV 4  /* mmh_solve did not reach the tolerance in max_cycles */
+
+#define MMH_MAXLEV 16
+
+/* Per-level V-cycle record (the reference prints these: mmh/mg64.f90:43-48). */
+typedef struct {
+  int nlev;                     /* levels visited, finest first */
+  int n[MMH_MAXLEV];            /* grid size of each level */
+  double r1[MMH_MAXLEV];        /* max|r| after pre-smoothing */
+  double relaxerr[MMH_MAXLEV];  /* max|r| after post-smoothing */
+  int guard[MMH_MAXLEV];        /* 1 if the divergence guard zeroed the level */
+  int entry_zerosum;            /* 1 if the initial guess was re-centred (mmh/multigrid.f90:42) */
+} mmh_mg_stats;
+
+/* calcdefp diagnostics (mmh/limiter.f90:657-667). */
+typedef struct {
+  double cmax, detmin, detmax, fluxmin, fluxmax;
+  int nexpand, ncompress;
+  mmh_mg_stats mg;
+} mmh_lim_stats;
+
+/* Solve to a relative residual (builder-defined; the reference does one V-cycle per outer
+ * iteration, mmh/limiter.f90:408). */
+typedef struct {
+  double tol;        /* stop when max|rhs - L u| / max|rhs| <= tol */
+  int max_cycles;
+  int schedule;      /* 0: reference sweeps nu = 4*ng/N per level, 32+32 on 4^3; 1: fixed */
+  int nu_pre, nu_post, nu_coarse;  /* schedule 1: sweeps per level and on 4^3 */
+  int ng;            /* schedule 0: the reference's NG (top size if 0) */
+  int mixed;         /* 0: FP64 V-cycles; 1: FP32 V-cycles inside FP64 defect correction */
+} mmh_solve_opts;
+
+typedef struct {
+  int cycles;
+  double relres[256];  /* relative residual after each cycle */
+  double rhs_norm;
+} mmh_solve_stats;
+
+/* ---- context ------------------------------------------------------------------------------ */
+int mmh_create(mmh_ctx** ctx, int device);
+int mmh_destroy(mmh_ctx* ctx);
+const char* mmh_error_string(mmh_ctx* ctx);
+int mmh_set_stream(mmh_ctx* ctx, void* hip_stream); /* NULL = the null stream (default) */
+int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffers */
+
+/* ---- solver (device pointers) --------------------------------------------------------------- */
+/* relax (mmh/gauss1.f90:64): nrelax colour sweeps of arr; mod(iopt,8) selects residual output:
+ * 1 restricted residual into resid ((N/2)^3), 2 max-norm into *rinf, 3 both. */
+int mmh_relax_d(mmh_ctx*, double* arr, double* resid, const double* rhs, const double* deformpot, const double* u,
+                int nu, double* rinf, double dx, int n, int nrelax, int iopt);
+int mmh_relax_f(mmh_ctx*, float* arr, float* resid, const float* rhs, const float* deformpot, const float* u, int nu,
+                float* rinf, float dx, int n, int nrelax, int iopt);
+
+/* multigrid (mmh/multigrid.f90:16): one V-cycle on u with rhs projected in place (rgzerosum);
+ * ng is the reference's compile-time NG that sets the sweep schedule (0 = n). */
+int mmh_multigrid_d(mmh_ctx*, double* u, double* rhs, const double* defpot, const double* ubig, double dx, int n, int nu,
+                    int iopt, int ng, mmh_mg_stats* st);
+int mmh_multigrid_f(mmh_ctx*, float* u, float* rhs, const float* defpot, const float* ubig, float dx, int n, int nu,
+                    int iopt, int ng, mmh_mg_stats* st);
+
+/* solve to tolerance with repeated V-cycles (FP64 or mixed precision); rhs projected in place. */
+int mmh_solve_d(mmh_ctx*, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
+                const mmh_solve_opts* opts, mmh_solve_stats* st);
+
+/* calcdefp (mmh/limiter.f90:17): mesh update defp from def and the density u(1,...); tmp (N^3)
+ * and tmp2 ((N+2) N^2) are scratch outputs as in the reference. */
+int mmh_calcdefp_d(mmh_ctx*, double* defp, double* tmp, double* tmp2, const double* def, const double* u, double dtold,
+                   double dtaumesh, int nfluid, int n, double compressmax, int ng, mmh_lim_stats* st);
+int mmh_calcdefp_f(mmh_ctx*, float* defp, float* tmp, float* tmp2, const float* def, const float* u, float dtold,
+                   float dtaumesh, int nfluid, int n, float compressmax, int ng, mmh_lim_stats* st);
+
+/* Steps 1-3 of calcdefp (mmh/limiter.f90:134-392): the right-hand side calcdefp passes to
+ * multigrid at limiter.f90:408 (clipped mismatch, smoothing, eflux with the expansion limiter and
+ * compression gate, smoothing), without the solve. */
+int mmh_limiter_rhs_d(mmh_ctx*, double* tmp, const double* def, const double* u, double dtold, double dtaumesh,
+                      int nfluid, int n, double compressmax);
+
+/* relaxing (mmh/relaxing.f90:21): relaxing-TVD transport of u(5,N,N,N) onto the new mesh. */
+int mmh_relaxing_d(mmh_ctx*, double* u, const double* def, const double* defp, double dt, int nfluid, int n);
+int mmh_relaxing_f(mmh_ctx*, float* u, const float* def, const float* defp, float dt, int nfluid, int n);
+
+/* scalar2gradient (mmh/gradient.f90:1): psi (N,N,N,3) = spectral gradient with the rlft3 sign. */
+int mmh_gradient_d(mmh_ctx*, const double* phi, double* psi, int n);
+int mmh_gradient_f(mmh_ctx*, const float* phi, float* psi, int n);
+/* defp2delta (mmh/reconstruct.f90:1): delta = spectral -laplacian(phi). */
+int mmh_delta_d(mmh_ctx*, const double* phi, double* delta, int n);
+int mmh_delta_f(mmh_ctx*, const float* phi, float* delta, int n);
+
+/* nsmooth (mmh/limiter.f90:677), restrict (mmh/mgutil.f90:74), inject (mmh/mgutil.f90:38). */
+int mmh_nsmooth_d(mmh_ctx*, double* a, int n);
+int mmh_nsmooth_f(mmh_ctx*, float* a, int n);
+int mmh_restrict_d(mmh_ctx*, double* half, const double* full, int n, int stride);
+int mmh_restrict_f(mmh_ctx*, float* half, const float* full, int n, int stride);
+int mmh_inject_d(mmh_ctx*, double* u, const double* uh, int n);
+int mmh_inject_f(mmh_ctx*, float* u, const float* uh, int n);
+/* rlft3 (mmh/recipes12.f90:77): in-place real 3-D FFT in the Numerical Recipes packing. */
+int mmh_rlft3_d(mmh_ctx*, double* data, double* speq, int n, int isign);
+
+/* Synthetic input: lognormal density rho = exp(G - sigma^2/2), mean 1, with G Gaussian of
+ * spectrum |k|^slope exp(-(|k|/(ks_frac*pi))^2) and variance sigma^2 (no reference counterpart:
+ * the reference reads density files, mmh/main.f90:61-65). */
+int mmh_lognormal_d(mmh_ctx*, double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed);
+
+/* Smoother timing: nsweeps colour sweeps of phi with a precomputed metric, timed with HIP events
+ * on the context stream; *ms = average per sweep.  G holds 6 N^3 coefficients (mmh_metric_d). */
+int mmh_metric_d(mmh_ctx*, double* G, const double* def, const double* u, int nu, int n, double dx, int lsquare);
+int mmh_metric_f(mmh_ctx*, float* G, const float* def, const float* u, int nu, int n, float dx, int lsquare);
+int mmh_time_sweeps_d(mmh_ctx*, double* phi, const double* rhs, const double* G, int n, double dx, int nsweeps, float* ms);
+int mmh_time_sweeps_f(mmh_ctx*, float* phi, const float* rhs, const float* G, int n, float dx, int nsweeps, float* ms);
+
+/* ---- host-pointer entry points (used by the Fortran drop-ins) ------------------------------- */
+int mmh_relax_host_f(mmh_ctx*, float* arr, float* resid, const float* rhs, const float* deformpot, const float* u,
+                     int nu, float* rinf, float dx, int n, int nrelax, int iopt);
+int mmh_relax_host_d(mmh_ctx*, double* arr, double* resid, const double* rhs, const double* deformpot, const double* u,
+                     int nu, double* rinf, double dx, int n, int nrelax, int iopt);
+int mmh_multigrid_host_f(mmh_ctx*, float* u, float* rhs, const float* defpot, const float* ubig, float dx, int n, int nu,
+                         int iopt, int ng);
+int mmh_multigrid_host_d(mmh_ctx*, double* u, double* rhs, const double* defpot, const double* ubig, double dx, int n,
+                         int nu, int iopt, int ng);
+int mmh_calcdefp_host_f(mmh_ctx*, float* defp, float* tmp, float* tmp2, const float* def, const float* u, float dtold,
+                        float dtaumesh, int nfluid, int n, float compressmax);
+int mmh_calcdefp_host_d(mmh_ctx*, double* defp, double* tmp, double* tmp2, const double* def, const double* u,
+                        double dtold, double dtaumesh, int nfluid, int n, double compressmax);
+int mmh_relaxing_host_f(mmh_ctx*, float* u, const float* def, const float* defp, float dt, int nfluid, int n);
+int mmh_relaxing_host_d(mmh_ctx*, double* u, const double* def, const double* defp, double dt, int nfluid, int n);
+int mmh_gradient_host_f(mmh_ctx*, const float* phi, float* psi, int n);
+int mmh_delta_host_f(mmh_ctx*, const float* phi, float* delta, int n);
+
+/* The process-wide context used by the Fortran drop-ins (created on first use on device 0). */
+mmh_ctx* mmh_default_ctx(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,338 @@
+// kernels.hpp -- gfx950 device kernels of the moving-mesh reconstruction solve.
+//
+// Arrays are column-major (x fastest), offset = i + n*(j + n*k), periodic in all axes.  Every
+// arithmetic statement keeps the association order of the reference Fortran statement it
+// implements (cited per kernel); the library is built with -ffp-contract=off, so FP64 and FP32
+// results are bit-identical to a sequential evaluation of those statements.  Global sums are
+// FP64 with a fixed blocked tree order (k_sum_stage), so they are deterministic run to run.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mmh {
+
+typedef int64_t i64;
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+__device__ __forceinline__ i64 ix3(int i, int j, int k, int n) { return (i64)i + (i64)n * ((i64)j + (i64)n * (i64)k); }
+template <class T> __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
+template <class T> __device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
+
+constexpr int RB = 256;  // reduction block
+constexpr int RPT = 8;   // elements per thread per reduction block
+
+// ------------------------------------------------------------------------------------------------
+// Deterministic FP64 reductions.  Block b covers elements [b*RB*RPT, (b+1)*RB*RPT); thread t adds
+// elements t, t+RB, ... of the block in order; the RB per-thread values are combined by a halving
+// tree.  Repeated on the block results until one value is left.
+// op 0: sum of x; op 1: max of |x|.
+template <class T, int OP>
+__global__ __launch_bounds__(RB) void k_reduce_stage(double* __restrict__ out, const T* __restrict__ x, i64 n) {
+  __shared__ double s[RB];
+  const int t = threadIdx.x;
+  const i64 base = (i64)blockIdx.x * (RB * RPT);
+  double a = 0.0;
+#pragma unroll
+  for (int m = 0; m < RPT; ++m) {
+    i64 q = base + (i64)m * RB + t;
+    if (q < n) {
+      double v = (double)x[q];
+      if (OP == 0) a = a + v;
+      else a = dmax(a, fabs(v));
+    }
+  }
+  s[t] = a;
+  __syncthreads();
+  for (int h = RB / 2; h > 0; h >>= 1) {
+    if (t < h) s[t] = OP == 0 ? s[t] + s[t + h] : dmax(s[t], s[t + h]);
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x] = s[0];
+}
+
+template <class T> __global__ void k_fill(T* __restrict__ a, T v, i64 n) {
+  i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) a[q] = v;
+}
+
+template <class T> __global__ void k_copy_strided(T* __restrict__ dst, const T* __restrict__ src, i64 n, int stride) {
+  i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = src[q * stride];
+}
+
+// ------------------------------------------------------------------------------------------------
+// Metric G = rho_box * adj(A)/det (mmh/gauss1.f90:1128-1205, cbajraw, triad branch) or
+// adj(A) adj(A)^T / det (lsquare, 1173-1181); A = I + Hess(def)/dx^2; det <= 1e-5 -> identity.
+// G is stored as 6 planes of n^3: G11, G12, G13, G22, G23, G33.
+template <class T>
+__global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __restrict__ def, const T* __restrict__ u,
+                                                int nu, int n, T dx, int lsquare) {
+  const i64 n3 = (i64)n * n * n;
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n3) return;
+  const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
+  const int ip = wrapi(i + 1, n), im = wrapi(i - 1, n), jp = wrapi(j + 1, n), jm = wrapi(j - 1, n), kp = wrapi(k + 1, n), km = wrapi(k - 1, n);
+  const T di2 = T(1) / (dx * dx);
+  const T d0 = def[p];
+  T phixx = (def[ix3(ip, j, k, n)] - T(2) * d0 + def[ix3(im, j, k, n)]) * di2;
+  T phiyy = (def[ix3(i, jp, k, n)] - T(2) * d0 + def[ix3(i, jm, k, n)]) * di2;
+  T phizz = (def[ix3(i, j, kp, n)] - T(2) * d0 + def[ix3(i, j, km, n)]) * di2;
+  T phixy = (def[ix3(ip, jp, k, n)] - def[ix3(im, jp, k, n)] - def[ix3(ip, jm, k, n)] + def[ix3(im, jm, k, n)]) * di2 / T(4);
+  T phiyz = (def[ix3(i, jp, kp, n)] - def[ix3(i, jp, km, n)] - def[ix3(i, jm, kp, n)] + def[ix3(i, jm, km, n)]) * di2 / T(4);
+  T phixz = (def[ix3(ip, j, kp, n)] - def[ix3(im, j, kp, n)] - def[ix3(ip, j, km, n)] + def[ix3(im, j, km, n)]) * di2 / T(4);
+  T a11 = T(1) + phixx, a12 = phixy, a13 = phixz, a22 = T(1) + phiyy, a23 = phiyz, a33 = T(1) + phizz;
+  T det = a11 * a22 * a33 + T(2) * a12 * a13 * a23 - (a13 * a13) * a22 - a11 * (a23 * a23) - (a12 * a12) * a33;
+  T b11 = a22 * a33 - a23 * a23;
+  T b12 = a13 * a23 - a12 * a33;
+  T b13 = a12 * a23 - a13 * a22;
+  T b22 = a11 * a33 - a13 * a13;
+  T b23 = a12 * a13 - a11 * a23;
+  T b33 = a11 * a22 - a12 * a12;
+  if (det <= T(1.e-5)) { b11 = 1; b12 = 0; b13 = 0; b22 = 1; b23 = 0; b33 = 1; det = 1; }
+  if (lsquare) {
+    G[0 * n3 + p] = (b11 * b11 + b12 * b12 + b13 * b13) / det;
+    G[1 * n3 + p] = (b11 * b12 + b12 * b22 + b13 * b23) / det;
+    G[2 * n3 + p] = (b11 * b13 + b12 * b23 + b13 * b33) / det;
+    G[3 * n3 + p] = (b12 * b12 + b22 * b22 + b23 * b23) / det;
+    G[4 * n3 + p] = (b12 * b13 + b22 * b23 + b23 * b33) / det;
+    G[5 * n3 + p] = (b13 * b13 + b23 * b23 + b33 * b33) / det;
+    return;
+  }
+  const int is[3] = {im, i, ip}, js[3] = {jm, j, jp}, ks[3] = {km, k, kp};
+  T urg = u[(i64)nu * ix3(im, jm, km, n)];
+  bool first = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (first) { first = false; continue; }
+        urg = urg + u[(i64)nu * ix3(is[a], js[b], ks[c], n)];
+      }
+  urg = urg / T(27);
+  G[0 * n3 + p] = urg * b11 / det;
+  G[1 * n3 + p] = urg * b12 / det;
+  G[2 * n3 + p] = urg * b13 / det;
+  G[3 * n3 + p] = urg * b22 / det;
+  G[4 * n3 + p] = urg * b23 / det;
+  G[5 * n3 + p] = urg * b33 / det;
+}
+
+// The 19-point symmetric curvilinear operator at one point (mmh/gauss1.f90:954-978).
+template <class T>
+__device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __restrict__ G, i64 n3, int n, int i, int j,
+                                          int k, T& t1, T& t2, T& tdiag) {
+  const T* G1 = G;
+  const T* G2 = G + n3;
+  const T* G3 = G + 2 * n3;
+  const T* G4 = G + 3 * n3;
+  const T* G5 = G + 4 * n3;
+  const T* G6 = G + 5 * n3;
+  const int ip = wrapi(i + 1, n), im = wrapi(i - 1, n), jp = wrapi(j + 1, n), jm = wrapi(j - 1, n), kp = wrapi(k + 1, n), km = wrapi(k - 1, n);
+  const i64 c = ix3(i, j, k, n);
+  T b11p = (G1[ix3(ip, j, k, n)] + G1[c]) / T(2);
+  T b11m = (G1[ix3(im, j, k, n)] + G1[c]) / T(2);
+  T b22p = (G4[ix3(i, jp, k, n)] + G4[c]) / T(2);
+  T b22m = (G4[ix3(i, jm, k, n)] + G4[c]) / T(2);
+  T b33p = (G6[ix3(i, j, kp, n)] + G6[c]) / T(2);
+  T b33m = (G6[ix3(i, j, km, n)] + G6[c]) / T(2);
+  tdiag = -(b11p + b11m + b22p + b22m + b33p + b33m);
+  t1 = (b11p * phi[ix3(ip, j, k, n)] + b11m * phi[ix3(im, j, k, n)] + b22p * phi[ix3(i, jp, k, n)] +
+        b22m * phi[ix3(i, jm, k, n)] + b33p * phi[ix3(i, j, kp, n)] + b33m * phi[ix3(i, j, km, n)]);
+  t2 = ((phi[ix3(ip, jp, k, n)] - phi[ix3(im, jp, k, n)]) * G2[ix3(i, jp, k, n)]
+      - (phi[ix3(ip, jm, k, n)] - phi[ix3(im, jm, k, n)]) * G2[ix3(i, jm, k, n)]
+      + (phi[ix3(ip, jp, k, n)] - phi[ix3(ip, jm, k, n)]) * G2[ix3(ip, j, k, n)]
+      - (phi[ix3(im, jp, k, n)] - phi[ix3(im, jm, k, n)]) * G2[ix3(im, j, k, n)]
+      + (phi[ix3(i, jp, kp, n)] - phi[ix3(i, jm, kp, n)]) * G5[ix3(i, j, kp, n)]
+      - (phi[ix3(i, jp, km, n)] - phi[ix3(i, jm, km, n)]) * G5[ix3(i, j, km, n)]
+      + (phi[ix3(i, jp, kp, n)] - phi[ix3(i, jp, km, n)]) * G5[ix3(i, jp, k, n)]
+      - (phi[ix3(i, jm, kp, n)] - phi[ix3(i, jm, km, n)]) * G5[ix3(i, jm, k, n)]
+      + (phi[ix3(ip, j, kp, n)] - phi[ix3(im, j, kp, n)]) * G3[ix3(i, j, kp, n)]
+      - (phi[ix3(ip, j, km, n)] - phi[ix3(im, j, km, n)]) * G3[ix3(i, j, km, n)]
+      + (phi[ix3(ip, j, kp, n)] - phi[ix3(ip, j, km, n)]) * G3[ix3(ip, j, k, n)]
+      - (phi[ix3(im, j, kp, n)] - phi[ix3(im, j, km, n)]) * G3[ix3(im, j, k, n)]) / T(4);
+}
+
+// One colour class of the 8-colour Gauss-Seidel sweep: points with (k, i, j) parity (kpar, io, jo)
+// (relaxplane, mmh/gauss1.f90:910-980).  One thread per point of the colour.
+template <class T>
+__global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G, int n,
+                                                T dx2, int kpar, int io, int jo) {
+  const int h = n >> 1;
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (i64)h * h * h) return;
+  const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
+  const i64 n3 = (i64)n * n * n;
+  T t1, t2, tdiag;
+  stencil19(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+  const i64 c = ix3(i, j, k, n);
+  phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
+}
+
+// Residual r = rhs - L[phi] (calcresid, mmh/gauss1.f90:683-709); optional store; per-block max|r|.
+template <class T>
+__global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restrict__ bmax, const T* __restrict__ phi,
+                                              const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2) {
+  __shared__ double s[RB];
+  const i64 n3 = (i64)n * n * n;
+  const i64 p = (i64)blockIdx.x * RB + threadIdx.x;
+  double a = 0.0;
+  if (p < n3) {
+    const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
+    T t1, t2, tdiag;
+    stencil19(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+    T rv = rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
+    if (r) r[p] = rv;
+    a = (double)fabs(rv);
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = RB / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bmax[blockIdx.x] = s[0];
+}
+
+// Full-weighting restriction of the residual as accumulated by calcresid
+// (mmh/gauss1.f90:819-859): coarse plane ck gets a main part from fine planes 2ck, 2ck+1 and a
+// share from fine plane 2ck-1; plane ck0 accumulates main then share, every other plane share
+// then main (the order in which r1lax's plane pipeline reaches them).
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0) {
+  const int h = n >> 1;
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (i64)h * h * h) return;
+  const int ci = (int)(q % h), cj = (int)((q / h) % h), ck = (int)(q / ((i64)h * h));
+  const int kb = 2 * ck, kbp = 2 * ck + 1, ks = wrapi(2 * ck - 1, n);
+  const int ib = 2 * ci, ibp = wrapi(ib + 1, n), ibm = wrapi(ib - 1, n);
+  const int jb = 2 * cj, jbp = wrapi(jb + 1, n), jbm = wrapi(jb - 1, n);
+#define R_(a, b, c) r[ix3(a, b, c, n)]
+  T acc = 0;
+  if (ck == ck0) {
+    acc = acc + R_(ib, jb, kb) / T(8)
+      + (R_(ibp, jb, kb) + R_(ibm, jb, kb) + R_(ib, jbp, kb) + R_(ib, jbm, kb) + R_(ib, jb, kbp)) / T(16)
+      + (R_(ibp, jbp, kb) + R_(ibp, jbm, kb) + R_(ibm, jbp, kb) + R_(ibm, jbm, kb) + R_(ibp, jb, kbp) + R_(ibm, jb, kbp) + R_(ib, jbp, kbp) + R_(ib, jbm, kbp)) / T(32)
+      + (R_(ibp, jbp, kbp) + R_(ibp, jbm, kbp) + R_(ibm, jbp, kbp) + R_(ibm, jbm, kbp)) / T(64);
+    acc = acc + R_(ib, jb, ks) / T(16)
+      + (R_(ibp, jb, ks) + R_(ibm, jb, ks) + R_(ib, jbp, ks) + R_(ib, jbm, ks)) / T(32)
+      + (R_(ibp, jbp, ks) + R_(ibp, jbm, ks) + R_(ibm, jbp, ks) + R_(ibm, jbm, ks)) / T(64);
+  } else {
+    acc = acc + R_(ib, jb, ks) / T(16)
+      + (R_(ibp, jb, ks) + R_(ibm, jb, ks) + R_(ib, jbp, ks) + R_(ib, jbm, ks)) / T(32)
+      + (R_(ibp, jbp, ks) + R_(ibp, jbm, ks) + R_(ibm, jbp, ks) + R_(ibm, jbm, ks)) / T(64);
+    acc = acc + R_(ib, jb, kb) / T(8)
+      + (R_(ibp, jb, kb) + R_(ibm, jb, kb) + R_(ib, jbp, kb) + R_(ib, jbm, kb) + R_(ib, jb, kbp)) / T(16)
+      + (R_(ibp, jbp, kb) + R_(ibp, jbm, kb) + R_(ibm, jbp, kb) + R_(ibm, jbm, kb) + R_(ibp, jb, kbp) + R_(ibm, jb, kbp) + R_(ib, jbp, kbp) + R_(ib, jbm, kbp)) / T(32)
+      + (R_(ibp, jbp, kbp) + R_(ibp, jbm, kbp) + R_(ibm, jbp, kbp) + R_(ibm, jbm, kbp)) / T(64);
+  }
+#undef R_
+  rc[q] = acc;
+}
+
+// 27-point weights 1/8, 1/16, 1/32, 1/64 around fine point (i2,j2,k2): restrict/restrictu
+// (mmh/mgutil.f90:106-112, 153-161) and nsmooth (mmh/limiter.f90:705-711) share term order.
+template <class T>
+__device__ __forceinline__ T box27(const T* __restrict__ u, int st, int n, int i2, int j2, int k2) {
+  const int ip = wrapi(i2 + 1, n), im = wrapi(i2 - 1, n), jp = wrapi(j2 + 1, n), jm = wrapi(j2 - 1, n), kp = wrapi(k2 + 1, n), km = wrapi(k2 - 1, n);
+#define U_(a, b, c) u[(i64)st * ix3(a, b, c, n)]
+  return U_(i2, j2, k2) / T(8)
+    + (U_(ip, j2, k2) + U_(im, j2, k2) + U_(i2, jp, k2) + U_(i2, jm, k2) + U_(i2, j2, kp) + U_(i2, j2, km)) / T(16)
+    + (U_(ip, jp, k2) + U_(ip, jm, k2) + U_(im, jp, k2) + U_(im, jm, k2) + U_(ip, j2, kp) + U_(ip, j2, km)
+       + U_(im, j2, kp) + U_(im, j2, km) + U_(i2, jp, kp) + U_(i2, jp, km) + U_(i2, jm, kp) + U_(i2, jm, km)) / T(32)
+    + (U_(ip, jp, kp) + U_(ip, jp, km) + U_(ip, jm, kp) + U_(ip, jm, km) + U_(im, jp, kp) + U_(im, jp, km)
+       + U_(im, jm, kp) + U_(im, jm, km)) / T(64);
+#undef U_
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict27(T* __restrict__ half, const T* __restrict__ full, int n, int stride) {
+  const int h = n >> 1;
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (i64)h * h * h) return;
+  const int i = (int)(q % h), j = (int)((q / h) % h), k = (int)(q / ((i64)h * h));
+  half[q] = box27(full, stride, n, 2 * i, 2 * j, 2 * k);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_smooth27(T* __restrict__ out, const T* __restrict__ in, int n) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (i64)n * n * n) return;
+  const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
+  out[p] = box27(in, 1, n, i, j, k);
+}
+
+// Trilinear prolongation (mmh/mgutil.f90:38-71 inject adds; 2-34 prolong overwrites).
+template <class T>
+__global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __restrict__ uh, int n, int add) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (i64)n * n * n) return;
+  const int h = n >> 1;
+  const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
+  const int ibl = i >> 1, ibh = ((i + 1) >> 1) % h, jbl = j >> 1, jbh = ((j + 1) >> 1) % h, kbl = k >> 1, kbh = ((k + 1) >> 1) % h;
+#define H_(a, b, c) uh[ix3(a, b, c, h)]
+  T s = (H_(ibl, jbl, kbl) + H_(ibl, jbl, kbh) + H_(ibl, jbh, kbl) + H_(ibl, jbh, kbh)
+       + H_(ibh, jbh, kbl) + H_(ibh, jbh, kbh) + H_(ibh, jbl, kbl) + H_(ibh, jbl, kbh)) / T(8);
+#undef H_
+  u[p] = add ? u[p] + s : s;
+}
+
+// Unclamped sqrt(g) of the mesh (mmh/gauss1.f90:1283-1295, rgzerosum).
+template <class T>
+__global__ __launch_bounds__(256) void k_det(T* __restrict__ det, const T* __restrict__ def, int n, T dx) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (i64)n * n * n) return;
+  const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
+  const int ip = wrapi(i + 1, n), im = wrapi(i - 1, n), jp = wrapi(j + 1, n), jm = wrapi(j - 1, n), kp = wrapi(k + 1, n), km = wrapi(k - 1, n);
+  const T di2 = T(1) / (dx * dx);
+  const T d0 = def[p];
+  T phixx = (def[ix3(ip, j, k, n)] - T(2) * d0 + def[ix3(im, j, k, n)]) * di2;
+  T phiyy = (def[ix3(i, jp, k, n)] - T(2) * d0 + def[ix3(i, jm, k, n)]) * di2;
+  T phizz = (def[ix3(i, j, kp, n)] - T(2) * d0 + def[ix3(i, j, km, n)]) * di2;
+  T phixy = (def[ix3(ip, jp, k, n)] - def[ix3(im, jp, k, n)] - def[ix3(ip, jm, k, n)] + def[ix3(im, jm, k, n)]) * di2 / T(4);
+  T phiyz = (def[ix3(i, jp, kp, n)] - def[ix3(i, jp, km, n)] - def[ix3(i, jm, kp, n)] + def[ix3(i, jm, km, n)]) * di2 / T(4);
+  T phixz = (def[ix3(ip, j, kp, n)] - def[ix3(im, j, kp, n)] - def[ix3(ip, j, km, n)] + def[ix3(im, j, km, n)]) * di2 / T(4);
+  T a11 = T(1) + phixx, a12 = phixy, a13 = phixz, a22 = T(1) + phiyy, a23 = phiyz, a33 = T(1) + phizz;
+  det[p] = a11 * a22 * a33 + T(2) * a12 * a13 * a23 - (a13 * a13) * a22 - a11 * (a23 * a23) - (a12 * a12) * a33;
+}
+
+// rhs -= dfact*det with dfact = (T)(asum/dsum) read from device scalars (mmh/gauss1.f90:1348,1404).
+template <class T>
+__global__ void k_rgz_update(T* __restrict__ arr, const T* __restrict__ det, const double* __restrict__ sums, i64 n3) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n3) return;
+  const T dfact = (T)(sums[1] / sums[0]);
+  arr[p] = arr[p] - dfact * det[p];
+}
+
+// zerosum (mmh/gauss1.f90:1450-1460) or, when the level's divergence guard fired, szero.
+// mode 0: always zerosum; mode 1: guard rinf[1] > rinf[0] (mg level); mode 2: the 4^3 guard
+// relaxerr > 1.001*r1 and r1 > 1e-5 (mmh/multigrid.f90:113).  guard_out records the decision.
+template <class T>
+__global__ void k_zerosum_guard(T* __restrict__ a, const double* __restrict__ sum, const double* __restrict__ r1re, i64 n3,
+                                int mode, int* __restrict__ guard_out) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  bool g = false;
+  if (mode == 1) g = (T)r1re[1] > (T)r1re[0];
+  else if (mode == 2) g = ((T)r1re[1] > (T)r1re[0] * T(1.001) && (T)r1re[0] > T(1.e-5));
+  if (p == 0 && guard_out) *guard_out = g ? 1 : 0;
+  if (p >= n3) return;
+  if (g) { a[p] = T(0); return; }
+  const double m = sum[0] / (double)n3;
+  a[p] = (T)((double)a[p] - m);
+}
+
+// usum + conditional zerosum at the multigrid entry (mmh/multigrid.f90:41-46).
+template <class T>
+__global__ void k_entry_zerosum(T* __restrict__ a, const double* __restrict__ sum, i64 n3, int* __restrict__ flag) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const T asum = (T)sum[0];
+  const bool z = asum > T(1.e-2);
+  if (p == 0 && flag) *flag = z ? 1 : 0;
+  if (p >= n3 || !z) return;
+  const double m = sum[0] / (double)n3;
+  a[p] = (T)((double)a[p] - m);
+}
+
+}  // namespace mmh

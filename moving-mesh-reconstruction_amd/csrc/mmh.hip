@@ -1,0 +1,1020 @@
+// mmh.hip -- host orchestration and C ABI of the MI355X moving-mesh reconstruction library.
+//
+// One context owns a HIP stream and a cache of device work buffers.  The V-cycle runs entirely
+// on the device: residual norms, guard decisions and mean removals are device scalars consumed by
+// later kernels, so a cycle issues no host synchronisation and no allocation once its buffers
+// exist.  Declarations and reference citations of the entry points are in include/mmh.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mmh.h"
+#include "kernels.hpp"
+#include "kernels_lim.hpp"
+
+using namespace mmh;
+
+struct mmh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<std::string, std::pair<void*, size_t>> ws;
+  std::set<std::string> uploaded;  // constant tables already copied to the device
+  std::string err;
+};
+
+namespace {
+
+struct MmhError : std::runtime_error {
+  int code;
+  MmhError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                      \
+  do {                                                                                                 \
+    hipError_t e_ = (x);                                                                               \
+    if (e_ != hipSuccess)                                                                              \
+      throw MmhError(e_ == hipErrorOutOfMemory ? MMH_ENOMEM : MMH_EDEVICE,                             \
+                     std::string(#x) + ": " + hipGetErrorString(e_));                                  \
+  } while (0)
+
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+
+inline void require(bool c, const char* msg) {
+  if (!c) throw MmhError(MMH_EINVAL, msg);
+}
+inline bool is_pow2(int n) { return n >= 4 && n <= 4096 && (n & (n - 1)) == 0; }
+__host__ __device__ inline i64 cube(int n) { return (i64)n * n * n; }
+inline unsigned nblk(i64 n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+void* ws_get(mmh_ctx* c, const std::string& key, size_t bytes) {
+  auto it = c->ws.find(key);
+  if (it != c->ws.end() && it->second.second >= bytes) return it->second.first;
+  if (it != c->ws.end()) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(it->second.first));
+    c->ws.erase(it);
+  }
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+  c->ws[key] = {p, bytes};
+  return p;
+}
+template <class T> T* wsT(mmh_ctx* c, const std::string& key, i64 count) {
+  return (T*)ws_get(c, key + ":" + std::to_string(sizeof(T)), sizeof(T) * (size_t)count);
+}
+
+template <class T> void fill(mmh_ctx* c, T* a, T v, i64 n) {
+  k_fill<T><<<nblk(n), 256, 0, c->stream>>>(a, v, n);
+  LAUNCHCHK();
+}
+
+// FP64 sum (op 0) or max|x| (op 1) of x[0..n) into the device double *dst, blocked tree order.
+template <class T> void reduce(mmh_ctx* c, const T* x, i64 n, int op, double* dst) {
+  const i64 ch = (i64)RB * RPT;
+  i64 nb = (n + ch - 1) / ch;
+  if (nb <= 1) {
+    if (op == 0) k_reduce_stage<T, 0><<<1, RB, 0, c->stream>>>(dst, x, n);
+    else k_reduce_stage<T, 1><<<1, RB, 0, c->stream>>>(dst, x, n);
+    LAUNCHCHK();
+    return;
+  }
+  double* a = (double*)ws_get(c, "red_a", sizeof(double) * nb);
+  double* b = (double*)ws_get(c, "red_b", sizeof(double) * ((nb + ch - 1) / ch + 1));
+  if (op == 0) k_reduce_stage<T, 0><<<(unsigned)nb, RB, 0, c->stream>>>(a, x, n);
+  else k_reduce_stage<T, 1><<<(unsigned)nb, RB, 0, c->stream>>>(a, x, n);
+  LAUNCHCHK();
+  while (nb > 1) {
+    i64 nb2 = (nb + ch - 1) / ch;
+    double* out = nb2 == 1 ? dst : b;
+    if (op == 0) k_reduce_stage<double, 0><<<(unsigned)nb2, RB, 0, c->stream>>>(out, a, nb);
+    else k_reduce_stage<double, 1><<<(unsigned)nb2, RB, 0, c->stream>>>(out, a, nb);
+    LAUNCHCHK();
+    std::swap(a, b);
+    nb = nb2;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Multigrid hierarchy.  Level 0 aliases the caller's arrays; coarser levels live in the context.
+template <class T> struct Hier {
+  int nlev = 0;
+  int n[MMH_MAXLEV];
+  int nu[MMH_MAXLEV];
+  T dx[MMH_MAXLEV];
+  T* u[MMH_MAXLEV];
+  T* rhs[MMH_MAXLEV];
+  const T* def[MMH_MAXLEV];
+  const T* u1[MMH_MAXLEV];
+  T* G[MMH_MAXLEV];
+  T* r[MMH_MAXLEV];
+  double* scal;  // per level: [0] r1, [1] relaxerr, [2] sum; then [3*MAXLEV] rhs sums
+  int* guard;    // per level, plus entry flag at [MAXLEV]
+};
+
+struct Sched {
+  int kind = 0;  // 0 reference, 1 fixed
+  int ng = 0;
+  int nu_pre = 2, nu_post = 2, nu_coarse = 32;
+  int pre(int n) const { return kind == 0 ? 4 * ng / n : nu_pre; }
+  int post(int n) const { return kind == 0 ? 4 * ng / n : nu_post; }
+  int coarse() const { return kind == 0 ? 32 : nu_coarse; }
+};
+
+// Builds level arrays and metrics for a problem of size n with defpot/ubig at level 0.
+template <class T>
+void build_hier(mmh_ctx* c, Hier<T>& H, const std::string& tag, T* u, T* rhs, const T* defpot, const T* ubig, int nu, T dx,
+                int n, int lsquare, bool want_G0 = true, T* G0 = nullptr) {
+  int nl = 0;
+  for (int m = n; m >= 4; m >>= 1) ++nl;
+  require(nl <= MMH_MAXLEV, "too many levels");
+  H.nlev = nl;
+  H.scal = (double*)ws_get(c, tag + "scal", sizeof(double) * 4 * MMH_MAXLEV);
+  H.guard = (int*)ws_get(c, tag + "guard", sizeof(int) * (MMH_MAXLEV + 1));
+  for (int l = 0; l < nl; ++l) {
+    const int m = n >> l;
+    H.n[l] = m;
+    H.dx[l] = l == 0 ? dx : H.dx[l - 1] * T(2);
+    const std::string t = tag + std::to_string(l);
+    if (l == 0) {
+      H.u[0] = u; H.rhs[0] = rhs; H.def[0] = defpot; H.u1[0] = ubig; H.nu[0] = nu;
+    } else {
+      H.u[l] = wsT<T>(c, t + "u", cube(m));
+      H.rhs[l] = wsT<T>(c, t + "rhs", cube(m));
+      T* d = wsT<T>(c, t + "def", cube(m));
+      T* w = wsT<T>(c, t + "u1", cube(m));
+      const int pm = n >> (l - 1);
+      k_restrict27<T><<<nblk(cube(m)), 256, 0, c->stream>>>(d, H.def[l - 1], pm, 1);
+      k_restrict27<T><<<nblk(cube(m)), 256, 0, c->stream>>>(w, H.u1[l - 1], pm, H.nu[l - 1]);
+      LAUNCHCHK();
+      H.def[l] = d; H.u1[l] = w; H.nu[l] = 1;
+    }
+    H.r[l] = wsT<T>(c, t + "r", cube(m));
+    if (l == 0 && G0) H.G[0] = G0;
+    else H.G[l] = wsT<T>(c, t + "G", 6 * cube(m));
+    if (l > 0 || want_G0) {
+      k_metric<T><<<nblk(cube(m)), 256, 0, c->stream>>>(H.G[l], H.def[l], H.u1[l], H.nu[l], m, H.dx[l], lsquare);
+      LAUNCHCHK();
+    }
+  }
+}
+
+template <class T> void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns) {
+  const T dx2 = dx * dx;
+  const int h = n / 2;
+  const unsigned nb = nblk(cube(h));
+  for (int s = 0; s < ns; ++s)
+    for (int kp = 0; kp < 2; ++kp)
+      for (int io = 0; io < 2; ++io)
+        for (int jo = 0; jo < 2; ++jo) {
+          k_colour<T><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+        }
+  LAUNCHCHK();
+}
+
+// relax on the device: sweeps, then residual (store when rc != null), max-norm into *rinf_dev,
+// and the calcresid restriction into rc.
+template <class T>
+void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n, T dx, int nrelax, int want_norm, T* rc,
+               double* rinf_dev) {
+  sweeps(c, phi, rhs, G, n, dx, nrelax);
+  if (!want_norm && !rc) return;
+  const i64 n3 = cube(n);
+  const unsigned nb = nblk(n3, RB);
+  double* bmax = (double*)ws_get(c, "bmax", sizeof(double) * nb);
+  k_resid<T><<<nb, RB, 0, c->stream>>>(rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx));
+  LAUNCHCHK();
+  reduce<double>(c, bmax, nb, 1, rinf_dev);
+  if (rc) {
+    const int nrmax = std::min(4, std::max(1, n / 4));
+    int nrl = nrelax > nrmax ? (nrelax % nrmax ? nrelax % nrmax : nrmax) : nrelax;
+    k_restrict_resid<T><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(rc, rscratch, n, nrl % (n / 2));
+    LAUNCHCHK();
+  }
+}
+
+// One V-cycle from level l (mgtemplate.f90 / mg*.f90; mg4 at multigrid.f90:94-121).
+template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const Sched& S) {
+  const int n = H.n[l];
+  const i64 n3 = cube(n);
+  double* sc = H.scal + 3 * l;
+  if (n == 4) {
+    const int nc = S.coarse();
+    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, 1, (T*)nullptr, sc + 0);
+    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, 1, (T*)nullptr, sc + 1);
+    reduce<T>(c, H.u[l], n3, 0, sc + 2);
+    k_zerosum_guard<T><<<1, 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, n == ng_top ? 0 : 2, H.guard + l);
+    LAUNCHCHK();
+    return;
+  }
+  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.pre(n), 1, H.rhs[l + 1], sc + 0);
+  fill<T>(c, H.u[l + 1], T(0), cube(n / 2));
+  vcycle(c, H, l + 1, ng_top, S);
+  k_prolong<T><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], H.u[l + 1], n, 1);
+  LAUNCHCHK();
+  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.post(n), 1, (T*)nullptr, sc + 1);
+  reduce<T>(c, H.u[l], n3, 0, sc + 2);
+  k_zerosum_guard<T><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, n == ng_top ? 0 : 1, H.guard + l);
+  LAUNCHCHK();
+}
+
+// rgzerosum (gauss1.f90:1219-1417) on the device.
+template <class T> void rgzerosum_dev(mmh_ctx* c, T* rhs, const T* def, T dx, int n, double* sums) {
+  const i64 n3 = cube(n);
+  T* det = wsT<T>(c, "det", n3);
+  k_det<T><<<nblk(n3), 256, 0, c->stream>>>(det, def, n, dx);
+  LAUNCHCHK();
+  reduce<T>(c, det, n3, 0, sums + 0);
+  reduce<T>(c, rhs, n3, 0, sums + 1);
+  k_rgz_update<T><<<nblk(n3), 256, 0, c->stream>>>(rhs, det, sums, n3);
+  LAUNCHCHK();
+}
+
+template <class T> void fetch_stats(mmh_ctx* c, Hier<T>& H, mmh_mg_stats* st) {
+  if (!st) return;
+  std::vector<double> sc(3 * MMH_MAXLEV);
+  std::vector<int> g(MMH_MAXLEV + 1);
+  HIPCHK(hipMemcpyAsync(sc.data(), H.scal, sizeof(double) * 3 * H.nlev, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(g.data(), H.guard, sizeof(int) * (MMH_MAXLEV + 1), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::memset(st, 0, sizeof(*st));
+  st->nlev = H.nlev;
+  for (int l = 0; l < H.nlev; ++l) {
+    st->n[l] = H.n[l];
+    st->r1[l] = sc[3 * l];
+    st->relaxerr[l] = sc[3 * l + 1];
+    st->guard[l] = g[l];
+  }
+  st->entry_zerosum = g[MMH_MAXLEV];
+}
+
+// multigrid entry (multigrid.f90:16-90).
+template <class T>
+void multigrid_dev(mmh_ctx* c, T* u, T* rhs, const T* defpot, const T* ubig, T dx, int n, int nu, int iopt, int ng,
+                   mmh_mg_stats* st) {
+  require(is_pow2(n) && n <= 1024, "multigrid: n must be a power of two in [4, 1024]");
+  if (ng <= 0) ng = n;
+  require(ng >= n && is_pow2(ng), "multigrid: ng must be a power of two >= n");
+  require(nu >= 1, "multigrid: nu >= 1");
+  const int lsquare = (iopt % 8) == 1;
+  Hier<T> H;
+  build_hier(c, H, "mg", u, rhs, defpot, ubig, nu, dx, n, lsquare);
+  double* sums = H.scal + 3 * MMH_MAXLEV;
+  rgzerosum_dev(c, rhs, defpot, dx, n, sums);
+  reduce<T>(c, u, cube(n), 0, sums + 2);
+  k_entry_zerosum<T><<<nblk(cube(n)), 256, 0, c->stream>>>(u, sums + 2, cube(n), H.guard + MMH_MAXLEV);
+  LAUNCHCHK();
+  Sched S;
+  S.kind = 0;
+  S.ng = ng;
+  vcycle(c, H, 0, ng, S);
+  fetch_stats(c, H, st);
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+template <class T>
+void relax_api(mmh_ctx* c, T* arr, T* resid, const T* rhs, const T* def, const T* u, int nu, T* rinf, T dx, int n, int nrelax,
+               int iopt) {
+  require(is_pow2(n) && n <= 1024, "relax: n must be a power of two in [4, 1024]");
+  require(iopt >= 0 && iopt <= 256, "relax: iopt out of range");
+  const i64 n3 = cube(n);
+  const int ires = iopt % 8;
+  const int lsquare = ((iopt / 8) % 8) == 1;
+  T* G = wsT<T>(c, "relaxG", 6 * n3);
+  k_metric<T><<<nblk(n3), 256, 0, c->stream>>>(G, def, u, nu, n, dx, lsquare);
+  LAUNCHCHK();
+  double* d = (double*)ws_get(c, "relaxrinf", sizeof(double));
+  T* rs = wsT<T>(c, "relaxr", n3);
+  const bool want_res = ires == 1 || ires == 3;
+  relax_dev(c, arr, rhs, G, rs, n, dx, nrelax, ires != 0, want_res ? resid : (T*)nullptr, d);
+  double h = 0;
+  if (ires == 2 || ires == 3) HIPCHK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (rinf) *rinf = (T)h;
+}
+
+// ------------------------------------------------------------------------------------------------
+// FFT driver
+template <class T> const C2<T>* twiddle_table(mmh_ctx* c, int L, int s) {
+  const std::string key = "tw" + std::to_string(L) + (s > 0 ? "p" : "m");
+  C2<T>* d = wsT<C2<T>>(c, key, std::max(1, L / 2));
+  const std::string ukey = key + ":" + std::to_string(sizeof(T));
+  if (!c->uploaded.count(ukey)) {
+    c->uploaded.insert(ukey);
+    std::vector<C2<T>> h(std::max(1, L / 2));
+    for (int q = 0; q < L / 2; ++q) {
+      double ang = 2.0 * M_PI * (double)q / (double)L;
+      h[q].r = (T)cos(ang);
+      h[q].i = (T)((double)s * sin(ang));
+    }
+    HIPCHK(hipMemcpyAsync(d, h.data(), sizeof(C2<T>) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return d;
+}
+
+template <class T> void fft3_dev(mmh_ctx* c, C2<T>* z, int d0, int d1, int d2, int s) {
+  const int dims[3] = {d0, d1, d2};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int L = dims[ax];
+    if (L == 1) continue;
+    const C2<T>* W = twiddle_table<T>(c, L, s);
+    const i64 nl = (i64)d0 * d1 * d2 / L;
+    const int thr = std::max(64, std::min(256, L / 2));
+    k_fft_lines<T><<<(unsigned)nl, thr, 2 * L * sizeof(C2<T>), c->stream>>>(z, W, L, d0, d1, ax);
+    LAUNCHCHK();
+  }
+}
+
+template <class T> __global__ void k_get_speq(C2<T>* sq, const C2<T>* z, int h1, i64 m) {
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < m) sq[q] = z[(i64)h1 * q];
+}
+
+template <class T> void rlft3_dev(mmh_ctx* c, T* data, T* speq, int n, int isign) {
+  C2<T>* z = reinterpret_cast<C2<T>*>(data);
+  C2<T>* sq = reinterpret_cast<C2<T>*>(speq);
+  const int h1 = n / 2;
+  const i64 m = (i64)n * n;
+  if (isign == 1) {
+    fft3_dev<T>(c, z, h1, n, n, 1);
+    k_get_speq<T><<<nblk(m), 256, 0, c->stream>>>(sq, z, h1, m);
+    LAUNCHCHK();
+  }
+  C2<T>* sold = wsT<C2<T>>(c, "rlft_sold", m);
+  HIPCHK(hipMemcpyAsync(sold, sq, sizeof(C2<T>) * m, hipMemcpyDeviceToDevice, c->stream));
+  const T c2i = T(-0.5) * T(isign);
+  k_rlft_k0<T><<<nblk(m), 256, 0, c->stream>>>(z, sq, sold, h1, n, n, c2i);
+  LAUNCHCHK();
+  const int npair = (h1 - 1) / 2;
+  if (npair > 0) {
+    std::vector<C2<T>> w(h1);
+    for (int i1 = 0; i1 < h1; ++i1) {
+      double ang = 2.0 * M_PI * (double)i1 / (double)n;
+      w[i1].r = (T)cos(ang);
+      w[i1].i = (T)((double)isign * sin(ang));
+    }
+    C2<T>* wd = wsT<C2<T>>(c, std::string("rlft_w") + (isign > 0 ? "p" : "m") + std::to_string(n), h1);
+    HIPCHK(hipMemcpyAsync(wd, w.data(), sizeof(C2<T>) * h1, hipMemcpyHostToDevice, c->stream));
+    k_rlft_pairs<T><<<nblk((i64)npair * m), 256, 0, c->stream>>>(z, wd, h1, n, n, c2i);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (isign == -1) fft3_dev<T>(c, z, h1, n, n, -1);
+}
+
+// Spectral multiplier pass on (data, speq); mode as in k_spectral.
+template <class T> void spectral(mmh_ctx* c, T* data, T* speq, int n, int mode) {
+  const T pi2 = T(3.14159265358979323846264338328);      // limiter.f90:59
+  const T piG = (T)(atan(1.0) * 4.0);                   // gradient.f90:147, reconstruct.f90:220
+  T* cs = wsT<T>(c, "spec_cos" + std::to_string(n), 3 * (i64)n);
+  if (mode == 0) {
+    std::vector<T> h(3 * (size_t)n);
+    for (int ii = 0; ii <= n / 2; ++ii) h[ii] = (T)std::cos((T)(T(2) * pi2 * T(ii) / T(n)));
+    for (int j = 0; j < n; ++j) h[n + j] = (T)std::cos((T)(T(2) * pi2 * (T(j + 1) - T(1.)) / T(n)));
+    for (int k = 0; k < n; ++k) h[2 * n + k] = h[n + k];
+    HIPCHK(hipMemcpyAsync(cs, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  const i64 cnt = (i64)(n / 2 + 1) * n * n;
+  k_spectral<T><<<nblk(cnt), 256, 0, c->stream>>>(data, speq, n, mode, cs, cs + n, cs + 2 * n, piG);
+  LAUNCHCHK();
+}
+
+template <class T> void nsmooth_dev(mmh_ctx* c, T* a, int n, int times) {
+  const i64 n3 = cube(n);
+  T* t = wsT<T>(c, "smooth_tmp", n3);
+  for (int r = 0; r < times; ++r) {
+    k_smooth27<T><<<nblk(n3), 256, 0, c->stream>>>(t, a, n);
+    HIPCHK(hipMemcpyAsync(a, t, sizeof(T) * n3, hipMemcpyDeviceToDevice, c->stream));
+  }
+  LAUNCHCHK();
+}
+
+template <class T> __global__ void k_pad(T* __restrict__ dst, const T* __restrict__ src, int n, i64 n3) {
+  // (n x n x n) -> leading dimension n+2 (defplim / tmp2 layout, limiter.f90:43)
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n3) return;
+  const i64 i = p % n, r = p / n;
+  dst[i + (i64)(n + 2) * r] = src[p];
+}
+template <class T> __global__ void k_pad_speq(T* __restrict__ dst, const T* __restrict__ sq, int n) {
+  const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (i64)n * n) return;
+  dst[(i64)n + (i64)(n + 2) * r] = sq[2 * r];
+  dst[(i64)n + 1 + (i64)(n + 2) * r] = sq[2 * r + 1];
+}
+
+// Steps 1-3 of calcdefp: the right-hand side handed to multigrid (limiter.f90:134-392).  hb
+// receives the per-block [nexpand, detmax, detmin, fluxmax, fluxmin] (valid after a sync).
+template <class T>
+void limiter_rhs_dev(mmh_ctx* c, T* tmp, const T* def, const T* u, T dtold, T dtaumesh, int nf, int n, T compressmax,
+                     std::vector<double>& hb) {
+  const T pi = T(3.14159265358979323846264338328);
+  const i64 n3 = cube(n);
+  k_mismatch<T><<<nblk(n3), 256, 0, c->stream>>>(tmp, u, nf, n3, dtold);
+  LAUNCHCHK();
+  nsmooth_dev(c, tmp, n, 3);
+  T* B = wsT<T>(c, "limB", 7 * n3);
+  k_binv<T><<<nblk(n3), 256, 0, c->stream>>>(B, def, n);
+  T* ef = wsT<T>(c, "limE", n3);
+  const unsigned nb = nblk(n3);
+  double* bst = (double*)ws_get(c, "limstat", sizeof(double) * 5 * nb);
+  k_eflux<T><<<nb, 256, 0, c->stream>>>(ef, tmp, B, def, u, nf, n, dtold, dtaumesh, compressmax, pi, bst);
+  LAUNCHCHK();
+  hb.assign(5 * (size_t)nb, 0.0);
+  HIPCHK(hipMemcpyAsync(hb.data(), bst, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(tmp, ef, sizeof(T) * n3, hipMemcpyDeviceToDevice, c->stream));
+  nsmooth_dev(c, tmp, n, 1);
+}
+
+// calcdefp (limiter.f90:17-673) on the device.
+template <class T>
+void calcdefp_dev(mmh_ctx* c, T* defp, T* tmp, T* tmp2, const T* def, const T* u, T dtold, T dtaumesh, int nf, int n, T compressmax,
+                  int ng, mmh_lim_stats* st) {
+  require(is_pow2(n) && n >= 16 && n <= 1024, "calcdefp: n must be a power of two in [16, 1024]");
+  require(nf >= 4, "calcdefp: nfluid must be >= 4 (u(2:4) are read)");
+  require(compressmax >= T(0), "calcdefp: compressmax < 0 selects the reference's diagnostic-only branch");
+  if (dtold < T(1.e-8)) dtold = 1;
+  if (ng <= 0) ng = n;
+  const T pi = T(3.14159265358979323846264338328);
+  const i64 n3 = cube(n);
+  std::vector<double> hb;
+  limiter_rhs_dev<T>(c, tmp, def, u, dtold, dtaumesh, nf, n, compressmax, hb);
+  fill<T>(c, defp, T(0), n3);
+  mmh_mg_stats* mst = st ? &st->mg : nullptr;
+  multigrid_dev<T>(c, defp, tmp, def, u, T(1.), n, nf, 2, ng, mst);
+  nsmooth_dev(c, defp, n, 2);
+  const unsigned nb = nblk(n3);
+  T* lim = wsT<T>(c, "limL", n3);
+  LAUNCHCHK();
+  double* bst2 = (double*)ws_get(c, "limstat2", sizeof(double) * 2 * nb);
+  k_complim<T><<<nb, 256, 0, c->stream>>>(lim, defp, def, n, dtold, compressmax, pi, bst2);
+  LAUNCHCHK();
+  std::vector<double> hb2(2 * (size_t)nb);
+  HIPCHK(hipMemcpyAsync(hb2.data(), bst2, sizeof(double) * hb2.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  long ncompress = 0;
+  double cmax = 0;
+  for (unsigned b = 0; b < nb; ++b) { ncompress += (long)hb2[2 * b]; cmax = std::max(cmax, hb2[2 * b + 1]); }
+  k_pad<T><<<nblk(n3), 256, 0, c->stream>>>(tmp2, lim, n, n3);
+  LAUNCHCHK();
+  if (ncompress > 0) {
+    T* sq = wsT<T>(c, "limS", 2 * (i64)n * n);
+    fill<T>(c, sq, T(0), 2 * (i64)n * n);
+    rlft3_dev<T>(c, lim, sq, n, 1);
+    spectral<T>(c, lim, sq, n, 0);
+    fill<T>(c, lim, T(0), 1);
+    rlft3_dev<T>(c, lim, sq, n, -1);
+    k_axpy_scale<T><<<nblk(n3), 256, 0, c->stream>>>(defp, lim, n, n3);
+    k_scale2<T><<<nblk(n3), 256, 0, c->stream>>>(lim, lim, n, n3);
+    k_pad<T><<<nblk(n3), 256, 0, c->stream>>>(tmp2, lim, n, n3);
+    k_pad_speq<T><<<nblk((i64)n * n), 256, 0, c->stream>>>(tmp2, sq, n);
+    LAUNCHCHK();
+  }
+  // exit convention of tmp (limiter.f90:621-628,669-671): all cells "hot" -> 1; diagnostics 0
+  fill<T>(c, tmp, T(1), n3);
+  fill<T>(c, tmp, T(0), 3);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (st) {
+    double detmax = 0, detmin = 1, fmaxv = 0, fminv = 0;
+    long nexp = 0;
+    for (unsigned b = 0; b < nb; ++b) {
+      nexp += (long)hb[5 * b];
+      detmax = std::max(detmax, hb[5 * b + 1]);
+      detmin = std::min(detmin, hb[5 * b + 2]);
+      fmaxv = std::max(fmaxv, hb[5 * b + 3]);
+      fminv = std::min(fminv, hb[5 * b + 4]);
+    }
+    st->cmax = cmax; st->detmin = detmin; st->detmax = detmax; st->fluxmin = fminv; st->fluxmax = fmaxv;
+    st->nexpand = (int)nexp; st->ncompress = (int)ncompress;
+  }
+}
+
+template <class T> void relaxing_dev(mmh_ctx* c, T* u, const T* def, const T* defp, T dt, int nf, int n) {
+  require(is_pow2(n) && n >= 16 && n <= 1024, "relaxing: n must be a power of two in [16, 1024]");
+  require(nf == 5, "relaxing: only nfluid = 5 is supported (relaxing.f90:307-310)");
+  const i64 n3 = cube(n);
+  T* B = wsT<T>(c, "tvdB", 7 * n3);
+  T* cc = wsT<T>(c, "tvdC", n3);
+  T* V = wsT<T>(c, "tvdV", 15 * n3);
+  T* u1 = wsT<T>(c, "tvdU1", 5 * n3);
+  const unsigned nb = nblk(n3);
+  k_binv<T><<<nb, 256, 0, c->stream>>>(B, def, n);
+  k_cfl<T><<<nb, 256, 0, c->stream>>>(cc, B, defp, u, n);
+  k_tvd_flux<T><<<nb, 256, 0, c->stream>>>(V, u, defp, B, cc, n);
+  k_tvd_step<T><<<nb, 256, 0, c->stream>>>(u1, u, u, V, cc, n, dt / T(2));
+  k_tvd_flux<T><<<nb, 256, 0, c->stream>>>(V, u1, defp, B, cc, n);
+  k_tvd_step<T><<<nb, 256, 0, c->stream>>>(u, u, u1, V, cc, n, dt);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+template <class T> void spectral_out(mmh_ctx* c, const T* phi, T* out, int n, int mode) {
+  const i64 n3 = cube(n);
+  T* d = wsT<T>(c, "specD", n3);
+  T* sq = wsT<T>(c, "specS", 2 * (i64)n * n);
+  HIPCHK(hipMemcpyAsync(d, phi, sizeof(T) * n3, hipMemcpyDeviceToDevice, c->stream));
+  fill<T>(c, sq, T(0), 2 * (i64)n * n);
+  rlft3_dev<T>(c, d, sq, n, 1);
+  spectral<T>(c, d, sq, n, mode);
+  rlft3_dev<T>(c, d, sq, n, -1);
+  k_scale2<T><<<nblk(n3), 256, 0, c->stream>>>(out, d, n, n3);
+  LAUNCHCHK();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Lognormal generator kernels
+__device__ __forceinline__ uint64_t smix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__global__ void k_white(C2<double>* z, i64 n3, uint64_t key) {
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n3) return;
+  uint64_t h1 = smix(key + 2 * (uint64_t)q), h2 = smix(key + 2 * (uint64_t)q + 1);
+  double u1 = ((double)(h1 >> 11) + 1.0) * 0x1.0p-53;
+  double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+  z[q].r = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+  z[q].i = 0.0;
+}
+__global__ void k_pk(C2<double>* z, int n, double slope, double ks) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= cube(n)) return;
+  const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
+  double mi = i <= n / 2 ? i : i - n, mj = j <= n / 2 ? j : j - n, mk = k <= n / 2 ? k : k - n;
+  double kx = 2.0 * M_PI * mi / n, ky = 2.0 * M_PI * mj / n, kz = 2.0 * M_PI * mk / n;
+  double kk = sqrt(kx * kx + ky * ky + kz * kz);
+  double a = kk > 0 ? sqrt(pow(kk, slope) * exp(-(kk / ks) * (kk / ks))) : 0.0;
+  z[p].r = z[p].r * a;
+  z[p].i = z[p].i * a;
+}
+__global__ void k_re_sq(double* g, double* g2, const C2<double>* z, i64 n3) {
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n3) return;
+  g[q] = z[q].r;
+  g2[q] = z[q].r * z[q].r;
+}
+__global__ void k_expmap(double* g, const double* sums, i64 n3, double sigma) {
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n3) return;
+  const double var = sums[0] / (double)n3;
+  const double sc = sigma / sqrt(var);
+  g[q] = exp(g[q] * sc - 0.5 * sigma * sigma);
+}
+__global__ void k_divmean(double* rho, const double* g, const double* sums, i64 n3) {
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n3) return;
+  rho[q] = g[q] / (sums[1] / (double)n3);
+}
+
+uint64_t smix_h(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+void lognormal_dev(mmh_ctx* c, double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed) {
+  require(is_pow2(n) && n <= 2048, "lognormal: n must be a power of two");
+  const i64 n3 = cube(n);
+  C2<double>* z = wsT<C2<double>>(c, "gen_z", n3);
+  double* g2 = wsT<double>(c, "gen_g2", n3);
+  double* sums = (double*)ws_get(c, "gen_sums", 2 * sizeof(double));
+  k_white<<<nblk(n3), 256, 0, c->stream>>>(z, n3, smix_h(seed ^ 0x6A09E667F3BCC909ULL));
+  LAUNCHCHK();
+  fft3_dev<double>(c, z, n, n, n, -1);
+  k_pk<<<nblk(n3), 256, 0, c->stream>>>(z, n, slope, ks_frac * M_PI);
+  LAUNCHCHK();
+  fft3_dev<double>(c, z, n, n, n, 1);
+  k_re_sq<<<nblk(n3), 256, 0, c->stream>>>(rho, g2, z, n3);
+  LAUNCHCHK();
+  reduce<double>(c, g2, n3, 0, sums);
+  k_expmap<<<nblk(n3), 256, 0, c->stream>>>(rho, sums, n3, sigma);
+  LAUNCHCHK();
+  reduce<double>(c, rho, n3, 0, sums + 1);
+  k_divmean<<<nblk(n3), 256, 0, c->stream>>>(rho, rho, sums, n3);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// ------------------------------------------------------------------------------------------------
+// Solve to tolerance.  FP64: repeated V-cycles, the relative residual read after each.  Mixed:
+// r = rhs - L u in FP64; an FP32 V-cycle solves L e = r from e = 0 (its 4^3 level in FP64); u += e.
+template <class S, class D> __global__ void k_convert(D* __restrict__ dst, const S* __restrict__ src, i64 n) {
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = (D)src[q];
+}
+__global__ void k_add_f2d(double* __restrict__ u, const float* __restrict__ e, i64 n) {
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) u[q] = u[q] + (double)e[q];
+}
+
+template <class T> double read_scalar(mmh_ctx* c, const double* d) {
+  double h = 0;
+  HIPCHK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return h;
+}
+
+// FP32 V-cycle whose 4^3 level runs in FP64 (the coarsest solve stays FP64).
+void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_top, const Sched& S) {
+  const int n = Hf.n[l];
+  if (n == 4) {
+    const i64 n3 = 64;
+    k_convert<float, double><<<1, 64, 0, c->stream>>>(Hd.rhs[l], Hf.rhs[l], n3);
+    fill<double>(c, Hd.u[l], 0.0, n3);
+    LAUNCHCHK();
+    vcycle<double>(c, Hd, l, ng_top, S);
+    k_convert<double, float><<<1, 64, 0, c->stream>>>(Hf.u[l], Hd.u[l], n3);
+    LAUNCHCHK();
+    return;
+  }
+  const i64 n3 = cube(n);
+  double* sc = Hf.scal + 3 * l;
+  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), 1, Hf.rhs[l + 1], sc + 0);
+  fill<float>(c, Hf.u[l + 1], 0.f, cube(n / 2));
+  vcycle_mixed(c, Hf, Hd, l + 1, ng_top, S);
+  k_prolong<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
+  LAUNCHCHK();
+  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), 1, (float*)nullptr, sc + 1);
+  reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
+  k_zerosum_guard<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], sc + 2, sc, n3, n == ng_top ? 0 : 1, Hf.guard + l);
+  LAUNCHCHK();
+}
+
+void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
+               const mmh_solve_opts* o, mmh_solve_stats* st) {
+  require(is_pow2(n) && n <= 1024, "solve: n must be a power of two in [4, 1024]");
+  require(o != nullptr, "solve: opts required");
+  require(o->max_cycles >= 1 && o->max_cycles <= 256, "solve: max_cycles in [1, 256]");
+  Sched S;
+  S.kind = o->schedule;
+  S.ng = o->ng > 0 ? o->ng : n;
+  S.nu_pre = o->nu_pre; S.nu_post = o->nu_post; S.nu_coarse = o->nu_coarse;
+  if (S.kind == 1) require(S.nu_pre >= 1 && S.nu_post >= 1 && S.nu_coarse >= 1, "solve: sweep counts >= 1");
+  const i64 n3 = cube(n);
+  Hier<double> H;
+  build_hier(c, H, "sd", u, rhs, defpot, ubig, nu, 1.0, n, 0);
+  double* sums = H.scal + 3 * MMH_MAXLEV;
+  rgzerosum_dev<double>(c, rhs, defpot, 1.0, n, sums);
+  reduce<double>(c, rhs, n3, 1, sums + 3);
+  const double rn = read_scalar<double>(c, sums + 3);
+  if (st) { std::memset(st, 0, sizeof(*st)); st->rhs_norm = rn; }
+  const double rden = rn > 0 ? rn : 1.0;
+  if (!o->mixed) {
+    for (int cyc = 0; cyc < o->max_cycles; ++cyc) {
+      vcycle<double>(c, H, 0, S.ng, S);
+      const double rr = read_scalar<double>(c, H.scal + 1) / rden;
+      if (st) { st->relres[cyc] = rr; st->cycles = cyc + 1; }
+      if (rr <= o->tol) return;
+    }
+    throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
+  }
+  // mixed: FP32 hierarchy with the FP64 metrics rounded; FP64 residual at the top.
+  Hier<float> F;
+  F.nlev = H.nlev;
+  F.scal = (double*)ws_get(c, "sfscal", sizeof(double) * 4 * MMH_MAXLEV);
+  F.guard = (int*)ws_get(c, "sfguard", sizeof(int) * (MMH_MAXLEV + 1));
+  for (int l = 0; l < H.nlev; ++l) {
+    const int m = H.n[l];
+    const std::string t = "sf" + std::to_string(l);
+    F.n[l] = m; F.dx[l] = (float)H.dx[l]; F.nu[l] = 1;
+    F.u[l] = wsT<float>(c, t + "u", cube(m));
+    F.rhs[l] = wsT<float>(c, t + "rhs", cube(m));
+    F.r[l] = wsT<float>(c, t + "r", cube(m));
+    F.G[l] = wsT<float>(c, t + "G", 6 * cube(m));
+    F.def[l] = nullptr; F.u1[l] = nullptr;
+    k_convert<double, float><<<nblk(6 * cube(m)), 256, 0, c->stream>>>(F.G[l], H.G[l], 6 * cube(m));
+    LAUNCHCHK();
+  }
+  double* r64 = H.r[0];
+  const unsigned nb = nblk(n3, RB);
+  double* bmax = (double*)ws_get(c, "sbmax", sizeof(double) * nb);
+  for (int cyc = 0; cyc < o->max_cycles; ++cyc) {
+    k_resid<double><<<nb, RB, 0, c->stream>>>(r64, bmax, u, rhs, H.G[0], n, 1.0);
+    LAUNCHCHK();
+    k_convert<double, float><<<nblk(n3), 256, 0, c->stream>>>(F.rhs[0], r64, n3);
+    fill<float>(c, F.u[0], 0.f, n3);
+    LAUNCHCHK();
+    vcycle_mixed(c, F, H, 0, S.ng, S);
+    k_add_f2d<<<nblk(n3), 256, 0, c->stream>>>(u, F.u[0], n3);
+    LAUNCHCHK();
+    k_resid<double><<<nb, RB, 0, c->stream>>>(nullptr, bmax, u, rhs, H.G[0], n, 1.0);
+    LAUNCHCHK();
+    reduce<double>(c, bmax, nb, 1, sums + 3);
+    const double rr = read_scalar<double>(c, sums + 3) / rden;
+    if (st) { st->relres[cyc] = rr; st->cycles = cyc + 1; }
+    if (rr <= o->tol) {
+      // remove the mean as the FP64 path's top-level zerosum does
+      reduce<double>(c, u, n3, 0, sums + 2);
+      k_zerosum_guard<double><<<nblk(n3), 256, 0, c->stream>>>(u, sums + 2, H.scal, n3, 0, nullptr);
+      LAUNCHCHK();
+      HIPCHK(hipStreamSynchronize(c->stream));
+      return;
+    }
+  }
+  throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
+}
+
+template <class T> void time_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, float* ms) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  sweeps(c, phi, rhs, G, n, dx, ns);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / std::max(1, ns);
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+}
+
+// Host-pointer wrapper: copies inputs to context buffers, runs f, copies outputs back.
+struct HostBuf {
+  mmh_ctx* c;
+  void* d;
+  void* h;
+  size_t bytes;
+  bool out;
+};
+
+template <class F> int guarded(mmh_ctx* c, F&& f) {
+  if (!c) return MMH_EINVAL;
+  try {
+    HIPCHK(hipSetDevice(c->device));
+    f();
+    c->err.clear();
+    return MMH_OK;
+  } catch (const MmhError& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return MMH_EDEVICE;
+  }
+}
+
+template <class T> T* up(mmh_ctx* c, const std::string& key, const T* h, i64 count) {
+  T* d = wsT<T>(c, "host_" + key, count);
+  if (h) HIPCHK(hipMemcpyAsync(d, h, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+  return d;
+}
+template <class T> void down(mmh_ctx* c, T* h, const T* d, i64 count) {
+  HIPCHK(hipMemcpyAsync(h, d, sizeof(T) * count, hipMemcpyDeviceToHost, c->stream));
+}
+
+std::mutex g_default_mu;
+mmh_ctx* g_default = nullptr;
+
+}  // namespace
+
+// ================================================================================================
+extern "C" {
+
+int mmh_create(mmh_ctx** out, int device) {
+  if (!out) return MMH_EINVAL;
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0 || device < 0 || device >= nd) return MMH_EDEVICE;
+  mmh_ctx* c = new mmh_ctx();
+  c->device = device;
+  *out = c;
+  return MMH_OK;
+}
+
+int mmh_release_workspace(mmh_ctx* c) {
+  return guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& kv : c->ws) HIPCHK(hipFree(kv.second.first));
+    c->ws.clear();
+    c->uploaded.clear();
+  });
+}
+
+int mmh_destroy(mmh_ctx* c) {
+  if (!c) return MMH_EINVAL;
+  int rc = mmh_release_workspace(c);
+  delete c;
+  return rc;
+}
+
+const char* mmh_error_string(mmh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mmh_set_stream(mmh_ctx* c, void* s) {
+  return guarded(c, [&] { c->stream = (hipStream_t)s; });
+}
+
+mmh_ctx* mmh_default_ctx(void) {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default) {
+    if (mmh_create(&g_default, 0) != MMH_OK) g_default = nullptr;
+  }
+  return g_default;
+}
+
+#define DEF_DEV(T, S)                                                                                              \
+  int mmh_relax_##S(mmh_ctx* c, T* arr, T* resid, const T* rhs, const T* dp, const T* u, int nu, T* rinf, T dx,  \
+                    int n, int nrelax, int iopt) {                                                                 \
+    return guarded(c, [&] { relax_api<T>(c, arr, resid, rhs, dp, u, nu, rinf, dx, n, nrelax, iopt); });             \
+  }                                                                                                                \
+  int mmh_multigrid_##S(mmh_ctx* c, T* u, T* rhs, const T* dp, const T* ub, T dx, int n, int nu, int iopt, int ng, \
+                        mmh_mg_stats* st) {                                                                        \
+    return guarded(c, [&] { multigrid_dev<T>(c, u, rhs, dp, ub, dx, n, nu, iopt, ng, st); });                       \
+  }                                                                                                                \
+  int mmh_calcdefp_##S(mmh_ctx* c, T* defp, T* tmp, T* tmp2, const T* def, const T* u, T dtold, T dtaumesh,        \
+                       int nf, int n, T cmp, int ng, mmh_lim_stats* st) {                                          \
+    return guarded(c, [&] {                                                                                        \
+      calcdefp_dev<T>(c, defp, tmp, tmp2, def, u, dtold, dtaumesh, nf, n, cmp, ng, st);                            \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_relaxing_##S(mmh_ctx* c, T* u, const T* def, const T* defp, T dt, int nf, int n) {                       \
+    return guarded(c, [&] { relaxing_dev<T>(c, u, def, defp, dt, nf, n); });                                        \
+  }                                                                                                                \
+  int mmh_gradient_##S(mmh_ctx* c, const T* phi, T* psi, int n) {                                                  \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n) && n >= 4, "gradient: n must be a power of two >= 4");                                    \
+      for (int d = 0; d < 3; ++d) spectral_out<T>(c, phi, psi + (i64)d * cube(n), n, 2 + d);                       \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_delta_##S(mmh_ctx* c, const T* phi, T* delta, int n) {                                                   \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n) && n >= 4, "delta: n must be a power of two >= 4");                                       \
+      spectral_out<T>(c, phi, delta, n, 1);                                                                        \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_nsmooth_##S(mmh_ctx* c, T* a, int n) {                                                                   \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n), "nsmooth: n must be a power of two >= 4");                                               \
+      nsmooth_dev<T>(c, a, n, 1);                                                                                  \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_restrict_##S(mmh_ctx* c, T* half, const T* full, int n, int stride) {                                    \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n) && stride >= 1, "restrict: bad size");                                                    \
+      k_restrict27<T><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(half, full, n, stride);                            \
+      LAUNCHCHK();                                                                                                 \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_inject_##S(mmh_ctx* c, T* u, const T* uh, int n) {                                                       \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n), "inject: bad size");                                                                     \
+      k_prolong<T><<<nblk(cube(n)), 256, 0, c->stream>>>(u, uh, n, 1);                                             \
+      LAUNCHCHK();                                                                                                 \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_metric_##S(mmh_ctx* c, T* G, const T* def, const T* u, int nu, int n, T dx, int lsq) {                   \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n) && nu >= 1, "metric: bad size");                                                          \
+      k_metric<T><<<nblk(cube(n)), 256, 0, c->stream>>>(G, def, u, nu, n, dx, lsq);                               \
+      LAUNCHCHK();                                                                                                 \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_time_sweeps_##S(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, float* ms) {          \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n) && ns >= 1 && ms, "time_sweeps: bad arguments");                                          \
+      time_sweeps<T>(c, phi, rhs, G, n, dx, ns, ms);                                                               \
+    });                                                                                                            \
+  }
+
+DEF_DEV(double, d)
+DEF_DEV(float, f)
+
+int mmh_solve_d(mmh_ctx* c, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
+                const mmh_solve_opts* opts, mmh_solve_stats* st) {
+  return guarded(c, [&] { solve_dev(c, u, rhs, defpot, ubig, n, nu, opts, st); });
+}
+
+int mmh_limiter_rhs_d(mmh_ctx* c, double* tmp, const double* def, const double* u, double dtold, double dtaumesh, int nf,
+                      int n, double compressmax) {
+  return guarded(c, [&] {
+    require(is_pow2(n) && n >= 16 && n <= 1024 && nf >= 4, "limiter_rhs: bad size");
+    require(compressmax >= 0, "limiter_rhs: compressmax < 0");
+    std::vector<double> hb;
+    limiter_rhs_dev<double>(c, tmp, def, u, dtold < 1.e-8 ? 1.0 : dtold, dtaumesh, nf, n, compressmax, hb);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_rlft3_d(mmh_ctx* c, double* data, double* speq, int n, int isign) {
+  return guarded(c, [&] {
+    require(is_pow2(n) && n >= 4 && (isign == 1 || isign == -1), "rlft3: bad arguments");
+    rlft3_dev<double>(c, data, speq, n, isign);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_lognormal_d(mmh_ctx* c, double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed) {
+  return guarded(c, [&] { lognormal_dev(c, rho, n, sigma, slope, ks_frac, seed); });
+}
+
+// ---- host-pointer entry points -------------------------------------------------------------
+#define DEF_HOST(T, S)                                                                                             \
+  int mmh_relax_host_##S(mmh_ctx* c, T* arr, T* resid, const T* rhs, const T* dp, const T* u, int nu, T* rinf,     \
+                         T dx, int n, int nrelax, int iopt) {                                                      \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n) && nu >= 1, "relax: bad size");                                                           \
+      const i64 n3 = cube(n), h3 = cube(n / 2);                                                                    \
+      T* a = up(c, "a", arr, n3);                                                                                  \
+      T* r = up(c, "r", resid, h3);                                                                                \
+      T* b = up(c, "b", rhs, n3);                                                                                  \
+      T* d = up(c, "d", dp, n3);                                                                                   \
+      T* w = up(c, "u", u, (i64)nu * n3);                                                                          \
+      relax_api<T>(c, a, r, b, d, w, nu, rinf, dx, n, nrelax, iopt);                                               \
+      down(c, arr, a, n3);                                                                                         \
+      if (resid) down(c, resid, r, h3);                                                                            \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_multigrid_host_##S(mmh_ctx* c, T* u, T* rhs, const T* dp, const T* ub, T dx, int n, int nu, int iopt,    \
+                             int ng) {                                                                             \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n) && nu >= 1, "multigrid: bad size");                                                       \
+      const i64 n3 = cube(n);                                                                                      \
+      T* a = up(c, "a", u, n3);                                                                                    \
+      T* b = up(c, "b", rhs, n3);                                                                                  \
+      T* d = up(c, "d", dp, n3);                                                                                   \
+      T* w = up(c, "u", ub, (i64)nu * n3);                                                                         \
+      multigrid_dev<T>(c, a, b, d, w, dx, n, nu, iopt, ng, nullptr);                                               \
+      down(c, u, a, n3);                                                                                           \
+      down(c, rhs, b, n3);                                                                                         \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_calcdefp_host_##S(mmh_ctx* c, T* defp, T* tmp, T* tmp2, const T* def, const T* u, T dtold, T dtaumesh,  \
+                            int nf, int n, T cmp) {                                                                \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n) && nf >= 4, "calcdefp: bad size");                                                        \
+      const i64 n3 = cube(n), p3 = (i64)(n + 2) * n * n;                                                           \
+      T* p = up<T>(c, "a", nullptr, n3);                                                                           \
+      T* t = up<T>(c, "b", nullptr, n3);                                                                           \
+      T* t2 = up<T>(c, "t2", nullptr, p3);                                                                         \
+      T* d = up(c, "d", def, n3);                                                                                  \
+      T* w = up(c, "u", u, (i64)nf * n3);                                                                          \
+      calcdefp_dev<T>(c, p, t, t2, d, w, dtold, dtaumesh, nf, n, cmp, n, nullptr);                                 \
+      down(c, defp, p, n3);                                                                                        \
+      down(c, tmp, t, n3);                                                                                         \
+      down(c, tmp2, t2, p3);                                                                                       \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }                                                                                                                \
+  int mmh_relaxing_host_##S(mmh_ctx* c, T* u, const T* def, const T* defp, T dt, int nf, int n) {                  \
+    return guarded(c, [&] {                                                                                        \
+      require(is_pow2(n) && nf == 5, "relaxing: bad size");                                                        \
+      const i64 n3 = cube(n);                                                                                      \
+      T* w = up(c, "u", u, 5 * n3);                                                                                \
+      T* d = up(c, "d", def, n3);                                                                                  \
+      T* p = up(c, "a", defp, n3);                                                                                 \
+      relaxing_dev<T>(c, w, d, p, dt, nf, n);                                                                      \
+      down(c, u, w, 5 * n3);                                                                                       \
+      HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
+    });                                                                                                            \
+  }
+
+DEF_HOST(double, d)
+DEF_HOST(float, f)
+
+int mmh_gradient_host_f(mmh_ctx* c, const float* phi, float* psi, int n) {
+  return guarded(c, [&] {
+    require(is_pow2(n) && n >= 4, "gradient: bad size");
+    const i64 n3 = cube(n);
+    float* a = up(c, "a", phi, n3);
+    float* o = up<float>(c, "g3", nullptr, 3 * n3);
+    for (int d = 0; d < 3; ++d) spectral_out<float>(c, a, o + (i64)d * n3, n, 2 + d);
+    down(c, psi, o, 3 * n3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_delta_host_f(mmh_ctx* c, const float* phi, float* delta, int n) {
+  return guarded(c, [&] {
+    require(is_pow2(n) && n >= 4, "delta: bad size");
+    const i64 n3 = cube(n);
+    float* a = up(c, "a", phi, n3);
+    float* o = up<float>(c, "b", nullptr, n3);
+    spectral_out<float>(c, a, o, n, 1);
+    down(c, delta, o, n3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+"""Python host layer of the MI355X moving-mesh reconstruction library.
+
+Loads the in-tree libmmh_amd.so (HIP, gfx950) and exposes the reference's operator interface on
+device arrays (torch tensors on a ROCm device, or raw device pointers):
+
+    relax(arr, resid, rhs, deformpot, u, dx, nrelax, iopt)      mmh/gauss1.f90:64
+    multigrid(u, rhs, defpot, ubig, dx, iopt, ng)                mmh/multigrid.f90:16
+    calcdefp(defp, tmp, tmp2, def_, u, dtold1, dtaumesh, nfluid) mmh/limiter.f90:17
+    relaxing(u, def_, defp, cmax, dt, nfluidcmp)                 mmh/relaxing.f90:21
+    scalar2gradient(indata, outdata)                             mmh/gradient.f90:1
+    defp2delta(indata, outdata)                                  mmh/reconstruct.f90:1
+    nsmooth(a)                                                   mmh/limiter.f90:677
+
+Arguments keep the reference's meaning and in/out behaviour (multigrid projects rhs in place,
+calcdefp overwrites tmp/tmp2).  Sizes come from the tensors instead of the compile-time NG.  Errors
+raise MmhError carrying the library's message; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_DIR, "libmmh_amd.so")
+FLIB_PATH = os.path.join(PKG_DIR, "libmmh_fortran.so")
+
+MMH_OK, MMH_EINVAL, MMH_EDEVICE, MMH_ENOMEM, MMH_ENOCONV = 0, 1, 2, 3, 4
+MAXLEV = 16
+
+
+class MmhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mmh error %d: %s" % (code, msg))
+        self.code = code
+
+
+class MgStats(C.Structure):
+    _fields_ = [("nlev", C.c_int), ("n", C.c_int * MAXLEV), ("r1", C.c_double * MAXLEV),
+                ("relaxerr", C.c_double * MAXLEV), ("guard", C.c_int * MAXLEV), ("entry_zerosum", C.c_int)]
+
+
+class LimStats(C.Structure):
+    _fields_ = [("cmax", C.c_double), ("detmin", C.c_double), ("detmax", C.c_double), ("fluxmin", C.c_double),
+                ("fluxmax", C.c_double), ("nexpand", C.c_int), ("ncompress", C.c_int), ("mg", MgStats)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_cycles", C.c_int), ("schedule", C.c_int), ("nu_pre", C.c_int),
+                ("nu_post", C.c_int), ("nu_coarse", C.c_int), ("ng", C.c_int), ("mixed", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("cycles", C.c_int), ("relres", C.c_double * 256), ("rhs_norm", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libmmh_amd.so.  Raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MmhError(MMH_EDEVICE, "libmmh_amd.so not built: run moving-mesh-reconstruction_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        L.mmh_error_string.restype = C.c_char_p
+        L.mmh_default_ctx.restype = C.c_void_p
+        _lib = L
+    return _lib
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor (or an int address)."""
+    if isinstance(x, int):
+        return C.c_void_p(x)
+    if x is None:
+        return C.c_void_p(0)
+    if not x.is_contiguous():
+        raise MmhError(MMH_EINVAL, "arrays must be contiguous")
+    return C.c_void_p(x.data_ptr())
+
+
+def _sfx(t):
+    import torch
+    if t.dtype == torch.float64:
+        return "d", C.c_double
+    if t.dtype == torch.float32:
+        return "f", C.c_float
+    raise MmhError(MMH_EINVAL, "dtype must be float64 or float32")
+
+
+def _cube_side(t, comps=1):
+    n3 = t.numel() // comps
+    n = round(n3 ** (1.0 / 3.0))
+    if n ** 3 * comps != t.numel():
+        raise MmhError(MMH_EINVAL, "array is not an N^3 field")
+    return n
+
+
+class Context:
+    """A library context bound to one device (one HIP stream, cached work buffers)."""
+
+    def __init__(self, device=0):
+        self._c = C.c_void_p()
+        rc = lib().mmh_create(C.byref(self._c), device)
+        if rc != MMH_OK:
+            raise MmhError(rc, "mmh_create failed (no HIP device %d?)" % device)
+
+    def close(self):
+        if self._c:
+            lib().mmh_destroy(self._c)
+            self._c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != MMH_OK:
+            raise MmhError(rc, lib().mmh_error_string(self._c).decode())
+
+    def call(self, name, *args):
+        self._check(getattr(lib(), name)(self._c, *args))
+
+    def set_stream(self, stream_ptr):
+        self.call("mmh_set_stream", C.c_void_p(stream_ptr))
+
+    def release_workspace(self):
+        self.call("mmh_release_workspace")
+
+    # ---- reference operator interface ------------------------------------------------------
+    def relax(self, arr, resid, rhs, deformpot, u, dx, nrelax, iopt, nu=None):
+        s, T = _sfx(arr)
+        n = _cube_side(arr)
+        nu = nu or u.numel() // arr.numel()
+        rinf = T(0)
+        self.call("mmh_relax_" + s, _ptr(arr), _ptr(resid), _ptr(rhs), _ptr(deformpot), _ptr(u), nu, C.byref(rinf),
+                  T(dx), n, nrelax, iopt)
+        return rinf.value
+
+    def multigrid(self, u, rhs, defpot, ubig, dx=1.0, iopt=2, ng=0, nu=None):
+        s, T = _sfx(u)
+        n = _cube_side(u)
+        nu = nu or ubig.numel() // u.numel()
+        st = MgStats()
+        self.call("mmh_multigrid_" + s, _ptr(u), _ptr(rhs), _ptr(defpot), _ptr(ubig), T(dx), n, nu, iopt, ng,
+                  C.byref(st))
+        return st
+
+    def solve(self, u, rhs, defpot, ubig, tol=1e-8, max_cycles=100, schedule=0, nu_pre=2, nu_post=2, nu_coarse=32,
+              ng=0, mixed=False, nu=None):
+        n = _cube_side(u)
+        nu = nu or ubig.numel() // u.numel()
+        o = SolveOpts(tol, max_cycles, schedule, nu_pre, nu_post, nu_coarse, ng, 1 if mixed else 0)
+        st = SolveStats()
+        rc = lib().mmh_solve_d(self._c, _ptr(u), _ptr(rhs), _ptr(defpot), _ptr(ubig), n, nu, C.byref(o), C.byref(st))
+        if rc not in (MMH_OK, MMH_ENOCONV):
+            self._check(rc)
+        return st, rc == MMH_OK
+
+    def calcdefp(self, defp, tmp, tmp2, def_, u, dtold1=1.0, dtaumesh=1.0, nfluid=5, compressmax=10.0, ng=0):
+        s, T = _sfx(defp)
+        n = _cube_side(defp)
+        st = LimStats()
+        self.call("mmh_calcdefp_" + s, _ptr(defp), _ptr(tmp), _ptr(tmp2), _ptr(def_), _ptr(u), T(dtold1), T(dtaumesh),
+                  nfluid, n, T(compressmax), ng, C.byref(st))
+        return st
+
+    def limiter_rhs(self, tmp, def_, u, dtold1=1.0, dtaumesh=1.0, nfluid=5, compressmax=10.0):
+        n = _cube_side(tmp)
+        self.call("mmh_limiter_rhs_d", _ptr(tmp), _ptr(def_), _ptr(u), C.c_double(dtold1), C.c_double(dtaumesh), nfluid,
+                  n, C.c_double(compressmax))
+
+    def relaxing(self, u, def_, defp, cmax=0.0, dt=1.0, nfluidcmp=5):
+        s, T = _sfx(u)
+        n = _cube_side(def_)
+        self.call("mmh_relaxing_" + s, _ptr(u), _ptr(def_), _ptr(defp), T(dt), nfluidcmp, n)
+
+    def scalar2gradient(self, indata, outdata):
+        s, _ = _sfx(indata)
+        self.call("mmh_gradient_" + s, _ptr(indata), _ptr(outdata), _cube_side(indata))
+
+    def defp2delta(self, indata, outdata):
+        s, _ = _sfx(indata)
+        self.call("mmh_delta_" + s, _ptr(indata), _ptr(outdata), _cube_side(indata))
+
+    def nsmooth(self, a):
+        s, _ = _sfx(a)
+        self.call("mmh_nsmooth_" + s, _ptr(a), _cube_side(a))
+
+    def restrict(self, half, full, stride=1):
+        s, _ = _sfx(full)
+        self.call("mmh_restrict_" + s, _ptr(half), _ptr(full), _cube_side(full, stride), stride)
+
+    def inject(self, u, uh):
+        s, _ = _sfx(u)
+        self.call("mmh_inject_" + s, _ptr(u), _ptr(uh), _cube_side(u))
+
+    def metric(self, G, def_, u, dx=1.0, lsquare=0, nu=None):
+        s, T = _sfx(def_)
+        n = _cube_side(def_)
+        nu = nu or u.numel() // def_.numel()
+        self.call("mmh_metric_" + s, _ptr(G), _ptr(def_), _ptr(u), nu, n, T(dx), lsquare)
+
+    def time_sweeps(self, phi, rhs, G, dx=1.0, nsweeps=10):
+        s, T = _sfx(phi)
+        ms = C.c_float(0)
+        self.call("mmh_time_sweeps_" + s, _ptr(phi), _ptr(rhs), _ptr(G), _cube_side(phi), T(dx), nsweeps, C.byref(ms))
+        return ms.value
+
+    def rlft3(self, data, speq, isign):
+        self.call("mmh_rlft3_d", _ptr(data), _ptr(speq), _cube_side(data), isign)
+
+    def lognormal(self, rho, sigma, seed, slope=-1.5, ks_frac=0.5):
+        self.call("mmh_lognormal_d", _ptr(rho), _cube_side(rho), C.c_double(sigma), C.c_double(slope),
+                  C.c_double(ks_frac), C.c_uint64(seed))
+
+
+def header_symbols(header=None):
+    """Function names declared in include/mmh.h."""
+    import re
+    header = header or os.path.join(os.path.dirname(PKG_DIR), "include", "mmh.h")
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mmh_[a-z0-9_]+)\s*\(", txt)))

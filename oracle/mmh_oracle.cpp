@@ -398,14 +398,22 @@ struct MgStats {
   int zerosum_entry;
 };
 
+// Sweep schedule: kind 0 is the reference's (nu = 4*NG/N per level, 32 + 32 on 4^3,
+// mg64.f90:27, multigrid.f90:108-112); kind 1 a fixed count per level (builder-defined).
+struct Sched {
+  int kind = 0, nu_pre = 2, nu_post = 2, nu_coarse = 32;
+};
+
 template <class T>
-void mg_level(T* u, T* rhs, const T* defpot, const T* ubig, int nu, int n, T dx, int iopt, int ng, int sumorder, MgStats* st, int lev) {
+void mg_level(T* u, T* rhs, const T* defpot, const T* ubig, int nu, int n, T dx, int iopt, int ng, int sumorder, MgStats* st, int lev,
+              const Sched& S = Sched()) {
   const int isolver = iopt % 8;
   if (n == 4) {
     T r1, relaxerr;
     std::vector<T> dummy(8);
-    relax(u, dummy.data(), rhs, defpot, ubig, nu, &r1, dx, 4, 32, isolver * 8 + 2);
-    relax(u, dummy.data(), rhs, defpot, ubig, nu, &relaxerr, dx, 4, 32, isolver * 8 + 2);
+    const int nc = S.kind == 0 ? 32 : S.nu_coarse;
+    relax(u, dummy.data(), rhs, defpot, ubig, nu, &r1, dx, 4, nc, isolver * 8 + 2);
+    relax(u, dummy.data(), rhs, defpot, ubig, nu, &relaxerr, dx, 4, nc, isolver * 8 + 2);
     bool g = (4 != ng && relaxerr > r1 * T(1.001) && r1 > T(1.e-5));
     if (st && lev < 16) { st->r1[lev] = r1; st->relaxerr[lev] = relaxerr; st->guard[lev] = g; st->nlev = lev + 1; }
     if (g) std::fill(u, u + 64, T(0));
@@ -415,15 +423,16 @@ void mg_level(T* u, T* rhs, const T* defpot, const T* ubig, int nu, int n, T dx,
   const int h = n / 2;
   const i64 h3 = (i64)h * h * h;
   std::vector<T> uhalf(h3), rhshalf(h3), defhalf(h3), usmall(h3);
-  const int nrelax = 4 * ng / n;
+  const int nrelax = S.kind == 0 ? 4 * ng / n : S.nu_pre;
+  const int nrelax2 = S.kind == 0 ? 4 * ng / n : S.nu_post;
   T r1, relaxerr;
   relax(u, rhshalf.data(), rhs, defpot, ubig, nu, &r1, dx, n, nrelax, isolver * 8 + 3);
   restrict27(defhalf.data(), defpot, n, 1);
   restrict27(usmall.data(), ubig, n, nu);
   std::fill(uhalf.begin(), uhalf.end(), T(0));
-  mg_level(uhalf.data(), rhshalf.data(), defhalf.data(), usmall.data(), 1, h, dx * T(2), iopt, ng, sumorder, st, lev + 1);
+  mg_level(uhalf.data(), rhshalf.data(), defhalf.data(), usmall.data(), 1, h, dx * T(2), iopt, ng, sumorder, st, lev + 1, S);
   prolong_add(u, uhalf.data(), n, true);
-  relax(u, rhshalf.data(), rhs, defpot, ubig, nu, &relaxerr, dx, n, nrelax, isolver * 8 + 2);
+  relax(u, rhshalf.data(), rhs, defpot, ubig, nu, &relaxerr, dx, n, nrelax2, isolver * 8 + 2);
   bool g = (n != ng && relaxerr > r1);
   if (st && lev < 16) { st->r1[lev] = r1; st->relaxerr[lev] = relaxerr; st->guard[lev] = g; }
   if (g) std::fill(u, u + (i64)n * n * n, T(0));
@@ -445,6 +454,29 @@ int multigrid(T* u, T* rhs, const T* defpot, const T* ubig, int nu, T dx, int n,
   }
   mg_level(u, rhs, defpot, ubig, nu, n, dx, iopt, ng, sumorder, st, 0);
   return 0;
+}
+
+// Solve to a relative residual (builder-defined): project rhs once (rgzerosum), then V-cycles from
+// the current u until max|rhs - L u| / max|rhs| <= tol, read after each cycle's top-level
+// post-smoothing (the relaxerr the reference prints, mg64.f90:43-45).
+template <class T>
+int solve(T* u, T* rhs, const T* defpot, const T* ubig, int nu, int n, double tol, int maxcyc, const Sched& S, int ng,
+          int sumorder, double* relres, int* cycles) {
+  if (!pow2(n)) return -1;
+  const i64 n3 = (i64)n * n * n;
+  rgzerosum(rhs, defpot, T(1), n, sumorder, nullptr);
+  T rn = 0;
+  for (i64 q = 0; q < n3; ++q) rn = fmax_(rn, (T)std::fabs(rhs[q]));
+  const double rden = rn > 0 ? (double)rn : 1.0;
+  MgStats st;
+  for (int c = 0; c < maxcyc; ++c) {
+    std::memset(&st, 0, sizeof(st));
+    mg_level(u, rhs, defpot, ubig, nu, n, T(1), 2, ng, sumorder, &st, 0, S);
+    relres[c] = st.relaxerr[0] / rden;
+    *cycles = c + 1;
+    if (relres[c] <= tol) return 0;
+  }
+  return 1;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -623,7 +655,7 @@ template <class T> inline void binv(const Hess<T>& h, T* b, T& det) {
 
 template <class T>
 int calcdefp(T* defp, T* tmp, T* tmp2, const T* def, const T* u, T dtold1, T dtaumesh, int nfluid, int n, T compressmax,
-             int ng, int sumorder, LimStats* st) {
+             int ng, int sumorder, LimStats* st, bool rhs_only = false) {
   if (!pow2(n) || nfluid < 1) return -1;
   const i64 n3 = (i64)n * n * n;
   const T pi = T(3.14159265358979323846264338328);
@@ -704,6 +736,7 @@ int calcdefp(T* defp, T* tmp, T* tmp2, const T* def, const T* u, T dtold1, T dta
   }
   std::memcpy(tmp, eflux.data(), sizeof(T) * n3);
   nsmooth(tmp, n);
+  if (rhs_only) return 0;
   // 4. solve (limiter.f90:394-408)
   std::fill(defp, defp + n3, T(0));
   MgStats* mst = st ? &st->mg : nullptr;
@@ -1100,6 +1133,19 @@ extern "C" {
   void or_delta_##S(const T* in, T* out, int n) { defp2delta(in, out, n); }
 
 OR_BOTH(DEF_SIMPLE)
+
+int or_limiter_rhs_d(double* tmp, const double* def, const double* u, double dtold, double dtaumesh, int nfluid, int n,
+                     double compressmax) {
+  std::vector<double> dummy(1), dummy2(1);
+  return calcdefp(dummy.data(), tmp, dummy2.data(), def, u, dtold, dtaumesh, nfluid, n, compressmax, n, 0, nullptr, true);
+}
+
+int or_solve_d(double* u, double* rhs, const double* defpot, const double* ubig, int nu, int n, double tol, int maxcyc,
+               int kind, int nu_pre, int nu_post, int nu_coarse, int ng, int so, double* relres, int* cycles) {
+  Sched S;
+  S.kind = kind; S.nu_pre = nu_pre; S.nu_post = nu_post; S.nu_coarse = nu_coarse;
+  return solve(u, rhs, defpot, ubig, nu, n, tol, maxcyc, S, ng > 0 ? ng : n, so, relres, cycles);
+}
 
 void or_lognormal(double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed) {
   lognormal(rho, n, sigma, slope, ks_frac, seed);

@@ -256,3 +256,20 @@ def ref_delta(lib, phi, n):
     nn = np.array([n, n, n], np.int32)
     lib.defp2delta_(ptr(phi), ptr(out), ptr(nn))
     return out
+
+
+def or_solve(u, rhs, defpot, ubig, nu, n, tol=1e-8, max_cycles=100, schedule=0, nu_pre=2, nu_post=2, nu_coarse=32,
+             ng=0, sumorder=1):
+    relres = np.zeros(256)
+    cyc = C.c_int(0)
+    rc = oracle().or_solve_d(ptr(u), ptr(rhs), ptr(defpot), ptr(ubig), nu, n, C.c_double(tol), max_cycles, schedule,
+                             nu_pre, nu_post, nu_coarse, ng, sumorder, ptr(relres), C.byref(cyc))
+    return rc, relres[:cyc.value]
+
+
+def or_limiter_rhs(defn, u, n, dtold=1.0, dtaumesh=1.0, nfluid=5, compressmax=10.0):
+    tmp = np.zeros(n ** 3)
+    rc = oracle().or_limiter_rhs_d(ptr(tmp), ptr(defn), ptr(u), C.c_double(dtold), C.c_double(dtaumesh), nfluid, n,
+                                   C.c_double(compressmax))
+    assert rc == 0, rc
+    return tmp

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP path against the oracle on the same seeded inputs.
+
+Integer-exact bar for floating point: every kernel keeps the reference's association order and
+the library is built without contraction, so FP64 and FP32 results must be bit-identical to the
+oracle run with the blocked sum order (sumorder=1) that the device reductions use.  Exceptions,
+each with its tolerance in the test: the limiter's eigenvalue gate uses device acos/cos
+(calcdefp with ncompress > 0: rel 1e-12 FP64 / 1e-5 FP32), and the lognormal generator uses device
+log/cos/exp (rel 1e-12)."""
+import numpy as np
+import pytest
+
+import refbind as R
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def aos(rho):
+    u = np.zeros((rho.size, 5), rho.dtype)
+    u[:, 0] = rho
+    return u.ravel()
+
+
+def smooth_def(n, amp, seed, dt):
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal(n ** 3).astype(dt)
+    for _ in range(4):
+        R.or_nsmooth(d, n)
+    return (d * dt(amp)).astype(dt)
+
+
+def field(n, sigma, seed, dt):
+    return R.or_lognormal(n, sigma, seed).astype(dt)
+
+
+DT = [np.float64, np.float32]
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_metric_and_sweeps_bitexact(ctx, dt, n):
+    print("case metric/sweeps n=%d %s" % (n, dt.__name__), flush=True)
+    rho = field(n, 1.5, 11, dt)
+    u = aos(rho)
+    d = smooth_def(n, 2.0, 3, dt)
+    G = R.or_metric(d, u, 5, n, 1.0)
+    Gd = dev(np.zeros(6 * n ** 3, dt))
+    ctx.metric(Gd, dev(d), dev(u), 1.0, 0, nu=5)
+    assert np.array_equal(host(Gd), G)
+    rhs = (1 - rho).astype(dt)
+    R.or_nsmooth(rhs, n)
+    phi = np.zeros(n ** 3, dt)
+    phid = dev(phi)
+    R.or_sweep(phi, rhs, G, n, 1.0, 3)
+    ctx.time_sweeps(phid, dev(rhs), dev(G), 1.0, 3)
+    assert np.array_equal(host(phid), phi)
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n,nrelax,iopt", [(16, 4, 19), (16, 3, 19), (32, 8, 19), (8, 32, 19), (4, 32, 18),
+                                           (16, 4, 11), (16, 4, 18)])
+def test_relax_bitexact(ctx, dt, n, nrelax, iopt):
+    print("case relax n=%d nrelax=%d iopt=%d %s" % (n, nrelax, iopt, dt.__name__), flush=True)
+    rho = field(n, 1.0, 5, dt)
+    u = aos(rho)
+    d = smooth_def(n, 1.5, 8, dt)
+    rhs = np.random.default_rng(2).standard_normal(n ** 3).astype(dt)
+    a = np.zeros(n ** 3, dt)
+    r = np.zeros((n // 2) ** 3, dt)
+    rinf = R.or_relax(a, r, rhs, d, u, 5, 1.0, n, nrelax, iopt)
+    ad, rd = dev(np.zeros(n ** 3, dt)), dev(np.zeros((n // 2) ** 3, dt))
+    rinf_d = ctx.relax(ad, rd, dev(rhs), dev(d), dev(u), 1.0, nrelax, iopt, nu=5)
+    assert np.array_equal(host(ad), a)
+    if iopt % 8 in (1, 3):
+        assert np.array_equal(host(rd), r)
+    assert rinf_d == rinf
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n,ng,sigma", [(16, 16, 1.0), (32, 32, 2.0), (32, 64, 1.0), (64, 64, 2.0)])
+def test_multigrid_bitexact(ctx, dt, n, ng, sigma):
+    print("case multigrid n=%d ng=%d %s" % (n, ng, dt.__name__), flush=True)
+    rho = field(n, sigma, 7, dt)
+    u = aos(rho)
+    d = smooth_def(n, 2.0, 4, dt)
+    rhs = (1 - rho).astype(dt)
+    R.or_nsmooth(rhs, n)
+    a, b = np.zeros(n ** 3, dt), rhs.copy()
+    st = R.or_multigrid(a, b, d, u, 5, 1.0, n, 2, ng, 1)
+    ad, bd = dev(np.zeros(n ** 3, dt)), dev(rhs)
+    sd = ctx.multigrid(ad, bd, dev(d), dev(u), 1.0, 2, ng, nu=5)
+    assert np.array_equal(host(bd), b)
+    assert np.array_equal(host(ad), a)
+    assert list(sd.guard[:sd.nlev]) == list(st.guard[:sd.nlev])
+    assert [sd.relaxerr[l] for l in range(sd.nlev)] == [st.relaxerr[l] for l in range(sd.nlev)]
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n,sigma,amp", [(16, 1.0, 0.0), (32, 2.0, 2.0), (32, 2.0, 10.0)])
+def test_calcdefp(ctx, dt, n, sigma, amp):
+    print("case calcdefp n=%d amp=%g %s" % (n, amp, dt.__name__), flush=True)
+    rho = field(n, sigma, 7, dt)
+    u = aos(rho)
+    d = smooth_def(n, amp, 4, dt) if amp else np.zeros(n ** 3, dt)
+    p, t, t2 = np.zeros(n ** 3, dt), np.zeros(n ** 3, dt), np.zeros((n + 2) * n * n, dt)
+    ls = R.or_calcdefp(p, t, t2, d, u, n, sumorder=1)
+    pd, td, t2d = dev(np.zeros(n ** 3, dt)), dev(np.zeros(n ** 3, dt)), dev(np.zeros((n + 2) * n * n, dt))
+    lg = ctx.calcdefp(pd, td, t2d, dev(d), dev(u))
+    assert lg.ncompress == ls.ncompress and lg.nexpand == ls.nexpand
+    got = host(pd)
+    if ls.ncompress == 0:
+        assert np.array_equal(got, p)
+    else:
+        tol = 1e-12 if dt == np.float64 else 1e-5
+        assert np.abs(got - p).max() <= tol * np.abs(p).max()
+    assert np.array_equal(host(td)[3:], t[3:])
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n", [16, 32])
+def test_relaxing_bitexact(ctx, dt, n):
+    print("case relaxing n=%d %s" % (n, dt.__name__), flush=True)
+    rho = field(n, 1.5, 9, dt)
+    u = aos(rho)
+    d = smooth_def(n, 2.0, 5, dt)
+    dp = smooth_def(n, 0.5, 6, dt)
+    ua = u.copy()
+    R.or_relaxing(ua, d, dp, n)
+    ud = dev(u)
+    ctx.relaxing(ud, dev(d), dev(dp))
+    assert np.array_equal(host(ud), ua)
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n", [16, 32])
+def test_gradient_delta_bitexact(ctx, dt, n):
+    print("case gradient/delta n=%d %s" % (n, dt.__name__), flush=True)
+    phi = smooth_def(n, 3.0, 12, dt)
+    g = R.or_gradient(phi, n)
+    dl = R.or_delta(phi, n)
+    gd, dd = dev(np.zeros(3 * n ** 3, dt)), dev(np.zeros(n ** 3, dt))
+    ctx.scalar2gradient(dev(phi), gd)
+    ctx.defp2delta(dev(phi), dd)
+    assert np.array_equal(host(gd), g)
+    assert np.array_equal(host(dd), dl)
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_small_ops_bitexact(ctx, n):
+    dt = np.float64
+    a = smooth_def(n, 1.0, 1, dt)
+    b = a.copy()
+    R.or_nsmooth(b, n)
+    ad = dev(a)
+    ctx.nsmooth(ad)
+    assert np.array_equal(host(ad), b)
+    h = R.or_restrict(a, n)
+    hd = dev(np.zeros((n // 2) ** 3, dt))
+    ctx.restrict(hd, dev(a))
+    assert np.array_equal(host(hd), h)
+    u = a.copy()
+    R.or_inject(u, h, n)
+    ud = dev(a)
+    ctx.inject(ud, dev(h))
+    assert np.array_equal(host(ud), u)
+    data = a.copy()
+    speq = np.zeros(2 * n * n, dt)
+    R.or_rlft3(data, speq, n, 1)
+    dd, sd = dev(a), dev(np.zeros(2 * n * n, dt))
+    ctx.rlft3(dd, sd, 1)
+    assert np.array_equal(host(dd), data) and np.array_equal(host(sd), speq)
+
+
+@pytest.mark.parametrize("n,sigma,seed", [(16, 1.0, 1234), (32, 2.0, 7)])
+def test_lognormal_generator(ctx, n, sigma, seed):
+    ref = R.or_lognormal(n, sigma, seed)
+    rd = dev(np.zeros(n ** 3))
+    ctx.lognormal(rd, sigma, seed)
+    got = host(rd)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert abs(got.mean() - 1.0) < 1e-12
+
+
+def test_error_paths(ctx):
+    import mmh_amd
+    x = dev(np.zeros(6 ** 3))
+    with pytest.raises(mmh_amd.MmhError) as e:
+        ctx.multigrid(x, x.clone(), x.clone(), x.clone())
+    assert e.value.code == mmh_amd.MMH_EINVAL
+
+
+@pytest.mark.parametrize("n,sched", [(32, 0), (32, 1), (64, 0)])
+def test_solve_fp64_bitexact(ctx, n, sched):
+    print("case solve n=%d sched=%d" % (n, sched), flush=True)
+    rho = field(n, 2.0, 42, np.float64)
+    u = aos(rho)
+    d = smooth_def(n, 1.0, 4, np.float64)
+    rhs = np.zeros(n ** 3)
+    rd = dev(rhs)
+    ctx.limiter_rhs(rd, dev(d), dev(u))
+    rhs = host(rd)
+    a, b = np.zeros(n ** 3), rhs.copy()
+    rc, rr = R.or_solve(a, b, d, u, 5, n, 1e-8, 100, sched, 2, 2, 16, 0, 1)
+    assert rc == 0
+    ad, bd = dev(np.zeros(n ** 3)), dev(rhs)
+    st, ok = ctx.solve(ad, bd, dev(d), dev(u), 1e-8, 100, sched, 2, 2, 16, nu=5)
+    assert ok and st.cycles == len(rr)
+    assert list(st.relres[:st.cycles]) == list(rr)
+    assert np.array_equal(host(ad), a)
+
+
+@pytest.mark.parametrize("n,sched", [(32, 0), (64, 0)])
+def test_solve_mixed_matches_fp64(ctx, n, sched):
+    print("case mixed n=%d sched=%d" % (n, sched), flush=True)
+    rho = field(n, 2.0, 42, np.float64)
+    u = aos(rho)
+    d = smooth_def(n, 1.0, 4, np.float64)
+    rd = dev(np.zeros(n ** 3))
+    ctx.limiter_rhs(rd, dev(d), dev(u))
+    rhs = host(rd)
+    a64, b64 = dev(np.zeros(n ** 3)), dev(rhs)
+    st64, ok64 = ctx.solve(a64, b64, dev(d), dev(u), 1e-10, 200, sched, 2, 2, 16, nu=5)
+    am, bm = dev(np.zeros(n ** 3)), dev(rhs)
+    stm, okm = ctx.solve(am, bm, dev(d), dev(u), 1e-8, 200, sched, 2, 2, 16, mixed=True, nu=5)
+    assert ok64 and okm
+    assert stm.relres[stm.cycles - 1] <= 1e-8
+    x64, xm = host(a64), host(am)
+    err = np.abs(xm - x64).max() / np.abs(x64).max()
+    print("mixed cycles %d fp64 cycles %d rel err %.3e" % (stm.cycles, st64.cycles, err))
+    assert err <= 1e-7
