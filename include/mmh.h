@@ -158,6 +158,8 @@ int mmh_relaxing_host_f(mmh_ctx*, float* u, const float* def, const float* defp,
 int mmh_relaxing_host_d(mmh_ctx*, double* u, const double* def, const double* defp, double dt, int nfluid, int n);
 int mmh_gradient_host_f(mmh_ctx*, const float* phi, float* psi, int n);
 int mmh_delta_host_f(mmh_ctx*, const float* phi, float* delta, int n);
+int mmh_nsmooth_host_f(mmh_ctx*, float* a, int n);
+int mmh_lognormal_host_d(mmh_ctx*, double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed);
 
 /* The process-wide context used by the Fortran drop-ins (created on first use on device 0). */
 mmh_ctx* mmh_default_ctx(void);

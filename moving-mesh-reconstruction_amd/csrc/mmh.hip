@@ -1005,6 +1005,27 @@ int mmh_gradient_host_f(mmh_ctx* c, const float* phi, float* psi, int n) {
   });
 }
 
+int mmh_nsmooth_host_f(mmh_ctx* c, float* a, int n) {
+  return guarded(c, [&] {
+    require(is_pow2(n), "nsmooth: bad size");
+    const i64 n3 = cube(n);
+    float* d = up(c, "a", a, n3);
+    nsmooth_dev<float>(c, d, n, 1);
+    down(c, a, d, n3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_lognormal_host_d(mmh_ctx* c, double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed) {
+  return guarded(c, [&] {
+    require(is_pow2(n), "lognormal: bad size");
+    double* d = up<double>(c, "rho", nullptr, cube(n));
+    lognormal_dev(c, d, n, sigma, slope, ks_frac, seed);
+    down(c, rho, d, cube(n));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
 int mmh_delta_host_f(mmh_ctx* c, const float* phi, float* delta, int n) {
   return guarded(c, [&] {
     require(is_pow2(n) && n >= 4, "delta: bad size");

@@ -1150,6 +1150,11 @@ int or_solve_d(double* u, double* rhs, const double* defpot, const double* ubig,
 void or_lognormal(double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed) {
   lognormal(rho, n, sigma, slope, ks_frac, seed);
 }
+double or_compression_d(double c11, double c12, double c13, double c22, double c23, double c33) {
+  double comp;
+  eigen_limits(c11, c12, c13, c22, c23, c33, 3.14159265358979323846264338328, comp);
+  return comp;
+}
 int or_mgstats_size(void) { return (int)sizeof(MgStats); }
 int or_limstats_size(void) { return (int)sizeof(LimStats); }
 }

@@ -1,0 +1,188 @@
+"""Closed forms and invariants that pin the oracle beyond the reference's own outputs (CPU only).
+Each check is chosen so a plausible slip (sign, index, transposed operand, dropped term) fails it."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import refbind as R
+
+
+def grid(n):
+    i = np.arange(n)
+    X, Y, Z = np.meshgrid(i, i, i, indexing="ij")
+    f = lambda A: A.ravel(order="F").astype(np.float64)
+    return f(X), f(Y), f(Z)
+
+
+def test_fft3_matches_numpy_both_signs():
+    rng = np.random.default_rng(0)
+    d0, d1, d2 = 8, 16, 4
+    a = rng.standard_normal((d2, d1, d0)) + 1j * rng.standard_normal((d2, d1, d0))
+    for s in (-1, 1):
+        z = a.ravel().copy()
+        R.or_fft3(z, d0, d1, d2, s)
+        ref = np.fft.fftn(a) if s < 0 else np.fft.ifftn(a) * a.size
+        assert np.abs(z.reshape(a.shape) - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_rlft3_packing_and_roundtrip():
+    n = 16
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(n ** 3)
+    data, speq = x.copy(), np.zeros(2 * n * n)
+    R.or_rlft3(data, speq, n, 1)
+    # forward kernel exp(+i...) (recipes12.f90:44,87): X = conj(numpy fft) for real input
+    X = np.conj(np.fft.fftn(x.reshape((n, n, n), order="F")))
+    z = data.view(np.complex128).reshape((n // 2, n, n), order="F")
+    assert np.abs(z - X[: n // 2]).max() < 1e-11
+    sq = speq.view(np.complex128).reshape((n, n), order="F")
+    assert np.abs(sq - X[n // 2]).max() < 1e-11
+    R.or_rlft3(data, speq, n, -1)
+    assert np.abs(data * 2.0 / n ** 3 - x).max() < 1e-13
+
+
+def test_delta_and_gradient_closed_forms():
+    n = 32
+    X, Y, Z = grid(n)
+    k = 2 * np.pi * 3 / n
+    phi = np.cos(k * X) * np.sin(2 * np.pi * Y / n)
+    lap = -(k ** 2 + (2 * np.pi / n) ** 2) * phi
+    assert np.abs(R.or_delta(phi, n) - (-lap)).max() < 1e-12
+    s = np.sin(k * Z)
+    g = R.or_gradient(s, n).reshape(3, -1)
+    # NR's +i forward convention makes scalar2gradient return minus the gradient
+    assert np.abs(g[2] - (-k * np.cos(k * Z))).max() < 1e-12
+    assert np.abs(g[0]).max() < 1e-12 and np.abs(g[1]).max() < 1e-12
+
+
+def test_metric_flat_mesh_is_box_mean_density():
+    n = 8
+    rng = np.random.default_rng(2)
+    rho = 1 + 0.5 * rng.random(n ** 3)
+    u = np.zeros((n ** 3, 5)); u[:, 0] = rho
+    G = R.or_metric(np.zeros(n ** 3), u.ravel(), 5, n, 1.0).reshape(6, -1)
+    r3 = rho.reshape((n, n, n), order="F")
+    box = sum(np.roll(np.roll(np.roll(r3, a, 0), b, 1), c, 2) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)) / 27
+    box = box.ravel(order="F")
+    for c, diag in enumerate([1, 0, 0, 1, 0, 1]):
+        assert np.allclose(G[c], box * diag, rtol=1e-14, atol=1e-15)
+
+
+def test_operator_symbol_symmetry_and_nullspace():
+    n = 16
+    X, Y, Z = grid(n)
+    u = np.zeros((n ** 3, 5)); u[:, 0] = 1.0
+    G = R.or_metric(np.zeros(n ** 3), u.ravel(), 5, n, 1.0)
+    kx, ky, kz = 2 * np.pi * np.array([1, 2, 3]) / n
+    phi = np.cos(kx * X + ky * Y + kz * Z)
+    r, _ = R.or_residual(phi, np.zeros(n ** 3), G, n, 1.0)
+    sym = 2 * np.cos(kx) + 2 * np.cos(ky) + 2 * np.cos(kz) - 6   # limiter.f90:569
+    assert np.abs(-r - sym * phi).max() < 1e-13
+    # curvilinear metric: symmetric operator with constants in its null space
+    rng = np.random.default_rng(3)
+    d = rng.standard_normal(n ** 3)
+    for _ in range(4):
+        R.or_nsmooth(d, n)
+    rho = R.or_lognormal(n, 1.0, 5)
+    u[:, 0] = rho
+    G = R.or_metric(d * 2, u.ravel(), 5, n, 1.0)
+    x, y = rng.standard_normal(n ** 3), rng.standard_normal(n ** 3)
+    Lx = -R.or_residual(x, np.zeros(n ** 3), G, n, 1.0)[0]
+    Ly = -R.or_residual(y, np.zeros(n ** 3), G, n, 1.0)[0]
+    assert abs(Lx @ y - x @ Ly) < 1e-11 * np.abs(Lx).sum()
+    L1 = -R.or_residual(np.ones(n ** 3), np.zeros(n ** 3), G, n, 1.0)[0]
+    assert np.abs(L1).max() < 1e-12
+
+
+def test_smoothing_symbol_and_transfers():
+    n = 16
+    X, Y, Z = grid(n)
+    k = 2 * np.pi * np.array([1, 3, 2]) / n
+    f = np.cos(k[0] * X) * np.cos(k[1] * Y) * np.cos(k[2] * Z)
+    g = f.copy()
+    R.or_nsmooth(g, n)
+    assert np.abs(g - np.prod((1 + np.cos(k)) / 2) * f).max() < 1e-14
+    # restriction samples the smoothed field at even points; prolongation reproduces coarse points
+    h = R.or_restrict(f, n)
+    Xc, Yc, Zc = grid(n // 2)
+    assert np.abs(h - np.prod((1 + np.cos(k)) / 2) * np.cos(k[0] * 2 * Xc) * np.cos(k[1] * 2 * Yc) * np.cos(k[2] * 2 * Zc)).max() < 1e-14
+    p = R.or_prolong(h, n).reshape((n, n, n), order="F")
+    assert np.abs(p[::2, ::2, ::2].ravel(order="F") - h).max() <= 4e-16 * np.abs(h).max()  # (8 h)/8 summed in order
+    assert np.allclose(R.or_prolong(np.full((n // 2) ** 3, 2.5), n), 2.5, rtol=0, atol=1e-15)
+
+
+def test_eigenvalue_limiter_matches_eigvalsh():
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        A = rng.standard_normal((3, 3)) * 0.4
+        C_ = np.eye(3) + (A + A.T) / 2
+        lam = np.linalg.eigvalsh(C_)
+        comp = R.oracle().or_compression_d
+        comp.restype = C.c_double
+        got = comp(*[C.c_double(v) for v in (C_[0, 0], C_[0, 1], C_[0, 2], C_[1, 1], C_[1, 2], C_[2, 2])])
+        assert abs(got - 1 / lam.min()) < 1e-9 * abs(1 / lam.min())
+
+
+def test_projection_mean_and_sums():
+    n = 16
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n ** 3)
+    assert abs(R.or_sum(x, 0) - R.or_sum(x, 1)) < 1e-12
+    ints = rng.integers(-1000, 1000, n ** 3).astype(np.float64)
+    assert R.or_sum(ints, 1) == ints.sum() == R.or_sum(ints, 0)
+    rho = R.or_lognormal(n, 1.5, 9)
+    u = np.zeros((n ** 3, 5)); u[:, 0] = rho
+    d = rng.standard_normal(n ** 3)
+    for _ in range(4):
+        R.or_nsmooth(d, n)
+    rhs = 1 - rho
+    phi, b = np.zeros(n ** 3), rhs.copy()
+    R.or_multigrid(phi, b, d, u.ravel(), 5, 1.0, n, 2, n, 0)
+    assert abs(b.sum()) < 1e-12 * np.abs(b).sum()   # rgzerosum: compatible rhs
+    assert abs(phi.mean()) < 1e-14 * np.abs(phi).max()  # zerosum at the top level
+
+
+def test_vcycle_solve_matches_fft_poisson():
+    # flat mesh, uniform density: L is the 7-point Laplacian, solvable exactly by FFT
+    n = 32
+    rng = np.random.default_rng(6)
+    rhs = rng.standard_normal(n ** 3)
+    for _ in range(3):
+        R.or_nsmooth(rhs, n)
+    rhs -= rhs.mean()
+    u = np.zeros((n ** 3, 5)); u[:, 0] = 1.0
+    phi, b = np.zeros(n ** 3), rhs.copy()
+    rc, rr = R.or_solve(phi, b, np.zeros(n ** 3), u.ravel(), 5, n, 1e-12, 200, 0, 2, 2, 32, 0, 0)
+    assert rc == 0 and rr[-1] <= 1e-12
+    k = 2 * np.pi * np.fft.fftfreq(n)
+    KX, KY, KZ = np.meshgrid(k, k, k, indexing="ij")
+    sym = 2 * np.cos(KX) + 2 * np.cos(KY) + 2 * np.cos(KZ) - 6
+    sym[0, 0, 0] = 1
+    fh = np.fft.fftn(rhs.reshape((n, n, n), order="F")) / sym
+    fh[0, 0, 0] = 0
+    exact = np.real(np.fft.ifftn(fh)).ravel(order="F")
+    assert np.abs(phi - exact).max() < 1e-10 * np.abs(exact).max()
+
+
+def test_relaxing_conserves_mass():
+    n = 16
+    rho = R.or_lognormal(n, 1.0, 3)
+    u = np.zeros((n ** 3, 5)); u[:, 0] = rho
+    u = u.ravel()
+    rng = np.random.default_rng(7)
+    d, dp = rng.standard_normal(n ** 3), rng.standard_normal(n ** 3)
+    for _ in range(4):
+        R.or_nsmooth(d, n); R.or_nsmooth(dp, n)
+    m0 = u[0::5].sum()
+    R.or_relaxing(u, d * 2, dp * 0.3, n)
+    assert abs(u[0::5].sum() - m0) < 1e-12 * m0
+    assert np.abs(u.reshape(-1, 5)[:, 1:]).max() == 0.0   # momentum/energy stay exactly zero
+
+
+def test_lognormal_generator_statistics():
+    n = 32
+    rho = R.or_lognormal(n, 2.0, 42)
+    assert abs(rho.mean() - 1) < 1e-13 and rho.min() > 0
+    g = np.log(rho)
+    assert abs(np.var(g) - 4.0) < 1e-9
