@@ -61,6 +61,8 @@ typedef struct {
   int nu_pre, nu_post, nu_coarse;  /* schedule 1: sweeps per level and on 4^3 */
   int ng;            /* schedule 0: the reference's NG (top size if 0) */
   int mixed;         /* 0: FP64 V-cycles; 1: FP32 V-cycles inside FP64 defect correction */
+  int krylov;        /* 1: flexible preconditioned CG, one symmetric V-cycle per iteration as the
+                        preconditioner (schedule 1 with nu_pre == nu_post; no divergence guard) */
 } mmh_solve_opts;
 
 typedef struct {
@@ -75,6 +77,9 @@ int mmh_destroy(mmh_ctx* ctx);
 const char* mmh_error_string(mmh_ctx* ctx);
 int mmh_set_stream(mmh_ctx* ctx, void* hip_stream); /* NULL = the null stream (default) */
 int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffers */
+/* Options: "sweep_impl" selects the array layout mmh_time_sweeps runs the smoother in: 0 = the
+ * sub-lattice layout the V-cycle uses (default), 1 = column-major.  Results are bit-identical. */
+int mmh_set_option(mmh_ctx* ctx, const char* key, int value);
 
 /* ---- solver (device pointers) --------------------------------------------------------------- */
 /* relax (mmh/gauss1.f90:64): nrelax colour sweeps of arr; mod(iopt,8) selects residual output:

@@ -18,6 +18,14 @@ __device__ __forceinline__ i64 ix3(int i, int j, int k, int n) { return (i64)i +
 template <class T> __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
 template <class T> __device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
 
+// Workgroups are dealt to the 8 XCDs round-robin (id mod 8), each XCD with its own L2.  Remapping
+// the id so that XCD x receives the contiguous range [x*nb/8, (x+1)*nb/8) keeps spatially adjacent
+// tiles, which share halo and neighbour-plane data, in one L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
+  constexpr unsigned X = 8;
+  return (nb % X) ? b : (b % X) * (nb / X) + b / X;
+}
+
 constexpr int RB = 256;  // reduction block
 constexpr int RPT = 8;   // elements per thread per reduction block
 
@@ -61,19 +69,62 @@ template <class T> __global__ void k_copy_strided(T* __restrict__ dst, const T* 
 }
 
 // ------------------------------------------------------------------------------------------------
+// Array layouts of multigrid level data.  L = 0: column-major (the reference's order).  L = 1:
+// sub-lattice order -- the 8 parity classes c = i%2 + 2*(j%2) + 4*(k%2) stored one after another,
+// each (n/2)^3 in column-major order of (i/2, j/2, k/2).  A colour sweep touches one class and
+// reads six others, all at unit stride in this order.
+template <int L> __device__ __forceinline__ i64 lx(int i, int j, int k, int n) {
+  if (L == 0) return ix3(i, j, k, n);
+  const int h = n >> 1;
+  const int c = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
+  return (i64)c * ((i64)h * h * h) + (i >> 1) + (i64)h * ((j >> 1) + (i64)h * (k >> 1));
+}
+
+// Point of linear thread index p (0 <= p < n^3) such that consecutive p are consecutive in memory.
+template <int L> __device__ __forceinline__ void point_of(i64 p, int n, int& i, int& j, int& k) {
+  if (L == 0) {
+    i = (int)(p % n); j = (int)((p / n) % n); k = (int)(p / ((i64)n * n));
+    return;
+  }
+  const int h = n >> 1;
+  const i64 h3 = (i64)h * h * h;
+  const int c = (int)(p / h3);
+  const i64 q = p - (i64)c * h3;
+  i = 2 * (int)(q % h) + (c & 1);
+  j = 2 * (int)((q / h) % h) + ((c >> 1) & 1);
+  k = 2 * (int)(q / ((i64)h * h)) + ((c >> 2) & 1);
+}
+
+template <class T, int TO>
+__global__ void k_relayout(T* __restrict__ dst, const T* __restrict__ src, int n, int comps) {
+  // TO = 1: column-major -> sub-lattice; TO = 0: sub-lattice -> column-major; comps fields of n^3
+  const i64 n3 = (i64)n * n * n;
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n3 * comps) return;
+  const int f = (int)(p / n3);
+  int i, j, k;
+  point_of<1>(p - f * n3, n, i, j, k);
+  const i64 a = f * n3 + lx<1>(i, j, k, n), b = f * n3 + ix3(i, j, k, n);
+  if (TO == 1) dst[a] = src[b];
+  else dst[b] = src[a];
+}
+
+// ------------------------------------------------------------------------------------------------
 // Metric G = rho_box * adj(A)/det (mmh/gauss1.f90:1128-1205, cbajraw, triad branch) or
 // adj(A) adj(A)^T / det (lsquare, 1173-1181); A = I + Hess(def)/dx^2; det <= 1e-5 -> identity.
-// G is stored as 6 planes of n^3: G11, G12, G13, G22, G23, G33.
-template <class T>
+// def and u are column-major; G is 6 fields (G11, G12, G13, G22, G23, G33) in layout L.
+template <class T, int L>
 __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __restrict__ def, const T* __restrict__ u,
                                                 int nu, int n, T dx, int lsquare) {
   const i64 n3 = (i64)n * n * n;
-  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n3) return;
-  const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
+  const i64 pp = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pp >= n3) return;
+  int i, j, k;
+  point_of<L>(pp, n, i, j, k);
+  const i64 p = lx<L>(i, j, k, n);
   const int ip = wrapi(i + 1, n), im = wrapi(i - 1, n), jp = wrapi(j + 1, n), jm = wrapi(j - 1, n), kp = wrapi(k + 1, n), km = wrapi(k - 1, n);
   const T di2 = T(1) / (dx * dx);
-  const T d0 = def[p];
+  const T d0 = def[ix3(i, j, k, n)];
   T phixx = (def[ix3(ip, j, k, n)] - T(2) * d0 + def[ix3(im, j, k, n)]) * di2;
   T phiyy = (def[ix3(i, jp, k, n)] - T(2) * d0 + def[ix3(i, jm, k, n)]) * di2;
   T phizz = (def[ix3(i, j, kp, n)] - T(2) * d0 + def[ix3(i, j, km, n)]) * di2;
@@ -119,8 +170,9 @@ __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __re
   G[5 * n3 + p] = urg * b33 / det;
 }
 
-// The 19-point symmetric curvilinear operator at one point (mmh/gauss1.f90:954-978).
-template <class T>
+// The 19-point symmetric curvilinear operator at one point (mmh/gauss1.f90:954-978), on arrays in
+// layout L.
+template <class T, int L>
 __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __restrict__ G, i64 n3, int n, int i, int j,
                                           int k, T& t1, T& t2, T& tdiag) {
   const T* G1 = G;
@@ -129,59 +181,66 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __
   const T* G4 = G + 3 * n3;
   const T* G5 = G + 4 * n3;
   const T* G6 = G + 5 * n3;
-  const int ip = wrapi(i + 1, n), im = wrapi(i - 1, n), jp = wrapi(j + 1, n), jm = wrapi(j - 1, n), kp = wrapi(k + 1, n), km = wrapi(k - 1, n);
-  const i64 c = ix3(i, j, k, n);
-  T b11p = (G1[ix3(ip, j, k, n)] + G1[c]) / T(2);
-  T b11m = (G1[ix3(im, j, k, n)] + G1[c]) / T(2);
-  T b22p = (G4[ix3(i, jp, k, n)] + G4[c]) / T(2);
-  T b22m = (G4[ix3(i, jm, k, n)] + G4[c]) / T(2);
-  T b33p = (G6[ix3(i, j, kp, n)] + G6[c]) / T(2);
-  T b33m = (G6[ix3(i, j, km, n)] + G6[c]) / T(2);
+  const int m = n - 1;
+  const int ip = (i + 1) & m, im = (i - 1) & m, jp = (j + 1) & m, jm = (j - 1) & m, kp = (k + 1) & m, km = (k - 1) & m;
+  const i64 c = lx<L>(i, j, k, n);
+#define X_(a, b, cc) lx<L>(a, b, cc, n)
+  T b11p = (G1[X_(ip, j, k)] + G1[c]) / T(2);
+  T b11m = (G1[X_(im, j, k)] + G1[c]) / T(2);
+  T b22p = (G4[X_(i, jp, k)] + G4[c]) / T(2);
+  T b22m = (G4[X_(i, jm, k)] + G4[c]) / T(2);
+  T b33p = (G6[X_(i, j, kp)] + G6[c]) / T(2);
+  T b33m = (G6[X_(i, j, km)] + G6[c]) / T(2);
   tdiag = -(b11p + b11m + b22p + b22m + b33p + b33m);
-  t1 = (b11p * phi[ix3(ip, j, k, n)] + b11m * phi[ix3(im, j, k, n)] + b22p * phi[ix3(i, jp, k, n)] +
-        b22m * phi[ix3(i, jm, k, n)] + b33p * phi[ix3(i, j, kp, n)] + b33m * phi[ix3(i, j, km, n)]);
-  t2 = ((phi[ix3(ip, jp, k, n)] - phi[ix3(im, jp, k, n)]) * G2[ix3(i, jp, k, n)]
-      - (phi[ix3(ip, jm, k, n)] - phi[ix3(im, jm, k, n)]) * G2[ix3(i, jm, k, n)]
-      + (phi[ix3(ip, jp, k, n)] - phi[ix3(ip, jm, k, n)]) * G2[ix3(ip, j, k, n)]
-      - (phi[ix3(im, jp, k, n)] - phi[ix3(im, jm, k, n)]) * G2[ix3(im, j, k, n)]
-      + (phi[ix3(i, jp, kp, n)] - phi[ix3(i, jm, kp, n)]) * G5[ix3(i, j, kp, n)]
-      - (phi[ix3(i, jp, km, n)] - phi[ix3(i, jm, km, n)]) * G5[ix3(i, j, km, n)]
-      + (phi[ix3(i, jp, kp, n)] - phi[ix3(i, jp, km, n)]) * G5[ix3(i, jp, k, n)]
-      - (phi[ix3(i, jm, kp, n)] - phi[ix3(i, jm, km, n)]) * G5[ix3(i, jm, k, n)]
-      + (phi[ix3(ip, j, kp, n)] - phi[ix3(im, j, kp, n)]) * G3[ix3(i, j, kp, n)]
-      - (phi[ix3(ip, j, km, n)] - phi[ix3(im, j, km, n)]) * G3[ix3(i, j, km, n)]
-      + (phi[ix3(ip, j, kp, n)] - phi[ix3(ip, j, km, n)]) * G3[ix3(ip, j, k, n)]
-      - (phi[ix3(im, j, kp, n)] - phi[ix3(im, j, km, n)]) * G3[ix3(im, j, k, n)]) / T(4);
+  t1 = (b11p * phi[X_(ip, j, k)] + b11m * phi[X_(im, j, k)] + b22p * phi[X_(i, jp, k)] +
+        b22m * phi[X_(i, jm, k)] + b33p * phi[X_(i, j, kp)] + b33m * phi[X_(i, j, km)]);
+  t2 = ((phi[X_(ip, jp, k)] - phi[X_(im, jp, k)]) * G2[X_(i, jp, k)]
+      - (phi[X_(ip, jm, k)] - phi[X_(im, jm, k)]) * G2[X_(i, jm, k)]
+      + (phi[X_(ip, jp, k)] - phi[X_(ip, jm, k)]) * G2[X_(ip, j, k)]
+      - (phi[X_(im, jp, k)] - phi[X_(im, jm, k)]) * G2[X_(im, j, k)]
+      + (phi[X_(i, jp, kp)] - phi[X_(i, jm, kp)]) * G5[X_(i, j, kp)]
+      - (phi[X_(i, jp, km)] - phi[X_(i, jm, km)]) * G5[X_(i, j, km)]
+      + (phi[X_(i, jp, kp)] - phi[X_(i, jp, km)]) * G5[X_(i, jp, k)]
+      - (phi[X_(i, jm, kp)] - phi[X_(i, jm, km)]) * G5[X_(i, jm, k)]
+      + (phi[X_(ip, j, kp)] - phi[X_(im, j, kp)]) * G3[X_(i, j, kp)]
+      - (phi[X_(ip, j, km)] - phi[X_(im, j, km)]) * G3[X_(i, j, km)]
+      + (phi[X_(ip, j, kp)] - phi[X_(ip, j, km)]) * G3[X_(ip, j, k)]
+      - (phi[X_(im, j, kp)] - phi[X_(im, j, km)]) * G3[X_(im, j, k)]) / T(4);
+#undef X_
 }
 
 // One colour class of the 8-colour Gauss-Seidel sweep: points with (k, i, j) parity (kpar, io, jo)
-// (relaxplane, mmh/gauss1.f90:910-980).  One thread per point of the colour.
-template <class T>
+// (relaxplane, mmh/gauss1.f90:910-980).  One thread per point of the colour; in layout 1 the
+// threads of a workgroup own consecutive points of the class and every stencil access is
+// unit-stride.
+template <class T, int L>
 __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G, int n,
                                                 T dx2, int kpar, int io, int jo) {
   const int h = n >> 1;
-  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 q = (i64)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
   const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
   const i64 n3 = (i64)n * n * n;
   T t1, t2, tdiag;
-  stencil19(phi, G, n3, n, i, j, k, t1, t2, tdiag);
-  const i64 c = ix3(i, j, k, n);
+  stencil19<T, L>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+  const i64 c = lx<L>(i, j, k, n);
   phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
 }
 
 // Residual r = rhs - L[phi] (calcresid, mmh/gauss1.f90:683-709); optional store; per-block max|r|.
-template <class T>
+template <class T, int L>
 __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restrict__ bmax, const T* __restrict__ phi,
                                               const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2) {
   __shared__ double s[RB];
   const i64 n3 = (i64)n * n * n;
-  const i64 p = (i64)blockIdx.x * RB + threadIdx.x;
+  const i64 pp = (i64)blockIdx.x * RB + threadIdx.x;
   double a = 0.0;
-  if (p < n3) {
-    const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
+  if (pp < n3) {
+    int i, j, k;
+    point_of<L>(pp, n, i, j, k);
+    const i64 p = lx<L>(i, j, k, n);
     T t1, t2, tdiag;
-    stencil19(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+    stencil19<T, L>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
     T rv = rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
     if (r) r[p] = rv;
     a = (double)fabs(rv);
@@ -195,20 +254,94 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
   if (threadIdx.x == 0) bmax[blockIdx.x] = s[0];
 }
 
+// q = L[p] (the operator of k_resid: (t1 + t2 + tdiag*p)/dx^2) and per-block sums of p*q in FP64,
+// block b covering elements [b*RB, (b+1)*RB) combined by the halving tree (deterministic).
+template <class T, int L>
+__global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __restrict__ bsum, const T* __restrict__ p,
+                                                  const T* __restrict__ G, int n, T di2) {
+  __shared__ double s[RB];
+  const i64 n3 = (i64)n * n * n;
+  const i64 pp = (i64)blockIdx.x * RB + threadIdx.x;
+  double a = 0.0;
+  if (pp < n3) {
+    int i, j, k;
+    point_of<L>(pp, n, i, j, k);
+    const i64 c = lx<L>(i, j, k, n);
+    T t1, t2, tdiag;
+    stencil19<T, L>(p, G, n3, n, i, j, k, t1, t2, tdiag);
+    T v = (t1 + t2 + tdiag * p[c]) * di2;
+    q[c] = v;
+    a = (double)p[c] * (double)v;
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = RB / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+}
+
+// Per-block sums of a*b (FP64), same blocking as k_apply_dot.
+template <class T>
+__global__ __launch_bounds__(RB) void k_dot(double* __restrict__ bsum, const T* __restrict__ a, const T* __restrict__ b, i64 n) {
+  __shared__ double s[RB];
+  const i64 p = (i64)blockIdx.x * RB + threadIdx.x;
+  s[threadIdx.x] = p < n ? (double)a[p] * (double)b[p] : 0.0;
+  __syncthreads();
+  for (int h = RB / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+}
+
+// x += alpha p; r -= alpha q (the previous r kept in rold); per-block max|r|.
+__global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ x, double* __restrict__ r, double* __restrict__ rold,
+                                                  const double* __restrict__ p, const double* __restrict__ q, double alpha,
+                                                  double* __restrict__ bmax, i64 n) {
+  __shared__ double s[RB];
+  const i64 i = (i64)blockIdx.x * RB + threadIdx.x;
+  double a = 0.0;
+  if (i < n) {
+    x[i] = x[i] + alpha * p[i];
+    const double ro = r[i];
+    rold[i] = ro;
+    const double rn = ro - alpha * q[i];
+    r[i] = rn;
+    a = fabs(rn);
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = RB / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bmax[blockIdx.x] = s[0];
+}
+
+// p = z + beta p.
+__global__ void k_cg_dir(double* __restrict__ p, const double* __restrict__ z, double beta, i64 n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = z[i] + beta * p[i];
+}
+
 // Full-weighting restriction of the residual as accumulated by calcresid
 // (mmh/gauss1.f90:819-859): coarse plane ck gets a main part from fine planes 2ck, 2ck+1 and a
 // share from fine plane 2ck-1; plane ck0 accumulates main then share, every other plane share
-// then main (the order in which r1lax's plane pipeline reaches them).
-template <class T>
+// then main (the order in which r1lax's plane pipeline reaches them).  r and rc in layout L.
+template <class T, int L>
 __global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0) {
   const int h = n >> 1;
   const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
-  const int ci = (int)(q % h), cj = (int)((q / h) % h), ck = (int)(q / ((i64)h * h));
+  int ci, cj, ck;
+  point_of<L>(q, h, ci, cj, ck);
   const int kb = 2 * ck, kbp = 2 * ck + 1, ks = wrapi(2 * ck - 1, n);
   const int ib = 2 * ci, ibp = wrapi(ib + 1, n), ibm = wrapi(ib - 1, n);
   const int jb = 2 * cj, jbp = wrapi(jb + 1, n), jbm = wrapi(jb - 1, n);
-#define R_(a, b, c) r[ix3(a, b, c, n)]
+#define R_(a, b, c) r[lx<L>(a, b, c, n)]
   T acc = 0;
   if (ck == ck0) {
     acc = acc + R_(ib, jb, kb) / T(8)
@@ -228,7 +361,7 @@ __global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, cons
       + (R_(ibp, jbp, kbp) + R_(ibp, jbm, kbp) + R_(ibm, jbp, kbp) + R_(ibm, jbm, kbp)) / T(64);
   }
 #undef R_
-  rc[q] = acc;
+  rc[lx<L>(ci, cj, ck, h)] = acc;
 }
 
 // 27-point weights 1/8, 1/16, 1/32, 1/64 around fine point (i2,j2,k2): restrict/restrictu
@@ -263,15 +396,18 @@ __global__ __launch_bounds__(256) void k_smooth27(T* __restrict__ out, const T* 
   out[p] = box27(in, 1, n, i, j, k);
 }
 
-// Trilinear prolongation (mmh/mgutil.f90:38-71 inject adds; 2-34 prolong overwrites).
-template <class T>
+// Trilinear prolongation (mmh/mgutil.f90:38-71 inject adds; 2-34 prolong overwrites); u (n^3) and
+// uh ((n/2)^3) in layout L.
+template <class T, int L>
 __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __restrict__ uh, int n, int add) {
-  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (i64)n * n * n) return;
+  const i64 pp = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pp >= (i64)n * n * n) return;
   const int h = n >> 1;
-  const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
+  int i, j, k;
+  point_of<L>(pp, n, i, j, k);
+  const i64 p = lx<L>(i, j, k, n);
   const int ibl = i >> 1, ibh = ((i + 1) >> 1) % h, jbl = j >> 1, jbh = ((j + 1) >> 1) % h, kbl = k >> 1, kbh = ((k + 1) >> 1) % h;
-#define H_(a, b, c) uh[ix3(a, b, c, h)]
+#define H_(a, b, c) uh[lx<L>(a, b, c, h)]
   T s = (H_(ibl, jbl, kbl) + H_(ibl, jbl, kbh) + H_(ibl, jbh, kbl) + H_(ibl, jbh, kbh)
        + H_(ibh, jbh, kbl) + H_(ibh, jbh, kbh) + H_(ibh, jbl, kbl) + H_(ibh, jbl, kbh)) / T(8);
 #undef H_

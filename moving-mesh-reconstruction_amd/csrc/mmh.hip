@@ -30,6 +30,7 @@ struct mmh_ctx {
   std::map<std::string, std::pair<void*, size_t>> ws;
   std::set<std::string> uploaded;  // constant tables already copied to the device
   std::string err;
+  int sweep_impl = 0;  // mmh_time_sweeps layout: 0 sub-lattice (the V-cycle's), 1 column-major
 };
 
 namespace {
@@ -125,6 +126,8 @@ struct Sched {
   int kind = 0;  // 0 reference, 1 fixed
   int ng = 0;
   int nu_pre = 2, nu_post = 2, nu_coarse = 32;
+  bool guard = true;       // the reference's divergence guard on coarse levels
+  bool symmetric = false;  // post-smoothing in reverse colour order (symmetric V-cycle)
   int pre(int n) const { return kind == 0 ? 4 * ng / n : nu_pre; }
   int post(int n) const { return kind == 0 ? 4 * ng / n : nu_post; }
   int coarse() const { return kind == 0 ? 32 : nu_coarse; }
@@ -162,42 +165,53 @@ void build_hier(mmh_ctx* c, Hier<T>& H, const std::string& tag, T* u, T* rhs, co
     if (l == 0 && G0) H.G[0] = G0;
     else H.G[l] = wsT<T>(c, t + "G", 6 * cube(m));
     if (l > 0 || want_G0) {
-      k_metric<T><<<nblk(cube(m)), 256, 0, c->stream>>>(H.G[l], H.def[l], H.u1[l], H.nu[l], m, H.dx[l], lsquare);
+      k_metric<T, 1><<<nblk(cube(m)), 256, 0, c->stream>>>(H.G[l], H.def[l], H.u1[l], H.nu[l], m, H.dx[l], lsquare);
       LAUNCHCHK();
     }
   }
 }
 
-template <class T> void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns) {
+template <class T> void relayout(mmh_ctx* c, T* dst, const T* src, int n, int comps, int to_sl) {
+  const i64 cnt = cube(n) * comps;
+  if (to_sl) k_relayout<T, 1><<<nblk(cnt), 256, 0, c->stream>>>(dst, src, n, comps);
+  else k_relayout<T, 0><<<nblk(cnt), 256, 0, c->stream>>>(dst, src, n, comps);
+  LAUNCHCHK();
+}
+
+// ns colour sweeps of phi: 8 launches per sweep, one per colour class; layout 1 (sub-lattice, the
+// V-cycle's) or 0 (column-major).
+template <class T>
+void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, int layout = 1, bool reverse = false) {
   const T dx2 = dx * dx;
   const int h = n / 2;
   const unsigned nb = nblk(cube(h));
   for (int s = 0; s < ns; ++s)
-    for (int kp = 0; kp < 2; ++kp)
-      for (int io = 0; io < 2; ++io)
-        for (int jo = 0; jo < 2; ++jo) {
-          k_colour<T><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
-        }
+    for (int cc = 0; cc < 8; ++cc) {
+      const int col = reverse ? 7 - cc : cc;
+      const int kp = col >> 2, io = (col >> 1) & 1, jo = col & 1;
+      if (layout == 1) k_colour<T, 1><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+      else k_colour<T, 0><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+    }
   LAUNCHCHK();
 }
 
-// relax on the device: sweeps, then residual (store when rc != null), max-norm into *rinf_dev,
-// and the calcresid restriction into rc.
+// relax on the device (level arrays in sub-lattice layout): sweeps, then residual (stored when
+// rc != null), max-norm into *rinf_dev, and the calcresid restriction into rc.
 template <class T>
 void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n, T dx, int nrelax, int want_norm, T* rc,
-               double* rinf_dev) {
-  sweeps(c, phi, rhs, G, n, dx, nrelax);
+               double* rinf_dev, bool reverse = false) {
+  sweeps(c, phi, rhs, G, n, dx, nrelax, 1, reverse);
   if (!want_norm && !rc) return;
   const i64 n3 = cube(n);
   const unsigned nb = nblk(n3, RB);
   double* bmax = (double*)ws_get(c, "bmax", sizeof(double) * nb);
-  k_resid<T><<<nb, RB, 0, c->stream>>>(rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx));
+  k_resid<T, 1><<<nb, RB, 0, c->stream>>>(rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx));
   LAUNCHCHK();
   reduce<double>(c, bmax, nb, 1, rinf_dev);
   if (rc) {
     const int nrmax = std::min(4, std::max(1, n / 4));
     int nrl = nrelax > nrmax ? (nrelax % nrmax ? nrelax % nrmax : nrmax) : nrelax;
-    k_restrict_resid<T><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(rc, rscratch, n, nrl % (n / 2));
+    k_restrict_resid<T, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(rc, rscratch, n, nrl % (n / 2));
     LAUNCHCHK();
   }
 }
@@ -210,24 +224,25 @@ template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const 
   if (n == 4) {
     const int nc = S.coarse();
     relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, 1, (T*)nullptr, sc + 0);
-    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, 1, (T*)nullptr, sc + 1);
+    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, 1, (T*)nullptr, sc + 1, S.symmetric);
     reduce<T>(c, H.u[l], n3, 0, sc + 2);
-    k_zerosum_guard<T><<<1, 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, n == ng_top ? 0 : 2, H.guard + l);
+    k_zerosum_guard<T><<<1, 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 2, H.guard + l);
     LAUNCHCHK();
     return;
   }
   relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.pre(n), 1, H.rhs[l + 1], sc + 0);
   fill<T>(c, H.u[l + 1], T(0), cube(n / 2));
   vcycle(c, H, l + 1, ng_top, S);
-  k_prolong<T><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], H.u[l + 1], n, 1);
+  k_prolong<T, 1><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], H.u[l + 1], n, 1);
   LAUNCHCHK();
-  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.post(n), 1, (T*)nullptr, sc + 1);
+  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.post(n), 1, (T*)nullptr, sc + 1, S.symmetric);
   reduce<T>(c, H.u[l], n3, 0, sc + 2);
-  k_zerosum_guard<T><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, n == ng_top ? 0 : 1, H.guard + l);
+  k_zerosum_guard<T><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
+                                                      H.guard + l);
   LAUNCHCHK();
 }
 
-// rgzerosum (gauss1.f90:1219-1417) on the device.
+// rgzerosum (gauss1.f90:1219-1417) on the device (column-major arrays).
 template <class T> void rgzerosum_dev(mmh_ctx* c, T* rhs, const T* def, T dx, int n, double* sums) {
   const i64 n3 = cube(n);
   T* det = wsT<T>(c, "det", n3);
@@ -257,7 +272,8 @@ template <class T> void fetch_stats(mmh_ctx* c, Hier<T>& H, mmh_mg_stats* st) {
   st->entry_zerosum = g[MMH_MAXLEV];
 }
 
-// multigrid entry (multigrid.f90:16-90).
+// multigrid entry (multigrid.f90:16-90).  The projection and the entry check act on the caller's
+// column-major arrays; the cycle itself runs on sub-lattice copies of u and rhs.
 template <class T>
 void multigrid_dev(mmh_ctx* c, T* u, T* rhs, const T* defpot, const T* ubig, T dx, int n, int nu, int iopt, int ng,
                    mmh_mg_stats* st) {
@@ -266,17 +282,23 @@ void multigrid_dev(mmh_ctx* c, T* u, T* rhs, const T* defpot, const T* ubig, T d
   require(ng >= n && is_pow2(ng), "multigrid: ng must be a power of two >= n");
   require(nu >= 1, "multigrid: nu >= 1");
   const int lsquare = (iopt % 8) == 1;
+  const i64 n3 = cube(n);
+  T* us = wsT<T>(c, "mg_us", n3);
+  T* rs = wsT<T>(c, "mg_rs", n3);
   Hier<T> H;
-  build_hier(c, H, "mg", u, rhs, defpot, ubig, nu, dx, n, lsquare);
+  build_hier(c, H, "mg", us, rs, defpot, ubig, nu, dx, n, lsquare);
   double* sums = H.scal + 3 * MMH_MAXLEV;
   rgzerosum_dev(c, rhs, defpot, dx, n, sums);
-  reduce<T>(c, u, cube(n), 0, sums + 2);
-  k_entry_zerosum<T><<<nblk(cube(n)), 256, 0, c->stream>>>(u, sums + 2, cube(n), H.guard + MMH_MAXLEV);
+  reduce<T>(c, u, n3, 0, sums + 2);
+  k_entry_zerosum<T><<<nblk(n3), 256, 0, c->stream>>>(u, sums + 2, n3, H.guard + MMH_MAXLEV);
   LAUNCHCHK();
+  relayout(c, us, u, n, 1, 1);
+  relayout(c, rs, rhs, n, 1, 1);
   Sched S;
   S.kind = 0;
   S.ng = ng;
   vcycle(c, H, 0, ng, S);
+  relayout(c, u, us, n, 1, 0);
   fetch_stats(c, H, st);
   HIPCHK(hipStreamSynchronize(c->stream));
 }
@@ -286,16 +308,24 @@ void relax_api(mmh_ctx* c, T* arr, T* resid, const T* rhs, const T* def, const T
                int iopt) {
   require(is_pow2(n) && n <= 1024, "relax: n must be a power of two in [4, 1024]");
   require(iopt >= 0 && iopt <= 256, "relax: iopt out of range");
+  require(iopt % 8 <= 3, "relax: mod(iopt,8) must be 0..3");
   const i64 n3 = cube(n);
   const int ires = iopt % 8;
   const int lsquare = ((iopt / 8) % 8) == 1;
   T* G = wsT<T>(c, "relaxG", 6 * n3);
-  k_metric<T><<<nblk(n3), 256, 0, c->stream>>>(G, def, u, nu, n, dx, lsquare);
+  k_metric<T, 1><<<nblk(n3), 256, 0, c->stream>>>(G, def, u, nu, n, dx, lsquare);
   LAUNCHCHK();
+  T* as = wsT<T>(c, "relax_as", n3);
+  T* bs = wsT<T>(c, "relax_bs", n3);
+  T* cs = wsT<T>(c, "relax_cs", cube(n / 2));
+  relayout(c, as, arr, n, 1, 1);
+  relayout(c, bs, rhs, n, 1, 1);
   double* d = (double*)ws_get(c, "relaxrinf", sizeof(double));
   T* rs = wsT<T>(c, "relaxr", n3);
   const bool want_res = ires == 1 || ires == 3;
-  relax_dev(c, arr, rhs, G, rs, n, dx, nrelax, ires != 0, want_res ? resid : (T*)nullptr, d);
+  relax_dev(c, as, bs, G, rs, n, dx, nrelax, ires != 0, want_res ? cs : (T*)nullptr, d);
+  relayout(c, arr, as, n, 1, 0);
+  if (want_res) relayout(c, resid, cs, n / 2, 1, 0);
   double h = 0;
   if (ires == 2 || ires == 3) HIPCHK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -646,44 +676,16 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), 1, Hf.rhs[l + 1], sc + 0);
   fill<float>(c, Hf.u[l + 1], 0.f, cube(n / 2));
   vcycle_mixed(c, Hf, Hd, l + 1, ng_top, S);
-  k_prolong<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
+  k_prolong<float, 1><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
   LAUNCHCHK();
-  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), 1, (float*)nullptr, sc + 1);
+  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), 1, (float*)nullptr, sc + 1, S.symmetric);
   reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
-  k_zerosum_guard<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], sc + 2, sc, n3, n == ng_top ? 0 : 1, Hf.guard + l);
+  k_zerosum_guard<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
+                                                          Hf.guard + l);
   LAUNCHCHK();
 }
 
-void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
-               const mmh_solve_opts* o, mmh_solve_stats* st) {
-  require(is_pow2(n) && n <= 1024, "solve: n must be a power of two in [4, 1024]");
-  require(o != nullptr, "solve: opts required");
-  require(o->max_cycles >= 1 && o->max_cycles <= 256, "solve: max_cycles in [1, 256]");
-  Sched S;
-  S.kind = o->schedule;
-  S.ng = o->ng > 0 ? o->ng : n;
-  S.nu_pre = o->nu_pre; S.nu_post = o->nu_post; S.nu_coarse = o->nu_coarse;
-  if (S.kind == 1) require(S.nu_pre >= 1 && S.nu_post >= 1 && S.nu_coarse >= 1, "solve: sweep counts >= 1");
-  const i64 n3 = cube(n);
-  Hier<double> H;
-  build_hier(c, H, "sd", u, rhs, defpot, ubig, nu, 1.0, n, 0);
-  double* sums = H.scal + 3 * MMH_MAXLEV;
-  rgzerosum_dev<double>(c, rhs, defpot, 1.0, n, sums);
-  reduce<double>(c, rhs, n3, 1, sums + 3);
-  const double rn = read_scalar<double>(c, sums + 3);
-  if (st) { std::memset(st, 0, sizeof(*st)); st->rhs_norm = rn; }
-  const double rden = rn > 0 ? rn : 1.0;
-  if (!o->mixed) {
-    for (int cyc = 0; cyc < o->max_cycles; ++cyc) {
-      vcycle<double>(c, H, 0, S.ng, S);
-      const double rr = read_scalar<double>(c, H.scal + 1) / rden;
-      if (st) { st->relres[cyc] = rr; st->cycles = cyc + 1; }
-      if (rr <= o->tol) return;
-    }
-    throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
-  }
-  // mixed: FP32 hierarchy with the FP64 metrics rounded; FP64 residual at the top.
-  Hier<float> F;
+void build_fp32(mmh_ctx* c, Hier<float>& F, Hier<double>& H) {
   F.nlev = H.nlev;
   F.scal = (double*)ws_get(c, "sfscal", sizeof(double) * 4 * MMH_MAXLEV);
   F.guard = (int*)ws_get(c, "sfguard", sizeof(int) * (MMH_MAXLEV + 1));
@@ -699,41 +701,191 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
     k_convert<double, float><<<nblk(6 * cube(m)), 256, 0, c->stream>>>(F.G[l], H.G[l], 6 * cube(m));
     LAUNCHCHK();
   }
+}
+
+double dot_dev(mmh_ctx* c, const double* a, const double* b, i64 n, double* dst) {
+  const unsigned nb = nblk(n, RB);
+  double* bs = (double*)ws_get(c, "dot_bs", sizeof(double) * nb);
+  k_dot<double><<<nb, RB, 0, c->stream>>>(bs, a, b, n);
+  LAUNCHCHK();
+  reduce<double>(c, bs, nb, 0, dst);
+  return read_scalar<double>(c, dst);
+}
+
+// Flexible preconditioned conjugate gradients on L x = b (x, b sub-lattice FP64).  The
+// preconditioner is one symmetric V-cycle from z = 0 (colours reversed in post-smoothing, no
+// divergence guard), in FP64 or, with F, in FP32 with its 4^3 level in FP64.  beta is the
+// Polak-Ribiere form z.(r - r_old) / z_old.r_old, which tolerates the FP32 preconditioner's rounding.
+void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int n, const Sched& S,
+         const mmh_solve_opts* o, mmh_solve_stats* st, double rden, double* sums) {
+  const i64 n3 = cube(n);
+  double* r = wsT<double>(c, "cg_r", n3);
+  double* ro = wsT<double>(c, "cg_ro", n3);
+  double* p = wsT<double>(c, "cg_p", n3);
+  double* q = wsT<double>(c, "cg_q", n3);
+  double* z = wsT<double>(c, "cg_z", n3);
+  const unsigned nb = nblk(n3, RB);
+  double* bm = (double*)ws_get(c, "cg_bm", sizeof(double) * nb);
+  double* bs = (double*)ws_get(c, "cg_bs", sizeof(double) * nb);
+  Hier<double> P = H;
+  P.u[0] = z;
+  P.rhs[0] = r;
+  auto precond = [&] {
+    if (!F) {
+      fill<double>(c, z, 0.0, n3);
+      vcycle<double>(c, P, 0, n, S);
+    } else {
+      k_convert<double, float><<<nblk(n3), 256, 0, c->stream>>>(F->rhs[0], r, n3);
+      fill<float>(c, F->u[0], 0.f, n3);
+      LAUNCHCHK();
+      vcycle_mixed(c, *F, H, 0, n, S);
+      k_convert<float, double><<<nblk(n3), 256, 0, c->stream>>>(z, F->u[0], n3);
+      LAUNCHCHK();
+    }
+  };
+  k_resid<double, 1><<<nb, RB, 0, c->stream>>>(r, bm, x, b, H.G[0], n, 1.0);
+  LAUNCHCHK();
+  precond();
+  double rz = dot_dev(c, r, z, n3, sums + 4);
+  HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  for (int it = 0; it < o->max_cycles; ++it) {
+    k_apply_dot<double, 1><<<nb, RB, 0, c->stream>>>(q, bs, p, H.G[0], n, 1.0);
+    LAUNCHCHK();
+    reduce<double>(c, bs, nb, 0, sums + 5);
+    const double pq = read_scalar<double>(c, sums + 5);
+    const double alpha = rz / pq;
+    k_cg_update<<<nb, RB, 0, c->stream>>>(x, r, ro, p, q, alpha, bm, n3);
+    LAUNCHCHK();
+    reduce<double>(c, bm, nb, 1, sums + 6);
+    const double rr = read_scalar<double>(c, sums + 6) / rden;
+    if (st) { st->relres[it] = rr; st->cycles = it + 1; }
+    if (rr <= o->tol) break;
+    if (it + 1 == o->max_cycles) throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
+    precond();
+    const double rz_new = dot_dev(c, z, r, n3, sums + 4);
+    const double zro = dot_dev(c, z, ro, n3, sums + 4);
+    const double beta = (rz_new - zro) / rz;
+    k_cg_dir<<<nblk(n3), 256, 0, c->stream>>>(p, z, beta, n3);
+    LAUNCHCHK();
+    rz = rz_new;
+  }
+  // report the true residual of the final iterate and remove its mean
+  k_resid<double, 1><<<nb, RB, 0, c->stream>>>(nullptr, bm, x, b, H.G[0], n, 1.0);
+  LAUNCHCHK();
+  reduce<double>(c, bm, nb, 1, sums + 6);
+  if (st && st->cycles > 0) st->relres[st->cycles - 1] = read_scalar<double>(c, sums + 6) / rden;
+  reduce<double>(c, x, n3, 0, sums + 2);
+  k_zerosum_guard<double><<<nblk(n3), 256, 0, c->stream>>>(x, sums + 2, H.scal, n3, 0, nullptr);
+  LAUNCHCHK();
+}
+
+void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
+               const mmh_solve_opts* o, mmh_solve_stats* st) {
+  require(is_pow2(n) && n <= 1024, "solve: n must be a power of two in [4, 1024]");
+  require(o != nullptr, "solve: opts required");
+  require(o->max_cycles >= 1 && o->max_cycles <= 256, "solve: max_cycles in [1, 256]");
+  Sched S;
+  S.kind = o->schedule;
+  S.ng = o->ng > 0 ? o->ng : n;
+  S.nu_pre = o->nu_pre; S.nu_post = o->nu_post; S.nu_coarse = o->nu_coarse;
+  if (S.kind == 1) require(S.nu_pre >= 1 && S.nu_post >= 1 && S.nu_coarse >= 1, "solve: sweep counts >= 1");
+  if (o->krylov) {
+    require(S.kind == 1 && S.nu_pre == S.nu_post, "solve: the Krylov solve needs a fixed symmetric schedule");
+    S.guard = false;
+    S.symmetric = true;
+    S.ng = n;
+  }
+  const i64 n3 = cube(n);
+  double* us = wsT<double>(c, "sd_us", n3);
+  double* rs = wsT<double>(c, "sd_rs", n3);
+  Hier<double> H;
+  build_hier(c, H, "sd", us, rs, defpot, ubig, nu, 1.0, n, 0);
+  double* sums = H.scal + 3 * MMH_MAXLEV;
+  rgzerosum_dev<double>(c, rhs, defpot, 1.0, n, sums);
+  reduce<double>(c, rhs, n3, 1, sums + 3);
+  relayout(c, us, u, n, 1, 1);
+  relayout(c, rs, rhs, n, 1, 1);
+  const double rn = read_scalar<double>(c, sums + 3);
+  if (st) { std::memset(st, 0, sizeof(*st)); st->rhs_norm = rn; }
+  const double rden = rn > 0 ? rn : 1.0;
+  auto finish = [&] {
+    relayout(c, u, us, n, 1, 0);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  };
+  if (o->krylov) {
+    Hier<float> F;
+    if (o->mixed) build_fp32(c, F, H);
+    try {
+      pcg(c, H, o->mixed ? &F : nullptr, us, rs, n, S, o, st, rden, sums);
+    } catch (...) {
+      finish();
+      throw;
+    }
+    return finish();
+  }
+  if (!o->mixed) {
+    for (int cyc = 0; cyc < o->max_cycles; ++cyc) {
+      vcycle<double>(c, H, 0, S.ng, S);
+      const double rr = read_scalar<double>(c, H.scal + 1) / rden;
+      if (st) { st->relres[cyc] = rr; st->cycles = cyc + 1; }
+      if (rr <= o->tol) return finish();
+    }
+    finish();
+    throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
+  }
+  // mixed defect correction: r = rhs - L u in FP64, an FP32 V-cycle solves L e = r, u += e.
+  Hier<float> F;
+  build_fp32(c, F, H);
   double* r64 = H.r[0];
   const unsigned nb = nblk(n3, RB);
   double* bmax = (double*)ws_get(c, "sbmax", sizeof(double) * nb);
   for (int cyc = 0; cyc < o->max_cycles; ++cyc) {
-    k_resid<double><<<nb, RB, 0, c->stream>>>(r64, bmax, u, rhs, H.G[0], n, 1.0);
+    k_resid<double, 1><<<nb, RB, 0, c->stream>>>(r64, bmax, us, rs, H.G[0], n, 1.0);
     LAUNCHCHK();
     k_convert<double, float><<<nblk(n3), 256, 0, c->stream>>>(F.rhs[0], r64, n3);
     fill<float>(c, F.u[0], 0.f, n3);
     LAUNCHCHK();
     vcycle_mixed(c, F, H, 0, S.ng, S);
-    k_add_f2d<<<nblk(n3), 256, 0, c->stream>>>(u, F.u[0], n3);
+    k_add_f2d<<<nblk(n3), 256, 0, c->stream>>>(us, F.u[0], n3);
     LAUNCHCHK();
-    k_resid<double><<<nb, RB, 0, c->stream>>>(nullptr, bmax, u, rhs, H.G[0], n, 1.0);
+    k_resid<double, 1><<<nb, RB, 0, c->stream>>>(nullptr, bmax, us, rs, H.G[0], n, 1.0);
     LAUNCHCHK();
     reduce<double>(c, bmax, nb, 1, sums + 3);
     const double rr = read_scalar<double>(c, sums + 3) / rden;
     if (st) { st->relres[cyc] = rr; st->cycles = cyc + 1; }
     if (rr <= o->tol) {
       // remove the mean as the FP64 path's top-level zerosum does
-      reduce<double>(c, u, n3, 0, sums + 2);
-      k_zerosum_guard<double><<<nblk(n3), 256, 0, c->stream>>>(u, sums + 2, H.scal, n3, 0, nullptr);
+      reduce<double>(c, us, n3, 0, sums + 2);
+      k_zerosum_guard<double><<<nblk(n3), 256, 0, c->stream>>>(us, sums + 2, H.scal, n3, 0, nullptr);
       LAUNCHCHK();
-      HIPCHK(hipStreamSynchronize(c->stream));
-      return;
+      return finish();
     }
   }
+  finish();
   throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
 }
 
+// Times ns sweeps of the multigrid smoother on copies of phi, rhs and G (column-major on entry)
+// in the layout the V-cycle uses; phi receives the result.
 template <class T> void time_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, float* ms) {
+  const i64 n3 = cube(n);
+  T* ps = wsT<T>(c, "ts_p", n3);
+  T* rs = wsT<T>(c, "ts_r", n3);
+  T* gs = wsT<T>(c, "ts_g", 6 * n3);
+  const int sl = c->sweep_impl == 0;
+  relayout(c, ps, phi, n, 1, 1);
+  relayout(c, rs, rhs, n, 1, 1);
+  relayout(c, gs, G, n, 6, 1);
+  if (!sl) {
+    HIPCHK(hipMemcpyAsync(ps, phi, sizeof(T) * n3, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(rs, rhs, sizeof(T) * n3, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(gs, G, sizeof(T) * 6 * n3, hipMemcpyDeviceToDevice, c->stream));
+  }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c->stream));
-  sweeps(c, phi, rhs, G, n, dx, ns);
+  sweeps(c, ps, rs, gs, n, dx, ns, sl);
   HIPCHK(hipEventRecord(e1, c->stream));
   HIPCHK(hipEventSynchronize(e1));
   float t = 0;
@@ -741,6 +893,9 @@ template <class T> void time_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G
   *ms = t / std::max(1, ns);
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));
+  if (sl) relayout(c, phi, ps, n, 1, 0);
+  else HIPCHK(hipMemcpyAsync(phi, ps, sizeof(T) * n3, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
 }
 
 // Host-pointer wrapper: copies inputs to context buffers, runs f, copies outputs back.
@@ -813,6 +968,18 @@ int mmh_destroy(mmh_ctx* c) {
 
 const char* mmh_error_string(mmh_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+int mmh_set_option(mmh_ctx* c, const char* key, int value) {
+  return guarded(c, [&] {
+    require(key != nullptr, "set_option: null key");
+    if (std::string(key) == "sweep_impl") {
+      require(value == 0 || value == 1, "set_option: sweep_impl is 0 or 1");
+      c->sweep_impl = value;
+    } else {
+      throw MmhError(MMH_EINVAL, std::string("set_option: unknown key ") + key);
+    }
+  });
+}
+
 int mmh_set_stream(mmh_ctx* c, void* s) {
   return guarded(c, [&] { c->stream = (hipStream_t)s; });
 }
@@ -876,7 +1043,7 @@ mmh_ctx* mmh_default_ctx(void) {
   int mmh_inject_##S(mmh_ctx* c, T* u, const T* uh, int n) {                                                       \
     return guarded(c, [&] {                                                                                        \
       require(is_pow2(n), "inject: bad size");                                                                     \
-      k_prolong<T><<<nblk(cube(n)), 256, 0, c->stream>>>(u, uh, n, 1);                                             \
+      k_prolong<T, 0><<<nblk(cube(n)), 256, 0, c->stream>>>(u, uh, n, 1);                                             \
       LAUNCHCHK();                                                                                                 \
       HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
     });                                                                                                            \
@@ -884,7 +1051,7 @@ mmh_ctx* mmh_default_ctx(void) {
   int mmh_metric_##S(mmh_ctx* c, T* G, const T* def, const T* u, int nu, int n, T dx, int lsq) {                   \
     return guarded(c, [&] {                                                                                        \
       require(is_pow2(n) && nu >= 1, "metric: bad size");                                                          \
-      k_metric<T><<<nblk(cube(n)), 256, 0, c->stream>>>(G, def, u, nu, n, dx, lsq);                               \
+      k_metric<T, 0><<<nblk(cube(n)), 256, 0, c->stream>>>(G, def, u, nu, n, dx, lsq);                               \
       LAUNCHCHK();                                                                                                 \
       HIPCHK(hipStreamSynchronize(c->stream));                                                                     \
     });                                                                                                            \

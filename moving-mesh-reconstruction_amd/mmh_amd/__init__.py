@@ -45,7 +45,8 @@ class LimStats(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_cycles", C.c_int), ("schedule", C.c_int), ("nu_pre", C.c_int),
-                ("nu_post", C.c_int), ("nu_coarse", C.c_int), ("ng", C.c_int), ("mixed", C.c_int)]
+                ("nu_post", C.c_int), ("nu_coarse", C.c_int), ("ng", C.c_int), ("mixed", C.c_int),
+                ("krylov", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -126,6 +127,9 @@ class Context:
     def set_stream(self, stream_ptr):
         self.call("mmh_set_stream", C.c_void_p(stream_ptr))
 
+    def set_option(self, key, value):
+        self.call("mmh_set_option", key.encode(), int(value))
+
     def release_workspace(self):
         self.call("mmh_release_workspace")
 
@@ -149,10 +153,10 @@ class Context:
         return st
 
     def solve(self, u, rhs, defpot, ubig, tol=1e-8, max_cycles=100, schedule=0, nu_pre=2, nu_post=2, nu_coarse=32,
-              ng=0, mixed=False, nu=None):
+              ng=0, mixed=False, nu=None, krylov=False):
         n = _cube_side(u)
         nu = nu or ubig.numel() // u.numel()
-        o = SolveOpts(tol, max_cycles, schedule, nu_pre, nu_post, nu_coarse, ng, 1 if mixed else 0)
+        o = SolveOpts(tol, max_cycles, schedule, nu_pre, nu_post, nu_coarse, ng, 1 if mixed else 0, 1 if krylov else 0)
         st = SolveStats()
         rc = lib().mmh_solve_d(self._c, _ptr(u), _ptr(rhs), _ptr(defpot), _ptr(ubig), n, nu, C.byref(o), C.byref(st))
         if rc not in (MMH_OK, MMH_ENOCONV):

@@ -322,10 +322,32 @@ template <class T> void prolong_add(T* u, const T* uh, int n, bool add) {
       }
 }
 
+// Sub-lattice order: the 8 parity classes (i%2 + 2*(j%2) + 4*(k%2)) one after another, each in
+// (i/2, j/2, k/2) column-major order.  The HIP library stores multigrid level arrays this way, so
+// its level sums (sumorder 2) run over this order.
+template <class T> std::vector<T> to_sublattice(const T* a, int n) {
+  const int h = n / 2;
+  const i64 h3 = (i64)h * h * h;
+  std::vector<T> s((i64)n * n * n);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        int c = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
+        s[c * h3 + (i >> 1) + (i64)h * ((j >> 1) + (i64)h * (k >> 1))] = a[IX(i, j, k, n)];
+      }
+  return s;
+}
+
 // zerosum (gauss1.f90:1420-1466): subtract the FP64 mean.
 template <class T> void zerosum(T* a, int n, int sumorder) {
   const i64 n3 = (i64)n * n * n;
-  double s = canon_sum(a, n3, 1, sumorder);
+  double s;
+  if (sumorder == 2) {
+    std::vector<T> sl = to_sublattice(a, n);
+    s = canon_sum(sl.data(), n3, 1, 1);
+  } else {
+    s = canon_sum(a, n3, 1, sumorder);
+  }
   s = s / (double)n3;
   for (i64 q = 0; q < n3; ++q) a[q] = (T)((double)a[q] - s);
 }

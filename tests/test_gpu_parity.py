@@ -2,7 +2,8 @@
 
 Integer-exact bar for floating point: every kernel keeps the reference's association order and
 the library is built without contraction, so FP64 and FP32 results must be bit-identical to the
-oracle run with the blocked sum order (sumorder=1) that the device reductions use.  Exceptions,
+oracle run with the sum order the device reductions use (sumorder=2: blocked tree sums, with the
+per-level mean removal summing the level arrays in their sub-lattice storage order).  Exceptions,
 each with its tolerance in the test: the limiter's eigenvalue gate uses device acos/cos
 (calcdefp with ncompress > 0: rel 1e-12 FP64 / 1e-5 FP32), and the lognormal generator uses device
 log/cos/exp (rel 1e-12)."""
@@ -96,7 +97,7 @@ def test_multigrid_bitexact(ctx, dt, n, ng, sigma):
     rhs = (1 - rho).astype(dt)
     R.or_nsmooth(rhs, n)
     a, b = np.zeros(n ** 3, dt), rhs.copy()
-    st = R.or_multigrid(a, b, d, u, 5, 1.0, n, 2, ng, 1)
+    st = R.or_multigrid(a, b, d, u, 5, 1.0, n, 2, ng, 2)
     ad, bd = dev(np.zeros(n ** 3, dt)), dev(rhs)
     sd = ctx.multigrid(ad, bd, dev(d), dev(u), 1.0, 2, ng, nu=5)
     assert np.array_equal(host(bd), b)
@@ -113,7 +114,7 @@ def test_calcdefp(ctx, dt, n, sigma, amp):
     u = aos(rho)
     d = smooth_def(n, amp, 4, dt) if amp else np.zeros(n ** 3, dt)
     p, t, t2 = np.zeros(n ** 3, dt), np.zeros(n ** 3, dt), np.zeros((n + 2) * n * n, dt)
-    ls = R.or_calcdefp(p, t, t2, d, u, n, sumorder=1)
+    ls = R.or_calcdefp(p, t, t2, d, u, n, sumorder=2)
     pd, td, t2d = dev(np.zeros(n ** 3, dt)), dev(np.zeros(n ** 3, dt)), dev(np.zeros((n + 2) * n * n, dt))
     lg = ctx.calcdefp(pd, td, t2d, dev(d), dev(u))
     assert lg.ncompress == ls.ncompress and lg.nexpand == ls.nexpand
@@ -210,7 +211,7 @@ def test_solve_fp64_bitexact(ctx, n, sched):
     ctx.limiter_rhs(rd, dev(d), dev(u))
     rhs = host(rd)
     a, b = np.zeros(n ** 3), rhs.copy()
-    rc, rr = R.or_solve(a, b, d, u, 5, n, 1e-8, 100, sched, 2, 2, 16, 0, 1)
+    rc, rr = R.or_solve(a, b, d, u, 5, n, 1e-8, 100, sched, 2, 2, 16, 0, 2)
     assert rc == 0
     ad, bd = dev(np.zeros(n ** 3)), dev(rhs)
     st, ok = ctx.solve(ad, bd, dev(d), dev(u), 1e-8, 100, sched, 2, 2, 16, nu=5)
@@ -238,3 +239,23 @@ def test_solve_mixed_matches_fp64(ctx, n, sched):
     err = np.abs(xm - x64).max() / np.abs(x64).max()
     print("mixed cycles %d fp64 cycles %d rel err %.3e" % (stm.cycles, st64.cycles, err))
     assert err <= 1e-7
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n", [16, 64, 128])
+def test_sublattice_sweep_equals_column_major(ctx, dt, n):
+    print("case sub-lattice sweep n=%d %s" % (n, dt.__name__), flush=True)
+    rho = field(n, 2.0, 3, dt)
+    u = aos(rho)
+    d = smooth_def(n, 2.0, 4, dt)
+    Gd = dev(np.zeros(6 * n ** 3, dt))
+    ctx.metric(Gd, dev(d), dev(u), 1.0, 0, nu=5)
+    rhs = dev((1 - rho).astype(dt))
+    out = []
+    for impl in (0, 1):
+        ctx.set_option("sweep_impl", impl)
+        phi = dev(np.zeros(n ** 3, dt))
+        ctx.time_sweeps(phi, rhs, Gd, 1.0, 3)
+        out.append(host(phi))
+    ctx.set_option("sweep_impl", 0)
+    assert np.array_equal(out[0], out[1])
