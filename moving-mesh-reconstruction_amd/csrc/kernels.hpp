@@ -228,6 +228,8 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
 }
 
 // Residual r = rhs - L[phi] (calcresid, mmh/gauss1.f90:683-709); optional store; per-block max|r|.
+// In layout 1 a thread owns one 2x2x2 cell and evaluates its 8 points, one per sub-lattice, so
+// the 8 class arrays and their neighbours are read in lockstep at unit stride.
 template <class T, int L>
 __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restrict__ bmax, const T* __restrict__ phi,
                                               const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2) {
@@ -235,7 +237,20 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
   const i64 n3 = (i64)n * n * n;
   const i64 pp = (i64)blockIdx.x * RB + threadIdx.x;
   double a = 0.0;
-  if (pp < n3) {
+  const int h = n >> 1;
+  if (L == 1 && pp < (i64)h * h * h) {
+    const int ci = (int)(pp % h), cj = (int)((pp / h) % h), ck = (int)(pp / ((i64)h * h));
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c) {
+      const int i = 2 * ci + (c & 1), j = 2 * cj + ((c >> 1) & 1), k = 2 * ck + (c >> 2);
+      const i64 p = lx<L>(i, j, k, n);
+      T t1, t2, tdiag;
+      stencil19<T, L>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+      T rv = rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
+      if (r) r[p] = rv;
+      a = dmax(a, (double)fabs(rv));
+    }
+  } else if (L == 0 && pp < n3) {
     int i, j, k;
     point_of<L>(pp, n, i, j, k);
     const i64 p = lx<L>(i, j, k, n);
@@ -263,7 +278,20 @@ __global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __r
   const i64 n3 = (i64)n * n * n;
   const i64 pp = (i64)blockIdx.x * RB + threadIdx.x;
   double a = 0.0;
-  if (pp < n3) {
+  const int h = n >> 1;
+  if (L == 1 && pp < (i64)h * h * h) {
+    const int ci = (int)(pp % h), cj = (int)((pp / h) % h), ck = (int)(pp / ((i64)h * h));
+#pragma unroll 1
+    for (int cc = 0; cc < 8; ++cc) {
+      const int i = 2 * ci + (cc & 1), j = 2 * cj + ((cc >> 1) & 1), k = 2 * ck + (cc >> 2);
+      const i64 c = lx<L>(i, j, k, n);
+      T t1, t2, tdiag;
+      stencil19<T, L>(p, G, n3, n, i, j, k, t1, t2, tdiag);
+      T v = (t1 + t2 + tdiag * p[c]) * di2;
+      q[c] = v;
+      a = a + (double)p[c] * (double)v;
+    }
+  } else if (L == 0 && pp < n3) {
     int i, j, k;
     point_of<L>(pp, n, i, j, k);
     const i64 c = lx<L>(i, j, k, n);

@@ -171,6 +171,10 @@ void build_hier(mmh_ctx* c, Hier<T>& H, const std::string& tag, T* u, T* rhs, co
   }
 }
 
+// Grid of the full-field stencil kernels (k_resid, k_apply_dot): one thread per point in layout 0,
+// one per 2x2x2 cell in layout 1.
+inline unsigned stencil_grid(int n, int layout) { return layout == 1 ? nblk(cube(n / 2), RB) : nblk(cube(n), RB); }
+
 template <class T> void relayout(mmh_ctx* c, T* dst, const T* src, int n, int comps, int to_sl) {
   const i64 cnt = cube(n) * comps;
   if (to_sl) k_relayout<T, 1><<<nblk(cnt), 256, 0, c->stream>>>(dst, src, n, comps);
@@ -202,12 +206,11 @@ void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n,
                double* rinf_dev, bool reverse = false) {
   sweeps(c, phi, rhs, G, n, dx, nrelax, 1, reverse);
   if (!want_norm && !rc) return;
-  const i64 n3 = cube(n);
-  const unsigned nb = nblk(n3, RB);
+  const unsigned nb = stencil_grid(n, 1);
   double* bmax = (double*)ws_get(c, "bmax", sizeof(double) * nb);
   k_resid<T, 1><<<nb, RB, 0, c->stream>>>(rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx));
   LAUNCHCHK();
-  reduce<double>(c, bmax, nb, 1, rinf_dev);
+  if (want_norm) reduce<double>(c, bmax, nb, 1, rinf_dev);
   if (rc) {
     const int nrmax = std::min(4, std::max(1, n / 4));
     int nrl = nrelax > nrmax ? (nrelax % nrmax ? nrelax % nrmax : nrmax) : nrelax;
@@ -223,19 +226,19 @@ template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const 
   double* sc = H.scal + 3 * l;
   if (n == 4) {
     const int nc = S.coarse();
-    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, 1, (T*)nullptr, sc + 0);
-    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, 1, (T*)nullptr, sc + 1, S.symmetric);
+    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 0);
+    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 1, S.symmetric);
     reduce<T>(c, H.u[l], n3, 0, sc + 2);
     k_zerosum_guard<T><<<1, 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 2, H.guard + l);
     LAUNCHCHK();
     return;
   }
-  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.pre(n), 1, H.rhs[l + 1], sc + 0);
+  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.pre(n), S.guard, H.rhs[l + 1], sc + 0);
   fill<T>(c, H.u[l + 1], T(0), cube(n / 2));
   vcycle(c, H, l + 1, ng_top, S);
   k_prolong<T, 1><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], H.u[l + 1], n, 1);
   LAUNCHCHK();
-  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.post(n), 1, (T*)nullptr, sc + 1, S.symmetric);
+  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.post(n), S.guard, (T*)nullptr, sc + 1, S.symmetric);
   reduce<T>(c, H.u[l], n3, 0, sc + 2);
   k_zerosum_guard<T><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
                                                       H.guard + l);
@@ -673,12 +676,12 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   }
   const i64 n3 = cube(n);
   double* sc = Hf.scal + 3 * l;
-  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), 1, Hf.rhs[l + 1], sc + 0);
+  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1], sc + 0);
   fill<float>(c, Hf.u[l + 1], 0.f, cube(n / 2));
   vcycle_mixed(c, Hf, Hd, l + 1, ng_top, S);
   k_prolong<float, 1><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
   LAUNCHCHK();
-  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), 1, (float*)nullptr, sc + 1, S.symmetric);
+  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1, S.symmetric);
   reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
   k_zerosum_guard<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
                                                           Hf.guard + l);
@@ -724,7 +727,7 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   double* p = wsT<double>(c, "cg_p", n3);
   double* q = wsT<double>(c, "cg_q", n3);
   double* z = wsT<double>(c, "cg_z", n3);
-  const unsigned nb = nblk(n3, RB);
+  const unsigned nb = nblk(n3, RB), ns = stencil_grid(n, 1);
   double* bm = (double*)ws_get(c, "cg_bm", sizeof(double) * nb);
   double* bs = (double*)ws_get(c, "cg_bs", sizeof(double) * nb);
   Hier<double> P = H;
@@ -743,15 +746,15 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
       LAUNCHCHK();
     }
   };
-  k_resid<double, 1><<<nb, RB, 0, c->stream>>>(r, bm, x, b, H.G[0], n, 1.0);
+  k_resid<double, 1><<<ns, RB, 0, c->stream>>>(r, bm, x, b, H.G[0], n, 1.0);
   LAUNCHCHK();
   precond();
   double rz = dot_dev(c, r, z, n3, sums + 4);
   HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
   for (int it = 0; it < o->max_cycles; ++it) {
-    k_apply_dot<double, 1><<<nb, RB, 0, c->stream>>>(q, bs, p, H.G[0], n, 1.0);
+    k_apply_dot<double, 1><<<ns, RB, 0, c->stream>>>(q, bs, p, H.G[0], n, 1.0);
     LAUNCHCHK();
-    reduce<double>(c, bs, nb, 0, sums + 5);
+    reduce<double>(c, bs, ns, 0, sums + 5);
     const double pq = read_scalar<double>(c, sums + 5);
     const double alpha = rz / pq;
     k_cg_update<<<nb, RB, 0, c->stream>>>(x, r, ro, p, q, alpha, bm, n3);
@@ -770,9 +773,9 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
     rz = rz_new;
   }
   // report the true residual of the final iterate and remove its mean
-  k_resid<double, 1><<<nb, RB, 0, c->stream>>>(nullptr, bm, x, b, H.G[0], n, 1.0);
+  k_resid<double, 1><<<ns, RB, 0, c->stream>>>(nullptr, bm, x, b, H.G[0], n, 1.0);
   LAUNCHCHK();
-  reduce<double>(c, bm, nb, 1, sums + 6);
+  reduce<double>(c, bm, ns, 1, sums + 6);
   if (st && st->cycles > 0) st->relres[st->cycles - 1] = read_scalar<double>(c, sums + 6) / rden;
   reduce<double>(c, x, n3, 0, sums + 2);
   k_zerosum_guard<double><<<nblk(n3), 256, 0, c->stream>>>(x, sums + 2, H.scal, n3, 0, nullptr);
@@ -837,7 +840,7 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
   Hier<float> F;
   build_fp32(c, F, H);
   double* r64 = H.r[0];
-  const unsigned nb = nblk(n3, RB);
+  const unsigned nb = stencil_grid(n, 1);
   double* bmax = (double*)ws_get(c, "sbmax", sizeof(double) * nb);
   for (int cyc = 0; cyc < o->max_cycles; ++cyc) {
     k_resid<double, 1><<<nb, RB, 0, c->stream>>>(r64, bmax, us, rs, H.G[0], n, 1.0);
