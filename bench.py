@@ -79,6 +79,34 @@ def cpu_baseline(n_gpu, cycles, seconds_budget=12.0):
                       % (out["k"], out["t"], n_gpu, cycles)}
 
 
+def traffic_from_profile(n):
+    """HBM bytes per k_colour<double> launch from the committed counter summary, or None."""
+    path = os.path.join(REPO, "profiles", "k_colour_f64_%d_hbm_traffic.txt" % n)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def smoother_only(ctx, torch, defn, u, rhs0, n, sweeps):
+    """k_colour<double> launches (8 per sweep) at n^3, then a plain 1 GiB device-to-device copy whose
+    FETCH_SIZE / WRITE_SIZE give the counters' scale against known bytes."""
+    n3 = n ** 3
+    G = torch.zeros(6 * n3, dtype=torch.float64, device="cuda")
+    ctx.metric(G, defn, u, 1.0, 0, nu=5)
+    phi = torch.zeros(n3, dtype=torch.float64, device="cuda")
+    ms = ctx.time_sweeps(phi, rhs0, G, 1.0, sweeps)
+    del G, phi
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("smoother-only: %d^3 FP64 %.4f ms/sweep (%d sweeps); copy 1 GiB x3 done" % (n, ms, sweeps), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +121,8 @@ def main():
     ap.add_argument("--ref-cycles", type=int, default=3, help="plain reference V-cycles timed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweeps", type=int, default=20, help="sweeps timed for the roofline")
+    ap.add_argument("--smoother-only", action="store_true",
+                    help="only the smoother timing plus a 1 GiB device copy (the counter-calibration run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (--gpus %d)" % args.gpus,
@@ -119,6 +149,9 @@ def main():
     ctx.limiter_rhs(rhs0, defn, u)
     torch.cuda.synchronize()
     print("bench: setup done", file=sys.stderr, flush=True)
+    if args.smoother_only:
+        smoother_only(ctx, torch, defn, u, rhs0, n, args.sweeps)
+        return
 
     def step(mixed):
         phi = torch.zeros(n3, **f64)
@@ -209,7 +242,7 @@ def main():
                    "sweep_ms_fp64": round(ms64, 4), "sweep_ms_fp32": round(ms32, 4),
                    "smoother_alg_GBs_fp32": round(B_ALG_WORDS * 4 * n3 / (ms32 * 1e-3) / 1e9, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(n),
                      "kernel": "k_colour<double> (one of 8 colour classes of a Gauss-Seidel sweep, %d^3)" % n,
                      "alg_bytes_per_launch": alg_bytes_launch, "avg_launch_ms": round(launch_ms, 5)},
         "cpu_baseline": cpu,

@@ -119,8 +119,9 @@ __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __re
   const i64 n3 = (i64)n * n * n;
   const i64 pp = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (pp >= n3) return;
+  // threads walk the column-major inputs in order; the store goes to layout L
   int i, j, k;
-  point_of<L>(pp, n, i, j, k);
+  point_of<0>(pp, n, i, j, k);
   const i64 p = lx<L>(i, j, k, n);
   const int ip = wrapi(i + 1, n), im = wrapi(i - 1, n), jp = wrapi(j + 1, n), jm = wrapi(j - 1, n), kp = wrapi(k + 1, n), km = wrapi(k - 1, n);
   const T di2 = T(1) / (dx * dx);
@@ -235,7 +236,10 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
                                               const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2) {
   __shared__ double s[RB];
   const i64 n3 = (i64)n * n * n;
-  const i64 pp = (i64)blockIdx.x * RB + threadIdx.x;
+  // XCD-aware tile order; the block result is stored at the tile's index so the reduction order
+  // does not depend on the remap
+  const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
+  const i64 pp = (i64)tile * RB + threadIdx.x;
   double a = 0.0;
   const int h = n >> 1;
   if (L == 1 && pp < (i64)h * h * h) {
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
     if ((int)threadIdx.x < h) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + h]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) bmax[blockIdx.x] = s[0];
+  if (threadIdx.x == 0) bmax[tile] = s[0];
 }
 
 // q = L[p] (the operator of k_resid: (t1 + t2 + tdiag*p)/dx^2) and per-block sums of p*q in FP64,
@@ -276,7 +280,10 @@ __global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __r
                                                   const T* __restrict__ G, int n, T di2) {
   __shared__ double s[RB];
   const i64 n3 = (i64)n * n * n;
-  const i64 pp = (i64)blockIdx.x * RB + threadIdx.x;
+  // XCD-aware tile order; the block result is stored at the tile's index so the reduction order
+  // does not depend on the remap
+  const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
+  const i64 pp = (i64)tile * RB + threadIdx.x;
   double a = 0.0;
   const int h = n >> 1;
   if (L == 1 && pp < (i64)h * h * h) {
@@ -307,7 +314,7 @@ __global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __r
     if ((int)threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+  if (threadIdx.x == 0) bsum[tile] = s[0];
 }
 
 // Per-block sums of a*b (FP64), same blocking as k_apply_dot.
@@ -364,8 +371,10 @@ __global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, cons
   const int h = n >> 1;
   const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
+  // coarse points in column-major order: in layout 1 the 27 fine reads of consecutive threads are
+  // then consecutive within each fine class array
   int ci, cj, ck;
-  point_of<L>(q, h, ci, cj, ck);
+  point_of<0>(q, h, ci, cj, ck);
   const int kb = 2 * ck, kbp = 2 * ck + 1, ks = wrapi(2 * ck - 1, n);
   const int ib = 2 * ci, ibp = wrapi(ib + 1, n), ibm = wrapi(ib - 1, n);
   const int jb = 2 * cj, jbp = wrapi(jb + 1, n), jbm = wrapi(jb - 1, n);
@@ -429,8 +438,30 @@ __global__ __launch_bounds__(256) void k_smooth27(T* __restrict__ out, const T* 
 template <class T, int L>
 __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __restrict__ uh, int n, int add) {
   const i64 pp = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pp >= (i64)n * n * n) return;
   const int h = n >> 1;
+  if (L == 1) {
+    // one thread per fine 2x2x2 cell (ci, cj, ck): its 8 points interpolate from the coarse block
+    // {ci, ci+1} x {cj, cj+1} x {ck, ck+1}; each point's 8-term sum keeps the order below
+    if (pp >= (i64)h * h * h) return;
+    int ci, cj, ck;
+    point_of<0>(pp, h, ci, cj, ck);
+    const int c1 = (ci + 1) % h, d1 = (cj + 1) % h, e1 = (ck + 1) % h;
+    T v[2][2][2];
+    v[0][0][0] = uh[lx<1>(ci, cj, ck, h)]; v[0][0][1] = uh[lx<1>(ci, cj, e1, h)];
+    v[0][1][0] = uh[lx<1>(ci, d1, ck, h)]; v[0][1][1] = uh[lx<1>(ci, d1, e1, h)];
+    v[1][0][0] = uh[lx<1>(c1, cj, ck, h)]; v[1][0][1] = uh[lx<1>(c1, cj, e1, h)];
+    v[1][1][0] = uh[lx<1>(c1, d1, ck, h)]; v[1][1][1] = uh[lx<1>(c1, d1, e1, h)];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int a = c & 1, b = (c >> 1) & 1, e = c >> 2;  // i = 2ci+a: low index ci, high index ci+a
+      const T s = (v[0][0][0] + v[0][0][e] + v[0][b][0] + v[0][b][e]
+                   + v[a][b][0] + v[a][b][e] + v[a][0][0] + v[a][0][e]) / T(8);
+      const i64 p = lx<1>(2 * ci + a, 2 * cj + b, 2 * ck + e, n);
+      u[p] = add ? u[p] + s : s;
+    }
+    return;
+  }
+  if (pp >= (i64)n * n * n) return;
   int i, j, k;
   point_of<L>(pp, n, i, j, k);
   const i64 p = lx<L>(i, j, k, n);

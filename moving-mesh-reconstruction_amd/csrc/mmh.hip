@@ -150,6 +150,13 @@ void build_hier(mmh_ctx* c, Hier<T>& H, const std::string& tag, T* u, T* rhs, co
     const std::string t = tag + std::to_string(l);
     if (l == 0) {
       H.u[0] = u; H.rhs[0] = rhs; H.def[0] = defpot; H.u1[0] = ubig; H.nu[0] = nu;
+      if (nu > 1) {
+        // density component of the AoS fluid array, made contiguous once for the metric's box mean
+        T* w = wsT<T>(c, t + "u1", cube(m));
+        k_copy_strided<T><<<nblk(cube(m)), 256, 0, c->stream>>>(w, ubig, cube(m), nu);
+        LAUNCHCHK();
+        H.u1[0] = w; H.nu[0] = 1;
+      }
     } else {
       H.u[l] = wsT<T>(c, t + "u", cube(m));
       H.rhs[l] = wsT<T>(c, t + "rhs", cube(m));
@@ -236,7 +243,7 @@ template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const 
   relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.pre(n), S.guard, H.rhs[l + 1], sc + 0);
   fill<T>(c, H.u[l + 1], T(0), cube(n / 2));
   vcycle(c, H, l + 1, ng_top, S);
-  k_prolong<T, 1><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], H.u[l + 1], n, 1);
+  k_prolong<T, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(H.u[l], H.u[l + 1], n, 1);
   LAUNCHCHK();
   relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.post(n), S.guard, (T*)nullptr, sc + 1, S.symmetric);
   reduce<T>(c, H.u[l], n3, 0, sc + 2);
@@ -679,7 +686,7 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1], sc + 0);
   fill<float>(c, Hf.u[l + 1], 0.f, cube(n / 2));
   vcycle_mixed(c, Hf, Hd, l + 1, ng_top, S);
-  k_prolong<float, 1><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
+  k_prolong<float, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
   LAUNCHCHK();
   relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1, S.symmetric);
   reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
