@@ -71,6 +71,18 @@ typedef struct {
   double rhs_norm;
 } mmh_solve_stats;
 
+/* Per outer iteration: the diagnostics the reference prints in its iteration loop
+ * (mmh/main.f90:94-108; sums in FP64 as there) and calcdefp's limiter record. */
+typedef struct {
+  int iter;
+  double defp_sigma;     /* sqrt(sum defp^2 / N^3) */
+  double def_sigma;      /* sqrt(sum def^2 / N^3) after def += defp */
+  double u_sigma;        /* sqrt(sum (u1 - 1)^2 / N^3) after relaxing */
+  double u_min, u_max;   /* extremes of u1 after relaxing */
+  long long u_negative;  /* count(u1 < 0) */
+  mmh_lim_stats lim;
+} mmh_iter_stats;
+
 /* ---- context ------------------------------------------------------------------------------ */
 int mmh_create(mmh_ctx** ctx, int device);
 int mmh_destroy(mmh_ctx* ctx);
@@ -112,6 +124,15 @@ int mmh_calcdefp_f(mmh_ctx*, float* defp, float* tmp, float* tmp2, const float* 
  * compression gate, smoothing), without the solve. */
 int mmh_limiter_rhs_d(mmh_ctx*, double* tmp, const double* def, const double* u, double dtold, double dtaumesh,
                       int nfluid, int n, double compressmax);
+
+/* The reconstruction iteration (mmh/main.f90:89-110): for iter = 1..niter,
+ * calcdefp(defp, tmp, tmp2, def, u, dtold, dtaumesh, nfluid); def += defp; relaxing(u, def, defp, dt).
+ * u (nfluid = 5 AoS) and def are updated in place, defp holds the last update; st (optional) has
+ * niter entries.  The reference uses dtold = dtaumesh = dt = 1 and compressmax = 10. */
+int mmh_reconstruct_d(mmh_ctx*, double* u, double* def, double* defp, int n, int nfluid, int niter, double dtold,
+                      double dtaumesh, double dt, double compressmax, mmh_iter_stats* st);
+int mmh_reconstruct_f(mmh_ctx*, float* u, float* def, float* defp, int n, int nfluid, int niter, float dtold,
+                      float dtaumesh, float dt, float compressmax, mmh_iter_stats* st);
 
 /* relaxing (mmh/relaxing.f90:21): relaxing-TVD transport of u(5,N,N,N) onto the new mesh. */
 int mmh_relaxing_d(mmh_ctx*, double* u, const double* def, const double* defp, double dt, int nfluid, int n);
@@ -165,6 +186,10 @@ int mmh_gradient_host_f(mmh_ctx*, const float* phi, float* psi, int n);
 int mmh_delta_host_f(mmh_ctx*, const float* phi, float* delta, int n);
 int mmh_nsmooth_host_f(mmh_ctx*, float* a, int n);
 int mmh_lognormal_host_d(mmh_ctx*, double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed);
+/* host form of mmh_reconstruct_f for the Fortran driver: diag (7 x niter, may be NULL) receives per
+ * iteration defp_sigma, def_sigma, u_sigma, u_min, u_max, u_negative, ncompress. */
+int mmh_reconstruct_host_f(mmh_ctx*, float* u, float* def, float* defp, int n, int nfluid, int niter, float dtold,
+                           float dtaumesh, float dt, float compressmax, double* diag);
 
 /* The process-wide context used by the Fortran drop-ins (created on first use on device 0). */
 mmh_ctx* mmh_default_ctx(void);

@@ -57,6 +57,14 @@ def build_fortran():
     tmp = FLIB + ".tmp"
     run([fc, "-shared", "-fPIC"] + objs + ["-o", tmp, "-L" + PKG, "-lmmh_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, FLIB)
+    # the reconstruction driver program (links the module layer and the HIP library)
+    main_src = os.path.join(FSRC, "mmh_main.f90")
+    if os.path.exists(main_src):
+        exe = os.path.join(PKG, "mmh_main")
+        mo = os.path.join(bdir, "mmh_main.o")
+        run([fc, "-O2", "-c", main_src, "-o", mo, "-module-dir", bdir])
+        run([fc, mo, "-o", exe + ".tmp", "-L" + PKG, "-lmmh_fortran", "-lmmh_amd", "-Wl,-rpath,$ORIGIN"])
+        os.replace(exe + ".tmp", exe)
     return True
 
 

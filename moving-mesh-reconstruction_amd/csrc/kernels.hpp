@@ -530,4 +530,41 @@ __global__ void k_entry_zerosum(T* __restrict__ a, const double* __restrict__ su
   a[p] = (T)((double)a[p] - m);
 }
 
+
+// Per-block diagnostics of the outer iteration (mmh/main.f90:97-106), FP64: sum defp^2, sum def^2,
+// sum (u1-1)^2, min u1, max u1, count(u1 < 0).  Block b covers [b*RB*RPT, (b+1)*RB*RPT).
+template <class T>
+__global__ __launch_bounds__(RB) void k_iter_moments(double* __restrict__ out, const T* __restrict__ defp,
+                                                     const T* __restrict__ def, const T* __restrict__ u, int nu, i64 n3) {
+  __shared__ double s[6][RB];
+  double a0 = 0, a1 = 0, a2 = 0, mn = 1e300, mx = -1e300, ng = 0;
+  const i64 base = (i64)blockIdx.x * RB * RPT;
+  for (int r = 0; r < RPT; ++r) {
+    const i64 p = base + (i64)r * RB + threadIdx.x;
+    if (p < n3) {
+      const double dp = (double)defp[p], d = (double)def[p], w = (double)u[(i64)nu * p];
+      a0 += dp * dp;
+      a1 += d * d;
+      a2 += (w - 1.0) * (w - 1.0);
+      mn = w < mn ? w : mn;
+      mx = w > mx ? w : mx;
+      ng += (w < 0.0) ? 1.0 : 0.0;
+    }
+  }
+  s[0][threadIdx.x] = a0; s[1][threadIdx.x] = a1; s[2][threadIdx.x] = a2;
+  s[3][threadIdx.x] = mn; s[4][threadIdx.x] = mx; s[5][threadIdx.x] = ng;
+  __syncthreads();
+  for (int h = RB / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      const int t = threadIdx.x;
+      s[0][t] += s[0][t + h]; s[1][t] += s[1][t + h]; s[2][t] += s[2][t + h];
+      s[3][t] = s[3][t + h] < s[3][t] ? s[3][t + h] : s[3][t];
+      s[4][t] = s[4][t + h] > s[4][t] ? s[4][t + h] : s[4][t];
+      s[5][t] += s[5][t + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[(i64)blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
 }  // namespace mmh

@@ -559,6 +559,44 @@ template <class T> void relaxing_dev(mmh_ctx* c, T* u, const T* def, const T* de
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
+// The reconstruction iteration of mmh/main.f90:89-110 with every field resident on the device.
+template <class T>
+void reconstruct_dev(mmh_ctx* c, T* u, T* def, T* defp, int n, int nf, int niter, T dtold, T dtaumesh, T dt,
+                     T compressmax, mmh_iter_stats* st) {
+  require(is_pow2(n) && n >= 16 && n <= 1024, "reconstruct: n must be a power of two in [16, 1024]");
+  require(nf == 5, "reconstruct: nfluid must be 5 (relaxing.f90:307-310)");
+  require(niter >= 0, "reconstruct: niter < 0");
+  const i64 n3 = cube(n);
+  T* tmp = wsT<T>(c, "rc_tmp", n3);
+  T* tmp2 = wsT<T>(c, "rc_tmp2", (i64)(n + 2) * n * n);
+  const unsigned nb = nblk(n3, RB * RPT);
+  double* mom = (double*)ws_get(c, "rc_mom", sizeof(double) * 6 * nb);
+  std::vector<double> hm(6 * (size_t)nb);
+  for (int it = 0; it < niter; ++it) {
+    mmh_iter_stats* s = st ? st + it : nullptr;
+    calcdefp_dev<T>(c, defp, tmp, tmp2, def, u, dtold, dtaumesh, nf, n, compressmax, n, s ? &s->lim : nullptr);
+    k_add<T><<<nblk(n3), 256, 0, c->stream>>>(def, defp, n3);
+    LAUNCHCHK();
+    relaxing_dev<T>(c, u, def, defp, dt, nf, n);
+    if (!s) continue;
+    k_iter_moments<T><<<nb, RB, 0, c->stream>>>(mom, defp, def, u, nf, n3);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(hm.data(), mom, sizeof(double) * hm.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double a0 = 0, a1 = 0, a2 = 0, mn = 1e300, mx = -1e300, ng = 0;
+    for (unsigned b = 0; b < nb; ++b) {
+      a0 += hm[6 * b]; a1 += hm[6 * b + 1]; a2 += hm[6 * b + 2];
+      mn = std::min(mn, hm[6 * b + 3]); mx = std::max(mx, hm[6 * b + 4]); ng += hm[6 * b + 5];
+    }
+    s->iter = it + 1;
+    s->defp_sigma = std::sqrt(a0 / (double)n3);
+    s->def_sigma = std::sqrt(a1 / (double)n3);
+    s->u_sigma = std::sqrt(a2 / (double)n3);
+    s->u_min = mn; s->u_max = mx; s->u_negative = (long long)ng;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 template <class T> void spectral_out(mmh_ctx* c, const T* phi, T* out, int n, int mode) {
   const i64 n3 = cube(n);
   T* d = wsT<T>(c, "specD", n3);
@@ -1021,6 +1059,10 @@ mmh_ctx* mmh_default_ctx(void) {
   int mmh_relaxing_##S(mmh_ctx* c, T* u, const T* def, const T* defp, T dt, int nf, int n) {                       \
     return guarded(c, [&] { relaxing_dev<T>(c, u, def, defp, dt, nf, n); });                                        \
   }                                                                                                                \
+  int mmh_reconstruct_##S(mmh_ctx* c, T* u, T* def, T* defp, int n, int nf, int niter, T dtold, T dtaumesh, T dt,   \
+                          T cmp, mmh_iter_stats* st) {                                                             \
+    return guarded(c, [&] { reconstruct_dev<T>(c, u, def, defp, n, nf, niter, dtold, dtaumesh, dt, cmp, st); });   \
+  }                                                                                                                \
   int mmh_gradient_##S(mmh_ctx* c, const T* phi, T* psi, int n) {                                                  \
     return guarded(c, [&] {                                                                                        \
       require(is_pow2(n) && n >= 4, "gradient: n must be a power of two >= 4");                                    \
@@ -1200,6 +1242,28 @@ int mmh_lognormal_host_d(mmh_ctx* c, double* rho, int n, double sigma, double sl
     lognormal_dev(c, d, n, sigma, slope, ks_frac, seed);
     down(c, rho, d, cube(n));
     HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_reconstruct_host_f(mmh_ctx* c, float* u, float* def, float* defp, int n, int nf, int niter, float dtold,
+                           float dtaumesh, float dt, float cmp, double* diag) {
+  return guarded(c, [&] {
+    require(is_pow2(n) && nf == 5 && niter >= 0, "reconstruct: bad size");
+    const i64 n3 = cube(n);
+    float* w = up(c, "u", u, 5 * n3);
+    float* d = up(c, "d", def, n3);
+    float* p = up<float>(c, "a", nullptr, n3);
+    std::vector<mmh_iter_stats> st(diag ? niter : 0);
+    reconstruct_dev<float>(c, w, d, p, n, nf, niter, dtold, dtaumesh, dt, cmp, diag ? st.data() : nullptr);
+    down(c, u, w, 5 * n3);
+    down(c, def, d, n3);
+    down(c, defp, p, n3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int it = 0; diag && it < niter; ++it) {
+      double* o = diag + 7 * (size_t)it;
+      o[0] = st[it].defp_sigma; o[1] = st[it].def_sigma; o[2] = st[it].u_sigma;
+      o[3] = st[it].u_min; o[4] = st[it].u_max; o[5] = (double)st[it].u_negative; o[6] = st[it].lim.ncompress;
+    }
   });
 }
 

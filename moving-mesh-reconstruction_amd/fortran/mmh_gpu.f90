@@ -139,6 +139,17 @@ module mmh_gpu
       integer(c_int) :: rc
     end function
 
+    function mmh_reconstruct_host_f(ctx, u, def, defp, n, nfluid, niter, dtold, dtaumesh, dt, compressmax, diag) &
+        bind(C, name='mmh_reconstruct_host_f') result(rc)
+      import :: c_ptr, c_int, c_float, c_double
+      type(c_ptr), value :: ctx
+      real(c_float) :: u(*), def(*), defp(*)
+      integer(c_int), value :: n, nfluid, niter
+      real(c_float), value :: dtold, dtaumesh, dt, compressmax
+      real(c_double) :: diag(*)
+      integer(c_int) :: rc
+    end function
+
     function c_strlen(s) bind(C, name='strlen') result(l)
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s

@@ -43,6 +43,12 @@ class LimStats(C.Structure):
                 ("fluxmax", C.c_double), ("nexpand", C.c_int), ("ncompress", C.c_int), ("mg", MgStats)]
 
 
+class IterStats(C.Structure):
+    """One outer iteration's diagnostics (the values mmh/main.f90:97-106 prints) and limiter record."""
+    _fields_ = [("iter", C.c_int), ("defp_sigma", C.c_double), ("def_sigma", C.c_double), ("u_sigma", C.c_double),
+                ("u_min", C.c_double), ("u_max", C.c_double), ("u_negative", C.c_longlong), ("lim", LimStats)]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_cycles", C.c_int), ("schedule", C.c_int), ("nu_pre", C.c_int),
                 ("nu_post", C.c_int), ("nu_coarse", C.c_int), ("ng", C.c_int), ("mixed", C.c_int),
@@ -60,6 +66,12 @@ def lib():
     """The loaded libmmh_amd.so.  Raises if it has not been built (no fallback path exists)."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process: PyTorch wheels carry their own libamdhip64, so it is loaded
+        # first and the library binds to the same runtime torch's device memory comes from.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise MmhError(MMH_EDEVICE, "libmmh_amd.so not built: run moving-mesh-reconstruction_amd/build.py")
         L = C.CDLL(LIB_PATH)
@@ -175,6 +187,16 @@ class Context:
         n = _cube_side(tmp)
         self.call("mmh_limiter_rhs_d", _ptr(tmp), _ptr(def_), _ptr(u), C.c_double(dtold1), C.c_double(dtaumesh), nfluid,
                   n, C.c_double(compressmax))
+
+    def reconstruct(self, u, def_, defp, niter, dtold=1.0, dtaumesh=1.0, dt=1.0, compressmax=10.0, nfluid=5):
+        """niter outer iterations (calcdefp, def += defp, relaxing; mmh/main.f90:89-110) on device
+        tensors; returns one IterStats per iteration."""
+        s, T = _sfx(u)
+        n = _cube_side(def_)
+        st = (IterStats * max(niter, 1))()
+        self.call("mmh_reconstruct_" + s, _ptr(u), _ptr(def_), _ptr(defp), n, nfluid, niter, T(dtold), T(dtaumesh),
+                  T(dt), T(compressmax), st)
+        return list(st)[:niter]
 
     def relaxing(self, u, def_, defp, cmax=0.0, dt=1.0, nfluidcmp=5):
         s, T = _sfx(u)
