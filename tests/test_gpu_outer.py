@@ -132,3 +132,41 @@ def test_fortran_driver_end_to_end(tmp_path):
     u3, d3, _, _ = oracle_loop(rho, n, 3, np.float32)
     assert np.array_equal(_read(wd + "def32/0.000def_20032.bin", n), d3)
     assert np.array_equal(_read(wd + "u32/0.000restartu0003-_20032.bin", n), u3.reshape(-1, 5)[:, 0])
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_fp64_path_within_1e10_of_reference_order(ctx, n):
+    """Config 2 (256^3, sigma = 2, seed 7) and a 64^3 variant: two outer iterations of the FP64
+    device path against the oracle summing in the reference's own order (sumorder 0, the order of
+    the reference at one thread, bit-identical to the -fdefault-real-8 reference build at 16-64^3).
+    The deformation potential, the mesh displacement (scalar2gradient) and the reconstructed
+    density (defp2delta) agree within 1e-10 relative (max-norm), at the same colouring and cycle
+    count."""
+    print("case fp64 vs reference order n=%d" % n, flush=True)
+    rho = R.or_lognormal(n, 2.0, 7)
+    u = np.zeros((n ** 3, 5))
+    u[:, 0] = rho
+    u = u.ravel().copy()
+    d = np.zeros(n ** 3)
+    for _ in range(2):
+        p, t, t2 = np.zeros(n ** 3), np.zeros(n ** 3), np.zeros((n + 2) * n * n)
+        R.or_calcdefp(p, t, t2, d, u, n, sumorder=0)
+        d += p
+        R.or_relaxing(u, d, p, n)
+    u0 = np.zeros((n ** 3, 5))
+    u0[:, 0] = rho
+    ud, dd, pd = dev(u0.ravel()), dev(np.zeros(n ** 3)), dev(np.zeros(n ** 3))
+    ctx.reconstruct(ud, dd, pd, 2)
+    import torch
+    gd, ld = torch.zeros(3 * n ** 3, dtype=torch.float64, device="cuda:0"), torch.zeros(n ** 3, dtype=torch.float64,
+                                                                                      device="cuda:0")
+    ctx.scalar2gradient(dd, gd)
+    ctx.defp2delta(dd, ld)
+
+    def rel(a, b):
+        return np.abs(a - b).max() / np.abs(b).max()
+
+    e = (rel(host(dd), d), rel(host(gd), R.or_gradient(d, n)), rel(host(ld), R.or_delta(d, n)),
+         rel(host(ud), u))
+    print("rel err def %.2e grad %.2e delta %.2e u %.2e" % e, flush=True)
+    assert max(e) <= 1e-10
