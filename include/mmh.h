@@ -83,6 +83,14 @@ typedef struct {
   mmh_lim_stats lim;
 } mmh_iter_stats;
 
+/* Record of a loss/gradient evaluation (mmh_recon_power_grad_d). */
+typedef struct {
+  double loss;
+  int cycles_forward, cycles_adjoint;      /* PCG iterations of the two solves */
+  double relres_forward, relres_adjoint;   /* their final relative residuals */
+  int ncompress;                           /* compression-limiter cells at the forward point */
+} mmh_adjoint_stats;
+
 /* ---- context ------------------------------------------------------------------------------ */
 int mmh_create(mmh_ctx** ctx, int device);
 int mmh_destroy(mmh_ctx* ctx);
@@ -112,6 +120,11 @@ int mmh_multigrid_f(mmh_ctx*, float* u, float* rhs, const float* defpot, const f
 int mmh_solve_d(mmh_ctx*, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
                 const mmh_solve_opts* opts, mmh_solve_stats* st);
 
+/* Full residual r = rhs - L phi in FP64 (the formula of calcresid, mmh/gauss1.f90:683-709, at every
+ * point; the reference only returns its norm and restriction), with the metric from def and u(1). */
+int mmh_residual_d(mmh_ctx*, double* r, const double* phi, const double* rhs, const double* def, const double* u,
+                   int nu, int n, double dx);
+
 /* calcdefp (mmh/limiter.f90:17): mesh update defp from def and the density u(1,...); tmp (N^3)
  * and tmp2 ((N+2) N^2) are scratch outputs as in the reference. */
 int mmh_calcdefp_d(mmh_ctx*, double* defp, double* tmp, double* tmp2, const double* def, const double* u, double dtold,
@@ -133,6 +146,32 @@ int mmh_reconstruct_d(mmh_ctx*, double* u, double* def, double* defp, int n, int
                       double dtaumesh, double dt, double compressmax, mmh_iter_stats* st);
 int mmh_reconstruct_f(mmh_ctx*, float* u, float* def, float* defp, int n, int nfluid, int niter, float dtold,
                       float dtaumesh, float dt, float compressmax, mmh_iter_stats* st);
+
+/* Binned power spectrum of a real N^3 field (no reference counterpart: the reference reads P(k)
+ * from text files, mmh/calcfisherinfo.f90:26-36).  P_b = sum over modes k in bin b of |F(k)|^2 /
+ * (N_b N^3), F the unnormalised DFT, N_b the mode count; nbins bins of |k| (grid units, k = 2 pi m / N)
+ * with log-spaced edges from 2 pi / N to pi, the last bin closed.  pk, nmodes: host arrays. */
+int mmh_power_d(mmh_ctx*, const double* field, int n, int nbins, double* pk, double* nmodes);
+
+/* Loss and gradient of binned reconstructed power for the first outer iteration (no reference
+ * counterpart).  Forward map at def = 0: the calcdefp right-hand side from rho
+ * (mmh/limiter.f90:134-392), the mesh equation L(rho) phi = rhs solved to tol (FP64 PCG, symmetric
+ * V-cycle preconditioner), defp = nsmooth^2(phi) (mmh/limiter.f90:421-423), delta = defp2delta(defp)
+ * (mmh/reconstruct.f90:1); loss = sum_b w[b] P_b(delta) with P_b as in mmh_power_d.  grad (device,
+ * N^3, may be NULL) receives dloss/drho by the implicit-function adjoint: one adjoint solve of the
+ * same symmetric operator and the analytic sensitivity of the metric G = rhobar I to rho.  Refuses
+ * (MMH_EINVAL) an input where the compression limiter fires.  rho, grad: device; w, pk: host. */
+int mmh_recon_power_grad_d(mmh_ctx*, const double* rho, int n, int nbins, const double* w, double tol, double* pk,
+                           double* grad, mmh_adjoint_stats* st);
+
+/* Fisher information as mmh/calcfisherinfo.f90 computes it from nsim realisations of nk binned
+ * spectra pk[s*nk + b]: cov (nk x nk, column-major) and covnorm = cov / (mean_i mean_j)
+ * (calccov, 280-292), the error bar of each bin's mean (calerror, 269-277), and for every kmax bin
+ * m the cumulative information fisher[m] = c^T inv(covnorm[0..m, 0..m]) c over the first m+1 bins
+ * (120-132), with the inverse by the same Gauss-Jordan elimination (inverse/Upper/Lower,
+ * 294-365).  c (nk) is the derivative vector; host arrays; no device work. */
+int mmh_fisher_d(const double* pk, int nsim, int nk, const double* c, double* cov, double* covnorm, double* fisher,
+                 double* error);
 
 /* relaxing (mmh/relaxing.f90:21): relaxing-TVD transport of u(5,N,N,N) onto the new mesh. */
 int mmh_relaxing_d(mmh_ctx*, double* u, const double* def, const double* defp, double dt, int nfluid, int n);

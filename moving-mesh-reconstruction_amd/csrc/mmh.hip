@@ -21,6 +21,7 @@
 #include "../../include/mmh.h"
 #include "kernels.hpp"
 #include "kernels_lim.hpp"
+#include "kernels_adj.hpp"
 
 using namespace mmh;
 
@@ -913,6 +914,157 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
   throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
 }
 
+// ------------------------------------------------------------------------------------------------
+// Binned power spectrum and the adjoint of the first outer iteration's mesh solve.
+std::vector<double> log_edges(int n, int nb) {
+  std::vector<double> e(nb + 1);
+  const double k0 = 2.0 * M_PI / n, k1 = M_PI;
+  for (int b = 0; b <= nb; ++b) e[b] = k0 * std::exp(std::log(k1 / k0) * (double)b / (double)nb);
+  e[0] = k0;
+  e[nb] = k1;
+  return e;
+}
+
+// Forward NR real FFT of f into (sd, sq) and per-bin sums of mult*|F|^2 (S) and mode counts (N).
+void spectrum_bins(mmh_ctx* c, const double* f, double* sd, double* sq, int n, int nb, const double* edges_dev,
+                   std::vector<double>& S, std::vector<double>& N) {
+  const i64 n3 = cube(n);
+  HIPCHK(hipMemcpyAsync(sd, f, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  fill<double>(c, sq, 0.0, 2 * (i64)n * n);
+  rlft3_dev<double>(c, sd, sq, n, 1);
+  const i64 cnt = (i64)(n / 2 + 1) * n * n;
+  const unsigned nblk_ = nblk(cnt);
+  double* part = (double*)ws_get(c, "pw_part", sizeof(double) * 2 * nb * (size_t)nblk_);
+  k_bin_power<<<nblk_, 256, 0, c->stream>>>(part, sd, sq, n, edges_dev, nb);
+  LAUNCHCHK();
+  std::vector<double> h(2 * (size_t)nb * nblk_);
+  HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  S.assign(nb, 0.0);
+  N.assign(nb, 0.0);
+  for (unsigned b = 0; b < nblk_; ++b)
+    for (int q = 0; q < nb; ++q) {
+      S[q] += h[(size_t)b * 2 * nb + q];
+      N[q] += h[(size_t)b * 2 * nb + nb + q];
+    }
+}
+
+double* upload_edges(mmh_ctx* c, int n, int nb) {
+  const std::vector<double> e = log_edges(n, nb);
+  double* d = (double*)ws_get(c, "pw_edges", sizeof(double) * (nb + 1));
+  HIPCHK(hipMemcpyAsync(d, e.data(), sizeof(double) * e.size(), hipMemcpyHostToDevice, c->stream));
+  return d;
+}
+
+void power_dev(mmh_ctx* c, const double* f, int n, int nb, double* pk, double* nmodes) {
+  require(is_pow2(n) && n >= 4 && n <= 1024, "power: n must be a power of two in [4, 1024]");
+  require(nb >= 1 && nb <= 64, "power: nbins in [1, 64]");
+  const i64 n3 = cube(n);
+  double* sd = wsT<double>(c, "pw_sd", n3);
+  double* sq = wsT<double>(c, "pw_sq", 2 * (i64)n * n);
+  double* e = upload_edges(c, n, nb);
+  std::vector<double> S, N;
+  spectrum_bins(c, f, sd, sq, n, nb, e, S, N);
+  for (int b = 0; b < nb; ++b) {
+    if (pk) pk[b] = N[b] > 0 ? S[b] / (N[b] * (double)n3) : 0.0;
+    if (nmodes) nmodes[b] = N[b];
+  }
+}
+
+void recon_power_dev(mmh_ctx* c, const double* rho, int n, int nb, const double* w, double tol, double* pk_out,
+                     double* grad, mmh_adjoint_stats* st) {
+  require(is_pow2(n) && n >= 16 && n <= 1024, "recon_power: n must be a power of two in [16, 1024]");
+  require(nb >= 1 && nb <= 64 && w != nullptr, "recon_power: nbins in [1, 64] and weights required");
+  require(tol > 0 && tol < 1, "recon_power: tol in (0, 1)");
+  const i64 n3 = cube(n);
+  const double pi = 3.14159265358979323846264338328;
+  double* u = wsT<double>(c, "ad_u", 5 * n3);
+  double* d0 = wsT<double>(c, "ad_def", n3);
+  double* rhs = wsT<double>(c, "ad_rhs", n3);
+  double* phi = wsT<double>(c, "ad_phi", n3);
+  double* w1 = wsT<double>(c, "ad_w1", n3);
+  double* dl = wsT<double>(c, "ad_delta", n3);
+  double* sd = wsT<double>(c, "pw_sd", n3);
+  double* sq = wsT<double>(c, "pw_sq", 2 * (i64)n * n);
+  k_to_aos<double><<<nblk(n3), 256, 0, c->stream>>>(u, rho, n3);
+  LAUNCHCHK();
+  fill<double>(c, d0, 0.0, n3);
+  // forward: rhs, solve, defp = S^2 phi, compression check, delta, binned power
+  std::vector<double> hb;
+  limiter_rhs_dev<double>(c, rhs, d0, u, 1.0, 1.0, 5, n, 10.0, hb);
+  fill<double>(c, phi, 0.0, n3);
+  mmh_solve_opts o{};
+  o.tol = tol; o.max_cycles = 256; o.schedule = 1; o.nu_pre = 2; o.nu_post = 2; o.nu_coarse = 8; o.ng = 0;
+  o.mixed = 0; o.krylov = 1;
+  mmh_solve_stats s1;
+  solve_dev(c, phi, rhs, d0, u, n, 5, &o, &s1);
+  HIPCHK(hipMemcpyAsync(w1, phi, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  nsmooth_dev<double>(c, w1, n, 2);
+  const unsigned nbk = nblk(n3);
+  double* lim = wsT<double>(c, "ad_lim", n3);
+  double* bst = (double*)ws_get(c, "ad_limstat", sizeof(double) * 2 * nbk);
+  k_complim<double><<<nbk, 256, 0, c->stream>>>(lim, w1, d0, n, 1.0, 10.0, pi, bst);
+  LAUNCHCHK();
+  std::vector<double> hc(2 * (size_t)nbk);
+  HIPCHK(hipMemcpyAsync(hc.data(), bst, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  long ncomp = 0;
+  for (unsigned b = 0; b < nbk; ++b) ncomp += (long)hc[2 * b];
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->cycles_forward = s1.cycles;
+    st->relres_forward = s1.cycles > 0 ? s1.relres[s1.cycles - 1] : 0.0;
+    st->ncompress = (int)ncomp;
+  }
+  if (ncomp > 0)
+    throw MmhError(MMH_EINVAL, "recon_power: the compression limiter fires at this input; the gradient is defined "
+                               "only where it is inactive");
+  spectral_out<double>(c, w1, dl, n, 1);
+  double* e = upload_edges(c, n, nb);
+  std::vector<double> S, N;
+  spectrum_bins(c, dl, sd, sq, n, nb, e, S, N);
+  double loss = 0.0;
+  std::vector<double> sym(nb, 0.0);
+  for (int b = 0; b < nb; ++b) {
+    const double p = N[b] > 0 ? S[b] / (N[b] * (double)n3) : 0.0;
+    if (pk_out) pk_out[b] = p;
+    loss += w[b] * p;
+    sym[b] = N[b] > 0 ? 2.0 * w[b] / N[b] : 0.0;  // d loss / d delta = 2 N^3 IFFT(W F), W = w/(N_b N^3)
+  }
+  if (st) st->loss = loss;
+  if (!grad) return;
+  // backward: g_delta = IFFT(sym FFT delta) with sym = 2 w_b / N_b; g_defp = D g_delta; g_phi = S^2 g_defp
+  double* symd = (double*)ws_get(c, "ad_sym", sizeof(double) * nb);
+  HIPCHK(hipMemcpyAsync(symd, sym.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+  const i64 cnt = (i64)(n / 2 + 1) * n * n;
+  k_bin_filter<<<nblk(cnt), 256, 0, c->stream>>>(sd, sq, n, e, nb, symd);
+  LAUNCHCHK();
+  rlft3_dev<double>(c, sd, sq, n, -1);
+  k_scale2<double><<<nblk(n3), 256, 0, c->stream>>>(w1, sd, n, n3);
+  LAUNCHCHK();
+  spectral_out<double>(c, w1, rhs, n, 1);
+  nsmooth_dev<double>(c, rhs, n, 2);
+  // adjoint solve L lambda = Q g_phi (the operator is symmetric; the solve removes the mean)
+  double* lam = dl;
+  fill<double>(c, lam, 0.0, n3);
+  mmh_solve_stats s2;
+  solve_dev(c, lam, rhs, d0, u, n, 5, &o, &s2);
+  if (st) {
+    st->cycles_adjoint = s2.cycles;
+    st->relres_adjoint = s2.cycles > 0 ? s2.relres[s2.cycles - 1] : 0.0;
+  }
+  // through the metric: g_rhobar, then the 27-point box mean's adjoint
+  k_adj_rhobar<<<nblk(n3), 256, 0, c->stream>>>(w1, lam, phi, n, 1.0);
+  k_box27_adj<<<nblk(n3), 256, 0, c->stream>>>(grad, w1, n);
+  LAUNCHCHK();
+  // through the right-hand side: S^4 lambda, the clip mask, 1/30
+  HIPCHK(hipMemcpyAsync(w1, lam, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  nsmooth_dev<double>(c, w1, n, 4);
+  k_rhs_adj<<<nblk(n3), 256, 0, c->stream>>>(grad, w1, rho, n3, 1.0);
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 // Times ns sweeps of the multigrid smoother on copies of phi, rhs and G (column-major on entry)
 // in the layout the V-cycle uses; phi receives the result.
 template <class T> void time_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, float* ms) {
@@ -1132,6 +1284,91 @@ int mmh_limiter_rhs_d(mmh_ctx* c, double* tmp, const double* def, const double* 
     limiter_rhs_dev<double>(c, tmp, def, u, dtold < 1.e-8 ? 1.0 : dtold, dtaumesh, nf, n, compressmax, hb);
     HIPCHK(hipStreamSynchronize(c->stream));
   });
+}
+
+int mmh_residual_d(mmh_ctx* c, double* r, const double* phi, const double* rhs, const double* def, const double* u,
+                   int nu, int n, double dx) {
+  return guarded(c, [&] {
+    require(is_pow2(n) && n >= 4 && n <= 1024 && nu >= 1, "residual: bad size");
+    const i64 n3 = cube(n);
+    double* G = wsT<double>(c, "res_G", 6 * n3);
+    k_metric<double, 0><<<nblk(n3), 256, 0, c->stream>>>(G, def, u, nu, n, dx, 0);
+    const unsigned nb = stencil_grid(n, 0);
+    double* bm = (double*)ws_get(c, "res_bm", sizeof(double) * nb);
+    k_resid<double, 0><<<nb, RB, 0, c->stream>>>(r, bm, phi, rhs, G, n, 1.0 / (dx * dx));
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_power_d(mmh_ctx* c, const double* field, int n, int nbins, double* pk, double* nmodes) {
+  return guarded(c, [&] { power_dev(c, field, n, nbins, pk, nmodes); });
+}
+
+int mmh_recon_power_grad_d(mmh_ctx* c, const double* rho, int n, int nbins, const double* w, double tol, double* pk,
+                           double* grad, mmh_adjoint_stats* st) {
+  return guarded(c, [&] { recon_power_dev(c, rho, n, nbins, w, tol, pk, grad, st); });
+}
+
+int mmh_fisher_d(const double* pk, int nsim, int nk, const double* cv, double* cov, double* covnorm, double* fisher,
+                 double* error) {
+  if (!pk || !cv || nsim < 2 || nk < 1) return MMH_EINVAL;
+  auto P = [&](int s, int b) { return pk[(size_t)s * nk + b]; };
+  std::vector<double> sums(nk, 0.0);
+  for (int b = 0; b < nk; ++b)
+    for (int s = 0; s < nsim; ++s) sums[b] += P(s, b);
+  std::vector<double> cv_(nk * (size_t)nk), cn(nk * (size_t)nk);
+  // calccov: centred series, dot product / (nsim - 1); normalised by the means
+  for (int y = 0; y < nk; ++y)
+    for (int x = 0; x < nk; ++x) {
+      const double m1 = sums[x] / nsim, m2 = sums[y] / nsim;
+      double d = 0.0;
+      for (int s = 0; s < nsim; ++s) d += (P(s, x) - m1) * (P(s, y) - m2);
+      const double c12 = d / (nsim - 1);
+      cv_[x + (size_t)nk * y] = c12;
+      cn[x + (size_t)nk * y] = c12 / sums[x] / sums[y] * (double)(nsim * nsim);
+    }
+  if (cov) std::copy(cv_.begin(), cv_.end(), cov);
+  if (covnorm) std::copy(cn.begin(), cn.end(), covnorm);
+  if (error) {
+    // calerror: standard error of the mean
+    for (int b = 0; b < nk; ++b) {
+      const double m = sums[b] / nsim;
+      double q = 0.0;
+      for (int s = 0; s < nsim; ++s) q += (P(s, b) - m) * (P(s, b) - m);
+      // sqrt(real(n_sim)) is a default-REAL (single precision) square root in calerror
+      error[b] = std::sqrt(q / (nsim - 1)) / (double)std::sqrt((float)nsim);
+    }
+  }
+  if (fisher) {
+    for (int m = 1; m <= nk; ++m) {
+      // Gauss-Jordan without pivoting on the leading m x m block: eliminate below, then above
+      std::vector<double> B(m * (size_t)m), I(m * (size_t)m, 0.0);
+      for (int jj = 0; jj < m; ++jj)
+        for (int ii = 0; ii < m; ++ii) B[ii + (size_t)m * jj] = cn[ii + (size_t)nk * jj];
+      for (int ii = 0; ii < m; ++ii) I[ii + (size_t)m * ii] = 1.0;
+      auto row_op = [&](int J, int Irow, double E, int c0) {
+        for (int col = c0; col < m; ++col) B[J + (size_t)m * col] = B[J + (size_t)m * col] - B[Irow + (size_t)m * col] * E;
+        for (int col = 0; col < m; ++col) I[J + (size_t)m * col] = I[J + (size_t)m * col] - I[Irow + (size_t)m * col] * E;
+      };
+      for (int Ii = 0; Ii < m - 1; ++Ii)
+        for (int J = Ii + 1; J < m; ++J) row_op(J, Ii, B[J + (size_t)m * Ii] / B[Ii + (size_t)m * Ii], Ii);
+      for (int Ii = m - 1; Ii >= 1; --Ii)
+        for (int J = Ii - 1; J >= 0; --J) row_op(J, Ii, B[J + (size_t)m * Ii] / B[Ii + (size_t)m * Ii], 0);
+      for (int ii = 0; ii < m; ++ii) {
+        const double dg = B[ii + (size_t)m * ii];
+        for (int col = 0; col < m; ++col) I[ii + (size_t)m * col] = I[ii + (size_t)m * col] / dg;
+      }
+      // fishinfo = sum_j (sum_i inv(i,j) c_i) c_j
+      double f = 0.0;
+      std::vector<double> t(m, 0.0);
+      for (int jj = 0; jj < m; ++jj)
+        for (int ii = 0; ii < m; ++ii) t[jj] = t[jj] + I[ii + (size_t)m * jj] * cv[ii];
+      for (int jj = 0; jj < m; ++jj) f = f + t[jj] * cv[jj];
+      fisher[m - 1] = f;
+    }
+  }
+  return MMH_OK;
 }
 
 int mmh_rlft3_d(mmh_ctx* c, double* data, double* speq, int n, int isign) {

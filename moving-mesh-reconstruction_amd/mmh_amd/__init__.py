@@ -49,6 +49,11 @@ class IterStats(C.Structure):
                 ("u_min", C.c_double), ("u_max", C.c_double), ("u_negative", C.c_longlong), ("lim", LimStats)]
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("loss", C.c_double), ("cycles_forward", C.c_int), ("cycles_adjoint", C.c_int),
+                ("relres_forward", C.c_double), ("relres_adjoint", C.c_double), ("ncompress", C.c_int)]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_cycles", C.c_int), ("schedule", C.c_int), ("nu_pre", C.c_int),
                 ("nu_post", C.c_int), ("nu_coarse", C.c_int), ("ng", C.c_int), ("mixed", C.c_int),
@@ -198,6 +203,32 @@ class Context:
                   T(dt), T(compressmax), st)
         return list(st)[:niter]
 
+    def residual(self, r, phi, rhs, def_, u, dx=1.0, nu=None):
+        """r = rhs - L(def, u) phi at every point, FP64 (calcresid's formula)."""
+        n = _cube_side(phi)
+        nu = nu or u.numel() // phi.numel()
+        self.call("mmh_residual_d", _ptr(r), _ptr(phi), _ptr(rhs), _ptr(def_), _ptr(u), nu, n, C.c_double(dx))
+
+    def power(self, field, nbins=16):
+        """Binned power spectrum of an N^3 FP64 device field: (P_b, mode counts) as numpy arrays."""
+        import numpy as np
+        n = _cube_side(field)
+        pk, nm = np.zeros(nbins), np.zeros(nbins)
+        self.call("mmh_power_d", _ptr(field), n, nbins, pk.ctypes.data_as(C.c_void_p), nm.ctypes.data_as(C.c_void_p))
+        return pk, nm
+
+    def recon_power_grad(self, rho, weights, grad=None, tol=1e-12):
+        """loss = sum_b w_b P_b(delta_rec(rho)) for the first outer iteration, its binned spectrum
+        and (if grad is a device tensor) dloss/drho written into grad.  Returns (loss, pk, stats)."""
+        import numpy as np
+        n = _cube_side(rho)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        pk = np.zeros(w.size)
+        st = AdjointStats()
+        self.call("mmh_recon_power_grad_d", _ptr(rho), n, w.size, w.ctypes.data_as(C.c_void_p), C.c_double(tol),
+                  pk.ctypes.data_as(C.c_void_p), _ptr(grad), C.byref(st))
+        return st.loss, pk, st
+
     def relaxing(self, u, def_, defp, cmax=0.0, dt=1.0, nfluidcmp=5):
         s, T = _sfx(u)
         n = _cube_side(def_)
@@ -241,6 +272,24 @@ class Context:
     def lognormal(self, rho, sigma, seed, slope=-1.5, ks_frac=0.5):
         self.call("mmh_lognormal_d", _ptr(rho), _cube_side(rho), C.c_double(sigma), C.c_double(slope),
                   C.c_double(ks_frac), C.c_uint64(seed))
+
+
+def fisher(pk, c):
+    """calcfisherinfo-style statistics of nsim x nk binned spectra pk and derivative vector c:
+    dict with cov, covnorm (nk x nk), error (nk) and the cumulative Fisher information (nk).
+    Host computation (mmh_fisher_d); needs no device."""
+    import numpy as np
+    pk = np.ascontiguousarray(pk, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    nsim, nk = pk.shape
+    cov, covn = np.zeros(nk * nk), np.zeros(nk * nk)
+    fis, err = np.zeros(nk), np.zeros(nk)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    rc = lib().mmh_fisher_d(vp(pk), nsim, nk, vp(c), vp(cov), vp(covn), vp(fis), vp(err))
+    if rc != MMH_OK:
+        raise MmhError(rc, "mmh_fisher_d: bad arguments")
+    return {"cov": cov.reshape(nk, nk, order="F"), "covnorm": covn.reshape(nk, nk, order="F"), "fisher": fis,
+            "error": err}
 
 
 def header_symbols(header=None):

@@ -10,6 +10,8 @@
 # Outputs (git-ignored): oracle/_ref/libmmhref_ng<NG>_f32.so   FP32 as shipped (-O2, serial)
 #                        oracle/_ref/libmmhref_ng<NG>_f64.so   -fdefault-real-8 promotion
 #                        oracle/_ref/libmmhref_ng<NG>_f32omp.so FP32, -O3 -fopenmp (CPU baseline)
+#                        oracle/_ref/libmmhref_fisher.so      calcfisherinfo.f90 (its calccov,
+#                        calerror and inverse subroutines are called; the program itself is not run)
 set -euo pipefail
 REF=${MMH_REFERENCE_DIR:-/root/reference}/mmh
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -41,3 +43,10 @@ for NG in $NGS; do
     echo "built $OUT/libmmhref_ng${NG}_$V.so"
   done
 done
+# the Fisher-matrix statistics (independent of NG)
+B=$OUT/build/fisher
+rm -rf "$B"; mkdir -p "$B"
+ln -s "$REF/calcfisherinfo.f90" "$B/calcfisherinfo.f90"
+(cd "$B" && $FC -O2 -fPIC -w -c calcfisherinfo.f90 -o calcfisherinfo.o)
+$FC -shared -fPIC "$B/calcfisherinfo.o" -o "$OUT/libmmhref_fisher.so"
+echo "built $OUT/libmmhref_fisher.so"
