@@ -861,6 +861,12 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
     relayout(c, u, us, n, 1, 0);
     HIPCHK(hipStreamSynchronize(c->stream));
   };
+  if (rn == 0.0) {
+    // L u = 0 has the mean-free solution 0
+    fill<double>(c, u, 0.0, n3);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return;
+  }
   if (o->krylov) {
     Hier<float> F;
     if (o->mixed) build_fp32(c, F, H);
