@@ -125,6 +125,25 @@ int mmh_solve_d(mmh_ctx*, double* u, double* rhs, const double* defpot, const do
 int mmh_residual_d(mmh_ctx*, double* r, const double* phi, const double* rhs, const double* def, const double* u,
                    int nu, int n, double dx);
 
+/* Per-realisation record of mmh_solve_batch_d. */
+typedef struct {
+  int cycles;     /* PCG iterations */
+  double relres;  /* final max|rhs - L u| / max|rhs| */
+  int converged;
+} mmh_batch_stats;
+
+/* Many realisations solved together (no reference counterpart: the reference loops one process
+ * per realisation, mmh/run100multigrid.sh).  Item i is the system of mmh_solve_d on device arrays
+ * u[i], rhs[i] (projected in place), defpot[i], ubig[i] of size n[i]^3; sizes may differ and are
+ * grouped.  Each group advances with one launch per stage for all its grids, one PCG iteration
+ * replayed as a HIP graph, and colour-kernel launch shapes tuned on first use.  Only the FP64
+ * Krylov form (opts: krylov = 1, schedule = 1, nu_pre = nu_post, mixed = 0); every item's result
+ * is bit-identical to its mmh_solve_d.  Options "batch_graph" and "batch_autotune" (mmh_set_option)
+ * switch graph replay and tuning off. */
+int mmh_solve_batch_d(mmh_ctx*, int count, const int* n, double* const* u, double* const* rhs,
+                      double* const* defpot, double* const* ubig, int nu, const mmh_solve_opts* opts,
+                      mmh_batch_stats* st);
+
 /* calcdefp (mmh/limiter.f90:17): mesh update defp from def and the density u(1,...); tmp (N^3)
  * and tmp2 ((N+2) N^2) are scratch outputs as in the reference. */
 int mmh_calcdefp_d(mmh_ctx*, double* defp, double* tmp, double* tmp2, const double* def, const double* u, double dtold,

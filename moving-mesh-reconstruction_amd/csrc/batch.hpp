@@ -1,0 +1,382 @@
+// batch.hpp -- many mesh solves in one pass: realisations of equal size are stacked into batched
+// level arrays and advanced together, one kernel launch per stage for the whole stack, with one
+// flexible-PCG iteration replayed as a captured HIP graph.  Included by mmh.hip (host side).
+//
+// Each realisation follows exactly the arithmetic of the single solve (mmh_solve_d with the
+// Krylov option): same kernels, same per-grid reduction order, per-grid CG scalars computed on the
+// device.  A realisation that reaches the tolerance gets alpha = beta = 0 from then on, which
+// freezes its iterate while the others continue, so every result is bit-identical to its single
+// solve.  The run100multigrid.sh workflow of the reference (one process per realisation, serial)
+// becomes one call.
+
+// Level arrays of B realisations of size n, stacked per level: field (l, s) at base[l] + s * m^3.
+struct BatchGroup {
+  int n = 0, B = 0, nlev = 0;
+  std::vector<int> items;  // indices into the caller's list
+  int m[MMH_MAXLEV];
+  double dx[MMH_MAXLEV];
+  double* u[MMH_MAXLEV];
+  double* rhs[MMH_MAXLEV];
+  double* r[MMH_MAXLEV];
+  double* G[MMH_MAXLEV];
+  // PCG vectors at the finest level (sub-lattice layout), B n^3 each
+  double *x, *b, *cr, *ro, *p, *q, *z;
+  // scratch of the per-grid reductions, and per-grid scalars
+  double *part, *part2, *bm, *bs;
+  double *sums, *rz, *pq, *rmax, *rden, *rznew, *zro, *alpha, *beta, *relres;
+  int *active, *iters;
+  std::string tag;
+};
+
+void batch_reduce(mmh_ctx* c, BatchGroup& g, const double* x, i64 n, int op, double* dst) {
+  // per grid: the staging of reduce() (blocks of RB*RPT, halving tree), grids side by side
+  const i64 ch = (i64)RB * RPT;
+  i64 nb = (n + ch - 1) / ch;
+  const unsigned B = g.B;
+  if (nb <= 1) {
+    if (op == 0) k_reduce_stage<double, 0><<<dim3(1, B), RB, 0, c->stream>>>(dst, x, n, n);
+    else k_reduce_stage<double, 1><<<dim3(1, B), RB, 0, c->stream>>>(dst, x, n, n);
+    LAUNCHCHK();
+    return;
+  }
+  double* a = g.part;
+  double* bb = g.part2;
+  if (op == 0) k_reduce_stage<double, 0><<<dim3((unsigned)nb, B), RB, 0, c->stream>>>(a, x, n, n);
+  else k_reduce_stage<double, 1><<<dim3((unsigned)nb, B), RB, 0, c->stream>>>(a, x, n, n);
+  LAUNCHCHK();
+  while (nb > 1) {
+    const i64 nb2 = (nb + ch - 1) / ch;
+    double* out = nb2 == 1 ? dst : bb;
+    if (op == 0) k_reduce_stage<double, 0><<<dim3((unsigned)nb2, B), RB, 0, c->stream>>>(out, a, nb, nb);
+    else k_reduce_stage<double, 1><<<dim3((unsigned)nb2, B), RB, 0, c->stream>>>(out, a, nb, nb);
+    LAUNCHCHK();
+    std::swap(a, bb);
+    nb = nb2;
+  }
+}
+
+std::pair<int, int> batch_shape(mmh_ctx* c, int m, int B) {
+  auto it = c->tune.find(std::to_string(m) + ":" + std::to_string(B));
+  return it == c->tune.end() ? std::make_pair(256, 1) : it->second;
+}
+
+void batch_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, const double* G, int l, int ns, bool reverse,
+                  std::pair<int, int> shape) {
+  const int m = g.m[l];
+  const i64 h3 = cube(m / 2);
+  const double dx2 = g.dx[l] * g.dx[l];
+  const int bsz = shape.first;
+  const dim3 grid(nblk(h3, bsz), g.B);
+  for (int sw = 0; sw < ns; ++sw)
+    for (int cc = 0; cc < 8; ++cc) {
+      const int col = reverse ? 7 - cc : cc;
+      k_colour<double, 1><<<grid, bsz, 0, c->stream>>>(phi, rhs, G, m, dx2, col >> 2, (col >> 1) & 1, col & 1, cube(m),
+                                                       shape.second);
+    }
+  LAUNCHCHK();
+}
+
+// Symmetric V(nu, nu) cycle of the batch from level l, the arithmetic of vcycle<double> with the
+// Krylov schedule (no guard, reverse colour order on the way up, mean removal on every level).
+void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, const Sched& S) {
+  const int m = g.m[l];
+  const i64 m3 = cube(m);
+  double* u = l == 0 ? u0 : g.u[l];
+  double* rh = l == 0 ? rhs0 : g.rhs[l];
+  const auto shape = batch_shape(c, m, g.B);
+  if (m == 4) {
+    batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), false, shape);
+    batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), S.symmetric, shape);
+    batch_reduce(c, g, u, m3, 0, g.sums);
+    k_zerosum_guard<double><<<dim3(1, g.B), 256, 0, c->stream>>>(u, g.sums, nullptr, m3, 0, nullptr, 1);
+    LAUNCHCHK();
+    return;
+  }
+  const int pre = S.pre(m);
+  batch_sweeps(c, g, u, rh, g.G[l], l, pre, false, shape);
+  const unsigned ns = stencil_grid(m, 1);
+  k_resid<double, 1><<<dim3(ns, g.B), RB, 0, c->stream>>>(g.r[l], g.bm, u, rh, g.G[l], m, 1.0 / (g.dx[l] * g.dx[l]), 1);
+  const int nrmax = std::min(4, std::max(1, m / 4));
+  const int nrl = pre > nrmax ? (pre % nrmax ? pre % nrmax : nrmax) : pre;
+  k_restrict_resid<double, 1><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(g.rhs[l + 1], g.r[l], m,
+                                                                                   nrl % (m / 2), 1);
+  k_fill<double><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(g.u[l + 1], 0.0, cube(m / 2), cube(m / 2));
+  LAUNCHCHK();
+  batch_vcycle(c, g, l + 1, u0, rhs0, S);
+  k_prolong<double, 1><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(u, g.u[l + 1], m, 1, 1);
+  LAUNCHCHK();
+  batch_sweeps(c, g, u, rh, g.G[l], l, S.post(m), S.symmetric, shape);
+  batch_reduce(c, g, u, m3, 0, g.sums);
+  k_zerosum_guard<double><<<dim3(nblk(m3), g.B), 256, 0, c->stream>>>(u, g.sums, nullptr, m3, 0, nullptr, 1);
+  LAUNCHCHK();
+}
+
+void batch_dot(mmh_ctx* c, BatchGroup& g, const double* a, const double* b, double* dst) {
+  const i64 n3 = cube(g.n);
+  const unsigned nb = nblk(n3, RB);
+  k_dot<double><<<dim3(nb, g.B), RB, 0, c->stream>>>(g.bs, a, b, n3, n3);
+  LAUNCHCHK();
+  batch_reduce(c, g, g.bs, nb, 0, dst);
+}
+
+void batch_precond(mmh_ctx* c, BatchGroup& g, const Sched& S) {
+  const i64 n3 = cube(g.n);
+  k_fill<double><<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.z, 0.0, n3, n3);
+  LAUNCHCHK();
+  batch_vcycle(c, g, 0, g.z, g.cr, S);
+}
+
+// One flexible-PCG iteration for every grid of the batch (the loop body of pcg()).
+void batch_iteration(mmh_ctx* c, BatchGroup& g, const Sched& S, double tol) {
+  const i64 n3 = cube(g.n);
+  const unsigned ns = stencil_grid(g.n, 1), nb = nblk(n3, RB), sb = nblk(g.B, 64);
+  k_apply_dot<double, 1><<<dim3(ns, g.B), RB, 0, c->stream>>>(g.q, g.bs, g.p, g.G[0], g.n, 1.0, 1);
+  LAUNCHCHK();
+  batch_reduce(c, g, g.bs, ns, 0, g.pq);
+  k_cg_scalars<<<sb, 64, 0, c->stream>>>(0, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
+                                         g.active, g.iters, g.relres);
+  k_cg_update_b<<<dim3(nb, g.B), RB, 0, c->stream>>>(g.x, g.cr, g.ro, g.p, g.q, g.alpha, g.bm, n3);
+  LAUNCHCHK();
+  batch_reduce(c, g, g.bm, nb, 1, g.rmax);
+  k_cg_scalars<<<sb, 64, 0, c->stream>>>(1, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
+                                         g.active, g.iters, g.relres);
+  batch_precond(c, g, S);
+  batch_dot(c, g, g.z, g.cr, g.rznew);
+  batch_dot(c, g, g.z, g.ro, g.zro);
+  k_cg_scalars<<<sb, 64, 0, c->stream>>>(2, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
+                                         g.active, g.iters, g.relres);
+  k_cg_dir_b<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.p, g.z, g.beta, n3);
+  LAUNCHCHK();
+}
+
+// Chooses the colour kernel's block size and XCD remap per level size by timing batched sweeps.
+void batch_autotune(mmh_ctx* c, BatchGroup& g) {
+  const int cand_b[3] = {64, 128, 256};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int l = 0; l < g.nlev; ++l) {
+    const int m = g.m[l];
+    const std::string key = std::to_string(m) + ":" + std::to_string(g.B);
+    if (c->tune.count(key)) continue;
+    if (m < 16) { c->tune[key] = {64, 0}; continue; }
+    double* phi = l == 0 ? g.z : g.u[l];
+    const double* rh = l == 0 ? g.cr : g.rhs[l];
+    float best = 1e30f;
+    std::pair<int, int> pick{256, 1};
+    for (int bi = 0; bi < 3; ++bi)
+      for (int rm = 0; rm < 2; ++rm) {
+        const std::pair<int, int> sh{cand_b[bi], rm};
+        batch_sweeps(c, g, phi, rh, g.G[l], l, 1, false, sh);  // warm
+        HIPCHK(hipEventRecord(e0, c->stream));
+        batch_sweeps(c, g, phi, rh, g.G[l], l, 2, false, sh);
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; pick = sh; }
+      }
+    c->tune[key] = pick;
+  }
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+}
+
+void batch_alloc(mmh_ctx* c, BatchGroup& g) {
+  const std::string t = "bg" + std::to_string(g.n) + "x" + std::to_string(g.B) + "_";
+  g.tag = t;
+  int nl = 0;
+  for (int m = g.n; m >= 4; m >>= 1) ++nl;
+  g.nlev = nl;
+  const i64 B = g.B;
+  for (int l = 0; l < nl; ++l) {
+    const int m = g.n >> l;
+    g.m[l] = m;
+    g.dx[l] = l == 0 ? 1.0 : 2.0 * g.dx[l - 1];
+    const std::string tl = t + std::to_string(l);
+    g.u[l] = l == 0 ? nullptr : wsT<double>(c, tl + "u", B * cube(m));
+    g.rhs[l] = l == 0 ? nullptr : wsT<double>(c, tl + "rhs", B * cube(m));
+    g.r[l] = wsT<double>(c, tl + "r", B * cube(m));
+    g.G[l] = wsT<double>(c, tl + "G", B * 6 * cube(m));
+  }
+  const i64 n3 = cube(g.n);
+  g.x = wsT<double>(c, t + "x", B * n3);
+  g.b = wsT<double>(c, t + "b", B * n3);
+  g.cr = wsT<double>(c, t + "cr", B * n3);
+  g.ro = wsT<double>(c, t + "ro", B * n3);
+  g.p = wsT<double>(c, t + "p", B * n3);
+  g.q = wsT<double>(c, t + "q", B * n3);
+  g.z = wsT<double>(c, t + "z", B * n3);
+  const i64 nbmax = nblk(n3, RB);
+  g.part = wsT<double>(c, t + "part", B * nbmax);
+  g.part2 = wsT<double>(c, t + "part2", B * nbmax);
+  g.bm = wsT<double>(c, t + "bm", B * nbmax);
+  g.bs = wsT<double>(c, t + "bs", B * nbmax);
+  double* sc = wsT<double>(c, t + "scal", 10 * B);
+  g.sums = sc; g.rz = sc + B; g.pq = sc + 2 * B; g.rmax = sc + 3 * B; g.rden = sc + 4 * B;
+  g.rznew = sc + 5 * B; g.zro = sc + 6 * B; g.alpha = sc + 7 * B; g.beta = sc + 8 * B; g.relres = sc + 9 * B;
+  int* ic = (int*)ws_get(c, t + "flags", sizeof(int) * 2 * B);
+  g.active = ic; g.iters = ic + B;
+}
+
+// Level data of item s: the metric of every level from def and u(1) (build_hier's arithmetic),
+// the projected right-hand side and the initial guess in sub-lattice order.  Returns max|rhs|.
+double batch_setup_item(mmh_ctx* c, BatchGroup& g, int s, double* u_user, double* rhs_user, const double* def,
+                        const double* ubig, int nu) {
+  const int n = g.n;
+  const i64 n3 = cube(n);
+  const double* dl = def;
+  const double* wl = ubig;
+  int nul = nu;
+  if (nu > 1) {
+    double* w = wsT<double>(c, g.tag + "u1c", n3);
+    k_copy_strided<double><<<nblk(n3), 256, 0, c->stream>>>(w, ubig, n3, nu);
+    LAUNCHCHK();
+    wl = w;
+    nul = 1;
+  }
+  for (int l = 0; l < g.nlev; ++l) {
+    const int m = g.m[l];
+    if (l > 0) {
+      double* d2 = wsT<double>(c, g.tag + "d" + std::to_string(l), cube(m));
+      double* w2 = wsT<double>(c, g.tag + "w" + std::to_string(l), cube(m));
+      k_restrict27<double><<<nblk(cube(m)), 256, 0, c->stream>>>(d2, dl, 2 * m, 1);
+      k_restrict27<double><<<nblk(cube(m)), 256, 0, c->stream>>>(w2, wl, 2 * m, nul);
+      LAUNCHCHK();
+      dl = d2;
+      wl = w2;
+      nul = 1;
+    }
+    k_metric<double, 1><<<nblk(cube(m)), 256, 0, c->stream>>>(g.G[l] + (i64)s * 6 * cube(m), dl, wl, nul, m, g.dx[l], 0);
+    LAUNCHCHK();
+  }
+  double* sums = wsT<double>(c, g.tag + "setup_sums", 8);
+  rgzerosum_dev<double>(c, rhs_user, def, 1.0, n, sums);
+  reduce<double>(c, rhs_user, n3, 1, sums + 3);
+  relayout(c, g.x + (i64)s * n3, u_user, n, 1, 1);
+  relayout(c, g.b + (i64)s * n3, rhs_user, n, 1, 1);
+  return read_scalar<double>(c, sums + 3);
+}
+
+void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, double* const* rhs,
+                     double* const* defpot, double* const* ubig, int nu, const mmh_solve_opts* o,
+                     mmh_batch_stats* st) {
+  require(count >= 1 && ns && u && rhs && defpot && ubig, "solve_batch: bad arguments");
+  require(o != nullptr && o->krylov == 1 && o->schedule == 1 && o->nu_pre == o->nu_post && o->mixed == 0,
+          "solve_batch: FP64 Krylov solve with a fixed symmetric schedule only");
+  require(o->max_cycles >= 1 && o->max_cycles <= 256 && o->tol > 0, "solve_batch: max_cycles in [1, 256], tol > 0");
+  require(nu >= 1, "solve_batch: nu >= 1");
+  Sched S;
+  S.kind = 1; S.nu_pre = o->nu_pre; S.nu_post = o->nu_post; S.nu_coarse = o->nu_coarse;
+  S.guard = false; S.symmetric = true;
+  require(S.nu_pre >= 1 && S.nu_coarse >= 1, "solve_batch: sweep counts >= 1");
+  // Graph capture needs a stream other than the null stream: the batch runs on its own stream
+  // (default priority), ordered after the work already queued on the context's stream.
+  struct StreamSwap {
+    mmh_ctx* c;
+    hipStream_t saved, own = nullptr;
+    hipEvent_t ev = nullptr;
+    explicit StreamSwap(mmh_ctx* cc) : c(cc), saved(cc->stream) {
+      HIPCHK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      HIPCHK(hipEventRecord(ev, saved));
+      HIPCHK(hipStreamWaitEvent(own, ev, 0));
+      c->stream = own;
+    }
+    ~StreamSwap() {
+      (void)hipStreamSynchronize(own);
+      c->stream = saved;
+      (void)hipEventDestroy(ev);
+      (void)hipStreamDestroy(own);
+    }
+  } swap_guard(c);
+  std::map<int, std::vector<int>> bysize;
+  for (int i = 0; i < count; ++i) {
+    require(is_pow2(ns[i]) && ns[i] >= 8 && ns[i] <= 1024, "solve_batch: sizes must be powers of two in [8, 1024]");
+    bysize[ns[i]].push_back(i);
+  }
+  bool all_ok = true;
+  for (auto& kv : bysize) {
+    BatchGroup g;
+    g.n = kv.first;
+    g.B = (int)kv.second.size();
+    g.items = kv.second;
+    batch_alloc(c, g);
+    const i64 n3 = cube(g.n);
+    std::vector<double> rden(g.B);
+    std::vector<int> act(g.B);
+    for (int s = 0; s < g.B; ++s) {
+      const int i = g.items[s];
+      const double rn = batch_setup_item(c, g, s, u[i], rhs[i], defpot[i], ubig[i], nu);
+      rden[s] = rn > 0 ? rn : 1.0;
+      act[s] = rn > 0 ? 1 : 0;
+      if (rn == 0) fill<double>(c, g.x + (i64)s * n3, 0.0, n3);  // L u = 0: the mean-free solution is 0
+    }
+    HIPCHK(hipMemcpyAsync(g.rden, rden.data(), sizeof(double) * g.B, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(g.active, act.data(), sizeof(int) * g.B, hipMemcpyHostToDevice, c->stream));
+    fill<double>(c, g.relres, 0.0, g.B);
+    {
+      std::vector<int> zeros(g.B, 0);
+      HIPCHK(hipMemcpyAsync(g.iters, zeros.data(), sizeof(int) * g.B, hipMemcpyHostToDevice, c->stream));
+    }
+    if (c->batch_autotune) batch_autotune(c, g);
+    // r = b - L x, z = M r, rz = <r, z>, p = z
+    const unsigned nsg = stencil_grid(g.n, 1);
+    k_resid<double, 1><<<dim3(nsg, g.B), RB, 0, c->stream>>>(g.cr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, 1);
+    LAUNCHCHK();
+    batch_precond(c, g, S);
+    batch_dot(c, g, g.cr, g.z, g.rz);
+    HIPCHK(hipMemcpyAsync(g.p, g.z, sizeof(double) * g.B * n3, hipMemcpyDeviceToDevice, c->stream));
+    std::vector<int> hact(g.B);
+    auto any_active = [&] {
+      HIPCHK(hipMemcpyAsync(hact.data(), g.active, sizeof(int) * g.B, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (int v : hact)
+        if (v) return true;
+      return false;
+    };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int it = 0;
+    if (any_active()) {
+      // the first iteration runs eagerly (it also sizes every workspace), the rest replay its graph
+      batch_iteration(c, g, S, o->tol);
+      ++it;
+      if (c->batch_graph && it < o->max_cycles && any_active()) {
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        batch_iteration(c, g, S, o->tol);
+        HIPCHK(hipStreamEndCapture(c->stream, &graph));
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      }
+      while (it < o->max_cycles && any_active()) {
+        if (exec) HIPCHK(hipGraphLaunch(exec, c->stream));
+        else batch_iteration(c, g, S, o->tol);
+        ++it;
+      }
+    }
+    if (exec) HIPCHK(hipGraphExecDestroy(exec));
+    if (graph) HIPCHK(hipGraphDestroy(graph));
+    const bool group_ok = !any_active();
+    all_ok = all_ok && group_ok;
+    // true residual of the final iterates, mean removal, back to the caller's layout
+    k_resid<double, 1><<<dim3(nsg, g.B), RB, 0, c->stream>>>(nullptr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, 1);
+    LAUNCHCHK();
+    batch_reduce(c, g, g.bm, nsg, 1, g.rmax);
+    batch_reduce(c, g, g.x, n3, 0, g.sums);
+    k_zerosum_guard<double><<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.sums, nullptr, n3, 0, nullptr, 1);
+    LAUNCHCHK();
+    std::vector<double> hr(g.B);
+    std::vector<int> hi(g.B);
+    HIPCHK(hipMemcpyAsync(hr.data(), g.rmax, sizeof(double) * g.B, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(hi.data(), g.iters, sizeof(int) * g.B, hipMemcpyDeviceToHost, c->stream));
+    for (int s = 0; s < g.B; ++s) relayout(c, u[g.items[s]], g.x + (i64)s * n3, g.n, 1, 0);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (st)
+      for (int s = 0; s < g.B; ++s) {
+        st[g.items[s]].cycles = hi[s];
+        st[g.items[s]].relres = hr[s] / rden[s];
+        st[g.items[s]].converged = hact[s] ? 0 : 1;
+      }
+  }
+  if (!all_ok) throw MmhError(MMH_ENOCONV, "solve_batch: tolerance not reached for every realisation");
+}

@@ -34,10 +34,14 @@ constexpr int RPT = 8;   // elements per thread per reduction block
 // elements t, t+RB, ... of the block in order; the RB per-thread values are combined by a halving
 // tree.  Repeated on the block results until one value is left.
 // op 0: sum of x; op 1: max of |x|.
+// Batched launches (gridDim.y = B): item y reads x + y*xstride and writes out + y*gridDim.x.
 template <class T, int OP>
-__global__ __launch_bounds__(RB) void k_reduce_stage(double* __restrict__ out, const T* __restrict__ x, i64 n) {
+__global__ __launch_bounds__(RB) void k_reduce_stage(double* __restrict__ out, const T* __restrict__ x, i64 n,
+                                                    i64 xstride = 0) {
   __shared__ double s[RB];
   const int t = threadIdx.x;
+  x += (i64)blockIdx.y * xstride;
+  out += (i64)blockIdx.y * gridDim.x;
   const i64 base = (i64)blockIdx.x * (RB * RPT);
   double a = 0.0;
 #pragma unroll
@@ -58,7 +62,8 @@ __global__ __launch_bounds__(RB) void k_reduce_stage(double* __restrict__ out, c
   if (t == 0) out[blockIdx.x] = s[0];
 }
 
-template <class T> __global__ void k_fill(T* __restrict__ a, T v, i64 n) {
+template <class T> __global__ void k_fill(T* __restrict__ a, T v, i64 n, i64 bstride = 0) {
+  a += (i64)blockIdx.y * bstride;
   i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) a[q] = v;
 }
@@ -214,11 +219,17 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __
 // (relaxplane, mmh/gauss1.f90:910-980).  One thread per point of the colour; in layout 1 the
 // threads of a workgroup own consecutive points of the class and every stencil access is
 // unit-stride.
+// Batched launches: gridDim.y = B grids of the same size stored one after another (bstride = n^3
+// values; G holds 6 n^3 per grid).  remap = 0 keeps the hardware's block order (a tuning choice).
 template <class T, int L>
 __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G, int n,
-                                                T dx2, int kpar, int io, int jo) {
+                                                T dx2, int kpar, int io, int jo, i64 bstride = 0, int remap = 1) {
   const int h = n >> 1;
-  const i64 q = (i64)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  phi += (i64)blockIdx.y * bstride;
+  rhs += (i64)blockIdx.y * bstride;
+  G += (i64)blockIdx.y * 6 * bstride;
+  const unsigned blk = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  const i64 q = (i64)blk * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
   const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
   const i64 n3 = (i64)n * n * n;
@@ -233,9 +244,18 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
 // the 8 class arrays and their neighbours are read in lockstep at unit stride.
 template <class T, int L>
 __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restrict__ bmax, const T* __restrict__ phi,
-                                              const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2) {
+                                              const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2,
+                                              int batched = 0) {
   __shared__ double s[RB];
   const i64 n3 = (i64)n * n * n;
+  if (batched) {
+    const i64 y = blockIdx.y;
+    if (r) r += y * n3;
+    phi += y * n3;
+    rhs += y * n3;
+    G += y * 6 * n3;
+    bmax += y * gridDim.x;
+  }
   // XCD-aware tile order; the block result is stored at the tile's index so the reduction order
   // does not depend on the remap
   const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -277,9 +297,16 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
 // block b covering elements [b*RB, (b+1)*RB) combined by the halving tree (deterministic).
 template <class T, int L>
 __global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __restrict__ bsum, const T* __restrict__ p,
-                                                  const T* __restrict__ G, int n, T di2) {
+                                                  const T* __restrict__ G, int n, T di2, int batched = 0) {
   __shared__ double s[RB];
   const i64 n3 = (i64)n * n * n;
+  if (batched) {
+    const i64 y = blockIdx.y;
+    q += y * n3;
+    p += y * n3;
+    G += y * 6 * n3;
+    bsum += y * gridDim.x;
+  }
   // XCD-aware tile order; the block result is stored at the tile's index so the reduction order
   // does not depend on the remap
   const unsigned tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -319,8 +346,12 @@ __global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __r
 
 // Per-block sums of a*b (FP64), same blocking as k_apply_dot.
 template <class T>
-__global__ __launch_bounds__(RB) void k_dot(double* __restrict__ bsum, const T* __restrict__ a, const T* __restrict__ b, i64 n) {
+__global__ __launch_bounds__(RB) void k_dot(double* __restrict__ bsum, const T* __restrict__ a, const T* __restrict__ b, i64 n,
+                                            i64 bstride = 0) {
   __shared__ double s[RB];
+  a += (i64)blockIdx.y * bstride;
+  b += (i64)blockIdx.y * bstride;
+  bsum += (i64)blockIdx.y * gridDim.x;
   const i64 p = (i64)blockIdx.x * RB + threadIdx.x;
   s[threadIdx.x] = p < n ? (double)a[p] * (double)b[p] : 0.0;
   __syncthreads();
@@ -362,13 +393,81 @@ __global__ void k_cg_dir(double* __restrict__ p, const double* __restrict__ z, d
   p[i] = z[i] + beta * p[i];
 }
 
+// Batched CG updates: grid y uses alpha[y] / beta[y] (0 once it has converged, which freezes x and r).
+__global__ __launch_bounds__(RB) void k_cg_update_b(double* __restrict__ x, double* __restrict__ r, double* __restrict__ rold,
+                                                    const double* __restrict__ p, const double* __restrict__ q,
+                                                    const double* __restrict__ alpha, double* __restrict__ bmax, i64 n) {
+  __shared__ double s[RB];
+  const i64 y = blockIdx.y;
+  x += y * n; r += y * n; rold += y * n; p += y * n; q += y * n;
+  const double al = alpha[y];
+  const i64 i = (i64)blockIdx.x * RB + threadIdx.x;
+  double a = 0.0;
+  if (i < n) {
+    x[i] = x[i] + al * p[i];
+    const double ro = r[i];
+    rold[i] = ro;
+    const double rn = ro - al * q[i];
+    r[i] = rn;
+    a = fabs(rn);
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = RB / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bmax[y * gridDim.x + blockIdx.x] = s[0];
+}
+
+__global__ void k_cg_dir_b(double* __restrict__ p, const double* __restrict__ z, const double* __restrict__ beta, i64 n) {
+  const i64 y = blockIdx.y;
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[y * n + i] = z[y * n + i] + beta[y] * p[y * n + i];
+}
+
+// Per-grid CG scalars, one thread per grid.  step 0: alpha = rz / pq for active grids, else 0.
+// step 1: relres = rmax / rden and the iteration count advance for active grids; a grid whose
+// relres <= tol becomes inactive.  step 2: beta = (rz_new - zro) / rz, rz = rz_new (0 for inactive grids).
+__global__ void k_cg_scalars(int step, int B, int it, double tol, double* __restrict__ alpha, double* __restrict__ beta,
+                             double* __restrict__ rz, const double* __restrict__ pq, const double* __restrict__ rmax,
+                             const double* __restrict__ rden, const double* __restrict__ rznew,
+                             const double* __restrict__ zro, int* __restrict__ active, int* __restrict__ iters,
+                             double* __restrict__ relres) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= B) return;
+  if (step == 0) {
+    alpha[y] = active[y] ? rz[y] / pq[y] : 0.0;
+  } else if (step == 1) {
+    if (active[y]) {
+      const double rr = rmax[y] / rden[y];
+      relres[y] = rr;
+      iters[y] = iters[y] + 1;
+      if (rr <= tol) active[y] = 0;
+    }
+  } else {
+    if (active[y]) {
+      beta[y] = (rznew[y] - zro[y]) / rz[y];
+      rz[y] = rznew[y];
+    } else {
+      beta[y] = 0.0;
+    }
+  }
+}
+
 // Full-weighting restriction of the residual as accumulated by calcresid
 // (mmh/gauss1.f90:819-859): coarse plane ck gets a main part from fine planes 2ck, 2ck+1 and a
 // share from fine plane 2ck-1; plane ck0 accumulates main then share, every other plane share
 // then main (the order in which r1lax's plane pipeline reaches them).  r and rc in layout L.
 template <class T, int L>
-__global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0) {
+__global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0,
+                                                        int batched = 0) {
   const int h = n >> 1;
+  if (batched) {
+    rc += (i64)blockIdx.y * h * h * h;
+    r += (i64)blockIdx.y * n * n * n;
+  }
   const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
   // coarse points in column-major order: in layout 1 the 27 fine reads of consecutive threads are
@@ -436,9 +535,14 @@ __global__ __launch_bounds__(256) void k_smooth27(T* __restrict__ out, const T* 
 // Trilinear prolongation (mmh/mgutil.f90:38-71 inject adds; 2-34 prolong overwrites); u (n^3) and
 // uh ((n/2)^3) in layout L.
 template <class T, int L>
-__global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __restrict__ uh, int n, int add) {
+__global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __restrict__ uh, int n, int add,
+                                                 int batched = 0) {
   const i64 pp = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const int h = n >> 1;
+  if (batched) {
+    u += (i64)blockIdx.y * n * n * n;
+    uh += (i64)blockIdx.y * h * h * h;
+  }
   if (L == 1) {
     // one thread per fine 2x2x2 cell (ci, cj, ck): its 8 points interpolate from the coarse block
     // {ci, ci+1} x {cj, cj+1} x {ck, ck+1}; each point's 8-term sum keeps the order below
@@ -506,8 +610,12 @@ __global__ void k_rgz_update(T* __restrict__ arr, const T* __restrict__ det, con
 // relaxerr > 1.001*r1 and r1 > 1e-5 (mmh/multigrid.f90:113).  guard_out records the decision.
 template <class T>
 __global__ void k_zerosum_guard(T* __restrict__ a, const double* __restrict__ sum, const double* __restrict__ r1re, i64 n3,
-                                int mode, int* __restrict__ guard_out) {
+                                int mode, int* __restrict__ guard_out, int batched = 0) {
   const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (batched) {  // mode 0 only: item y removes the mean sum[y] / n3 of its grid
+    a += (i64)blockIdx.y * n3;
+    sum += blockIdx.y;
+  }
   bool g = false;
   if (mode == 1) g = (T)r1re[1] > (T)r1re[0];
   else if (mode == 2) g = ((T)r1re[1] > (T)r1re[0] * T(1.001) && (T)r1re[0] > T(1.e-5));

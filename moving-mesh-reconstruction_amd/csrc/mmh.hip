@@ -32,6 +32,9 @@ struct mmh_ctx {
   std::set<std::string> uploaded;  // constant tables already copied to the device
   std::string err;
   int sweep_impl = 0;  // mmh_time_sweeps layout: 0 sub-lattice (the V-cycle's), 1 column-major
+  int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
+  int batch_autotune = 1;  // batched solve: pick the colour kernel's launch shape by timing
+  std::map<std::string, std::pair<int, int>> tune;  // (grid size, batch) -> (block size, XCD remap)
 };
 
 namespace {
@@ -1138,6 +1141,8 @@ template <class T> void down(mmh_ctx* c, T* h, const T* d, i64 count) {
   HIPCHK(hipMemcpyAsync(h, d, sizeof(T) * count, hipMemcpyDeviceToHost, c->stream));
 }
 
+#include "batch.hpp"
+
 std::mutex g_default_mu;
 mmh_ctx* g_default = nullptr;
 
@@ -1177,9 +1182,15 @@ const char* mmh_error_string(mmh_ctx* c) { return c ? c->err.c_str() : "null con
 int mmh_set_option(mmh_ctx* c, const char* key, int value) {
   return guarded(c, [&] {
     require(key != nullptr, "set_option: null key");
-    if (std::string(key) == "sweep_impl") {
+    const std::string k(key);
+    if (k == "sweep_impl") {
       require(value == 0 || value == 1, "set_option: sweep_impl is 0 or 1");
       c->sweep_impl = value;
+    } else if (k == "batch_graph") {
+      c->batch_graph = value ? 1 : 0;
+    } else if (k == "batch_autotune") {
+      c->batch_autotune = value ? 1 : 0;
+      if (!value) c->tune.clear();
     } else {
       throw MmhError(MMH_EINVAL, std::string("set_option: unknown key ") + key);
     }
@@ -1305,6 +1316,12 @@ int mmh_residual_d(mmh_ctx* c, double* r, const double* phi, const double* rhs, 
     LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(c->stream));
   });
+}
+
+int mmh_solve_batch_d(mmh_ctx* c, int count, const int* n, double* const* u, double* const* rhs,
+                      double* const* defpot, double* const* ubig, int nu, const mmh_solve_opts* opts,
+                      mmh_batch_stats* st) {
+  return guarded(c, [&] { solve_batch_dev(c, count, n, u, rhs, defpot, ubig, nu, opts, st); });
 }
 
 int mmh_power_d(mmh_ctx* c, const double* field, int n, int nbins, double* pk, double* nmodes) {

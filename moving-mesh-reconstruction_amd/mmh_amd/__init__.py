@@ -54,6 +54,10 @@ class AdjointStats(C.Structure):
                 ("relres_forward", C.c_double), ("relres_adjoint", C.c_double), ("ncompress", C.c_int)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("cycles", C.c_int), ("relres", C.c_double), ("converged", C.c_int)]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_cycles", C.c_int), ("schedule", C.c_int), ("nu_pre", C.c_int),
                 ("nu_post", C.c_int), ("nu_coarse", C.c_int), ("ng", C.c_int), ("mixed", C.c_int),
@@ -179,6 +183,19 @@ class Context:
         if rc not in (MMH_OK, MMH_ENOCONV):
             self._check(rc)
         return st, rc == MMH_OK
+
+    def solve_batch(self, items, tol=1e-8, max_cycles=200, nu_pre=2, nu_coarse=8, nu=5):
+        """items: list of (u, rhs, defpot, ubig) device tensors (sizes may differ).  One batched FP64
+        PCG solve; returns (list of BatchStats, all_converged)."""
+        k = len(items)
+        ns = (C.c_int * k)(*[_cube_side(it[0]) for it in items])
+        arr = [(C.c_void_p * k)(*[_ptr(it[j]).value for it in items]) for j in range(4)]
+        o = SolveOpts(tol, max_cycles, 1, nu_pre, nu_pre, nu_coarse, 0, 0, 1)
+        st = (BatchStats * k)()
+        rc = lib().mmh_solve_batch_d(self._c, k, ns, arr[0], arr[1], arr[2], arr[3], nu, C.byref(o), st)
+        if rc not in (MMH_OK, MMH_ENOCONV):
+            self._check(rc)
+        return list(st), rc == MMH_OK
 
     def calcdefp(self, defp, tmp, tmp2, def_, u, dtold1=1.0, dtaumesh=1.0, nfluid=5, compressmax=10.0, ng=0):
         s, T = _sfx(defp)

@@ -1,0 +1,90 @@
+"""GPU: the batched solve (many realisations of uneven sizes in one call) against single solves.
+Every realisation must be bit-identical to its own mmh_solve_d (FP64 PCG), with graph replay and
+with eager launches alike (the graph is replayed tens of times per solve and checked against the
+eager result), and with or without launch-shape tuning."""
+import numpy as np
+import pytest
+
+import refbind as R
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+F64 = dict(dtype=torch.float64, device="cuda:0")
+SOLVE = dict(tol=1e-8, max_cycles=200, nu_pre=2, nu_coarse=8)
+
+
+def problem(ctx, n, sigma, seed):
+    rho = torch.zeros(n ** 3, **F64)
+    ctx.lognormal(rho, sigma, seed)
+    u = torch.zeros(n ** 3, 5, **F64)
+    u[:, 0] = rho
+    u = u.contiguous().view(-1)
+    d = torch.zeros(n ** 3, **F64)
+    rhs = torch.zeros(n ** 3, **F64)
+    ctx.limiter_rhs(rhs, d, u)
+    return u, rhs, d
+
+
+CASES = [(32, 0.5, 0), (32, 2.0, 1), (64, 1.2, 2), (32, 1.0, 3), (64, 1.8, 4), (128, 1.0, 5)]
+
+
+def run_batch(ctx, probs):
+    items = [(torch.zeros_like(r), r.clone(), d, u) for (u, r, d) in probs]
+    st, ok = ctx.solve_batch(items, **SOLVE)
+    torch.cuda.synchronize()
+    return [it[0].cpu().numpy() for it in items], st, ok
+
+
+@pytest.fixture(scope="module")
+def probs(ctx):
+    return [problem(ctx, *c) for c in CASES]
+
+
+def test_batch_bit_identical_to_single_solves(ctx, probs):
+    print("case batch vs single: %s" % (CASES,), flush=True)
+    xs, st, ok = run_batch(ctx, probs)
+    assert ok
+    for (u, r, d), x, s, case in zip(probs, xs, st, CASES):
+        phi = torch.zeros_like(r)
+        st1, ok1 = ctx.solve(phi, r.clone(), d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
+                             nu_coarse=8, nu=5, krylov=True)
+        assert ok1 and s.converged
+        assert s.cycles == st1.cycles, (case, s.cycles, st1.cycles)
+        assert s.relres == st1.relres[st1.cycles - 1]
+        assert np.array_equal(x, phi.cpu().numpy()), case
+
+
+def test_batch_graph_replay_equals_eager(ctx, probs):
+    print("case batch graph vs eager", flush=True)
+    xg, stg, okg = run_batch(ctx, probs)
+    ctx.set_option("batch_graph", 0)
+    try:
+        xe, ste, oke = run_batch(ctx, probs)
+    finally:
+        ctx.set_option("batch_graph", 1)
+    assert okg and oke
+    for a, b, s1, s2 in zip(xg, xe, stg, ste):
+        assert np.array_equal(a, b)
+        assert s1.cycles == s2.cycles
+
+
+def test_batch_tuning_does_not_change_results(ctx, probs):
+    print("case batch autotune off", flush=True)
+    xa, _, _ = run_batch(ctx, probs)
+    ctx.set_option("batch_autotune", 0)
+    try:
+        xb, _, _ = run_batch(ctx, probs)
+    finally:
+        ctx.set_option("batch_autotune", 1)
+    for a, b in zip(xa, xb):
+        assert np.array_equal(a, b)
+
+
+def test_batch_zero_rhs_item(ctx, probs):
+    print("case batch zero rhs", flush=True)
+    u, r, d = probs[0]
+    items = [(torch.ones_like(r), torch.zeros_like(r), d, u), (torch.zeros_like(r), r.clone(), d, u)]
+    st, ok = ctx.solve_batch(items, **SOLVE)
+    assert ok and st[0].cycles == 0 and st[0].converged
+    assert float(items[0][0].abs().max()) == 0.0
