@@ -107,6 +107,61 @@ def smoother_only(ctx, torch, defn, u, rhs0, n, sweeps):
     print("smoother-only: %d^3 FP64 %.4f ms/sweep (%d sweeps); copy 1 GiB x3 done" % (n, ms, sweeps), flush=True)
 
 
+def batched_leg(ctx, torch):
+    """Batched uneven realisations: 40 x 32^3, 40 x 64^3 and 20 x 128^3 lognormal fields, sigma
+    uniform in [0.5, 2.0] (seeded), seeds 0..99, each solved from phi = 0 to 1e-8 (FP64 PCG).
+    Realisations per second for one batched call against a host loop of single solves."""
+    import numpy as np
+    f64 = dict(dtype=torch.float64, device="cuda")
+    sizes = [32] * 40 + [64] * 40 + [128] * 20
+    sig = np.random.default_rng(2024).uniform(0.5, 2.0, len(sizes))
+    probs = []
+    for i, (n, s) in enumerate(zip(sizes, sig)):
+        rho = torch.zeros(n ** 3, **f64)
+        ctx.lognormal(rho, float(s), i)
+        u = torch.zeros(n ** 3, 5, **f64)
+        u[:, 0] = rho
+        u = u.contiguous().view(-1)
+        d = torch.zeros(n ** 3, **f64)
+        rhs = torch.zeros(n ** 3, **f64)
+        ctx.limiter_rhs(rhs, d, u)
+        probs.append((u, rhs, d))
+    opts = dict(tol=1e-8, max_cycles=200, nu_pre=2, nu_coarse=8)
+
+    def batched():
+        items = [(torch.zeros_like(r), r.clone(), d, u) for (u, r, d) in probs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, ok = ctx.solve_batch(items, **opts)
+        torch.cuda.synchronize()
+        if not ok:
+            raise RuntimeError("batched solve did not converge")
+        return time.perf_counter() - t0, st
+
+    def looped():
+        work = [(torch.zeros_like(r), r.clone(), d, u) for (u, r, d) in probs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for phi, r, d, u in work:
+            st, ok = ctx.solve(phi, r, d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2, nu_coarse=8,
+                               nu=5, krylov=True)
+            if not ok:
+                raise RuntimeError("single solve did not converge")
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    batched()  # warm-up: workspaces and launch-shape tuning
+    tb, st = batched()
+    print("bench: batched 100 realisations %.3f s" % tb, file=sys.stderr, flush=True)
+    looped()
+    tl = looped()
+    print("bench: looped 100 realisations %.3f s" % tl, file=sys.stderr, flush=True)
+    its = [s.cycles for s in st]
+    return {"workload": "40x32^3 + 40x64^3 + 20x128^3 lognormal, sigma U[0.5,2], seeds 0..99, to 1e-8 (FP64 PCG)",
+            "batched_realizations_per_s": round(len(sizes) / tb, 2), "looped_realizations_per_s": round(len(sizes) / tl, 2),
+            "batched_speedup": round(tl / tb, 3), "pcg_iterations_min_max": [min(its), max(its)]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +176,7 @@ def main():
     ap.add_argument("--ref-cycles", type=int, default=3, help="plain reference V-cycles timed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweeps", type=int, default=20, help="sweeps timed for the roofline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-realisations leg")
     ap.add_argument("--smoother-only", action="store_true",
                     help="only the smoother timing plus a 1 GiB device copy (the counter-calibration run)")
     args = ap.parse_args()
@@ -196,6 +252,10 @@ def main():
         del phi, rhs
         print("bench: reference V-cycle %.1f ms" % ref_ms, file=sys.stderr, flush=True)
 
+    batch = None
+    if not args.no_batch:
+        batch = batched_leg(ctx, torch)
+
     # roofline of the dominant kernel (k_colour: one colour class per launch), FP64 and FP32
     G = torch.zeros(6 * n3, **f64)
     ctx.metric(G, defn, u, 1.0, 0, nu=5)
@@ -240,7 +300,8 @@ def main():
                    "mixed_rel_err_vs_fp64": err, "final_relres_fp64": r64, "final_relres_mixed": rmx,
                    "reference_vcycle_ms_fp64": ref_ms, "reference_relres_after_%d_cycles" % args.ref_cycles: ref_relres,
                    "sweep_ms_fp64": round(ms64, 4), "sweep_ms_fp32": round(ms32, 4),
-                   "smoother_alg_GBs_fp32": round(B_ALG_WORDS * 4 * n3 / (ms32 * 1e-3) / 1e9, 1)},
+                   "smoother_alg_GBs_fp32": round(B_ALG_WORDS * 4 * n3 / (ms32 * 1e-3) / 1e9, 1),
+                   "batched_realizations": batch},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(n),
                      "kernel": "k_colour<double> (one of 8 colour classes of a Gauss-Seidel sweep, %d^3)" % n,
