@@ -97,8 +97,15 @@ int mmh_destroy(mmh_ctx* ctx);
 const char* mmh_error_string(mmh_ctx* ctx);
 int mmh_set_stream(mmh_ctx* ctx, void* hip_stream); /* NULL = the null stream (default) */
 int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffers */
-/* Options: "sweep_impl" selects the array layout mmh_time_sweeps runs the smoother in: 0 = the
- * sub-lattice layout the V-cycle uses (default), 1 = column-major.  Results are bit-identical. */
+/* Options (results never depend on them; all are bit-identical):
+ *   "sweep_impl"  mmh_time_sweeps: 0 = the solver's smoother (sub-lattice layout; the wavefront
+ *                 kernel where "wave" is on), 1 = colour kernels on column-major arrays,
+ *                 2 = colour kernels on sub-lattice arrays
+ *   "wave"        0 (default) / 1: whole-sweep wavefront kernel on levels with n >= "wave_min_n"
+ *                 (it measures slower than the colour kernels on MI355X; kept as an option)
+ *   "wave_min_n"  smallest level size for the wavefront kernel (default 128)
+ *   "wave_p"      1 (default), 2 or 4 points per thread in flight per colour phase
+ *   "batch_graph", "batch_autotune": see mmh_solve_batch_d */
 int mmh_set_option(mmh_ctx* ctx, const char* key, int value);
 
 /* ---- solver (device pointers) --------------------------------------------------------------- */

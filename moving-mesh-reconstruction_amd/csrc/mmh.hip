@@ -22,6 +22,7 @@
 #include "kernels.hpp"
 #include "kernels_lim.hpp"
 #include "kernels_adj.hpp"
+#include "kernels_wave.hpp"
 
 using namespace mmh;
 
@@ -32,6 +33,9 @@ struct mmh_ctx {
   std::set<std::string> uploaded;  // constant tables already copied to the device
   std::string err;
   int sweep_impl = 0;  // mmh_time_sweeps layout: 0 sub-lattice (the V-cycle's), 1 column-major
+  int wave = 0;            // whole-sweep wavefront kernel for levels with n >= wave_min_n (opt-in)
+  int wave_min_n = 128;
+  int wave_p = 1;          // points per thread in flight per colour phase of the wavefront kernel
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
   int batch_autotune = 1;  // batched solve: pick the colour kernel's launch shape by timing
   std::map<std::string, std::pair<int, int>> tune;  // (grid size, batch) -> (block size, XCD remap)
@@ -195,11 +199,51 @@ template <class T> void relayout(mmh_ctx* c, T* dst, const T* src, int n, int co
 
 // ns colour sweeps of phi: 8 launches per sweep, one per colour class; layout 1 (sub-lattice, the
 // V-cycle's) or 0 (column-major).
+// Tile shape of the wavefront sweep: FP32 64 x 32, FP64 32 x 32 points (four LDS planes stay
+// under 64 KiB, two workgroups per CU).
+template <class T> struct WaveTile;
+template <> struct WaveTile<float> { static constexpr int TI = 64, TJ = 32; };
+template <> struct WaveTile<double> { static constexpr int TI = 32, TJ = 32; };
+
+template <class T> bool wave_fits(mmh_ctx* c, int n) {
+  constexpr int TI = WaveTile<T>::TI, TJ = WaveTile<T>::TJ;
+  return c->wave && n >= c->wave_min_n && n >= TI + 2 * WAVE_HI && n >= TJ + 2 * WAVE_HJ && n % TI == 0 && n % TJ == 0;
+}
+
+// One wavefront sweep from `in` into `out` (forward or reverse colour order).
+template <class T> void wave_sweep(mmh_ctx* c, T* out, const T* in, const T* rhs, const T* G, int n, T dx2, bool rev) {
+  constexpr int TI = WaveTile<T>::TI, TJ = WaveTile<T>::TJ;
+  const i64 tiles = (i64)(n / TI) * (n / TJ);
+  int K = 8;  // planes of each parity per segment: the largest of 8, 4, 2 giving >= 1024 workgroups
+  while (K > 2 && tiles * (n / (2 * K)) < 1024) K >>= 1;
+  require(n % (2 * K) == 0, "wave sweep: n must be a multiple of 2K");
+  const unsigned nwg = (unsigned)(tiles * (n / (2 * K)));
+  const size_t lds = sizeof(T) * 4 * (TI + 2 * WAVE_HI) * (TJ + 2 * WAVE_HJ);
+#define WAVE_LAUNCH(P)                                                                                         \
+  if (rev) k_sweep_wave<T, TI, TJ, true, P><<<nwg, 512, lds, c->stream>>>(out, in, rhs, G, n, dx2, K);        \
+  else k_sweep_wave<T, TI, TJ, false, P><<<nwg, 512, lds, c->stream>>>(out, in, rhs, G, n, dx2, K);
+  if (c->wave_p == 4) { WAVE_LAUNCH(4) }
+  else if (c->wave_p == 2) { WAVE_LAUNCH(2) }
+  else { WAVE_LAUNCH(1) }
+#undef WAVE_LAUNCH
+  LAUNCHCHK();
+}
+
 template <class T>
 void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, int layout = 1, bool reverse = false) {
   const T dx2 = dx * dx;
   const int h = n / 2;
   const unsigned nb = nblk(cube(h));
+  if (layout == 1 && ns >= 2 && wave_fits<T>(c, n)) {
+    // pairs of sweeps through a second buffer; an odd last sweep goes colour by colour below
+    T* tmp = wsT<T>(c, "wave_tmp" + std::to_string(n), cube(n));
+    const int pairs = ns / 2;
+    for (int q = 0; q < pairs; ++q) {
+      wave_sweep<T>(c, tmp, phi, rhs, G, n, dx2, reverse);
+      wave_sweep<T>(c, phi, tmp, rhs, G, n, dx2, reverse);
+    }
+    ns -= 2 * pairs;
+  }
   for (int s = 0; s < ns; ++s)
     for (int cc = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
@@ -1081,7 +1125,9 @@ template <class T> void time_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G
   T* ps = wsT<T>(c, "ts_p", n3);
   T* rs = wsT<T>(c, "ts_r", n3);
   T* gs = wsT<T>(c, "ts_g", 6 * n3);
-  const int sl = c->sweep_impl == 0;
+  const int sl = c->sweep_impl != 1;
+  const int wave_saved = c->wave;
+  if (c->sweep_impl == 2) c->wave = 0;
   relayout(c, ps, phi, n, 1, 1);
   relayout(c, rs, rhs, n, 1, 1);
   relayout(c, gs, G, n, 6, 1);
@@ -1100,6 +1146,7 @@ template <class T> void time_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G
   float t = 0;
   HIPCHK(hipEventElapsedTime(&t, e0, e1));
   *ms = t / std::max(1, ns);
+  c->wave = wave_saved;
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));
   if (sl) relayout(c, phi, ps, n, 1, 0);
@@ -1184,8 +1231,16 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
     require(key != nullptr, "set_option: null key");
     const std::string k(key);
     if (k == "sweep_impl") {
-      require(value == 0 || value == 1, "set_option: sweep_impl is 0 or 1");
+      require(value >= 0 && value <= 2, "set_option: sweep_impl is 0, 1 or 2");
       c->sweep_impl = value;
+    } else if (k == "wave") {
+      c->wave = value ? 1 : 0;
+    } else if (k == "wave_p") {
+      require(value == 1 || value == 2 || value == 4, "set_option: wave_p is 1, 2 or 4");
+      c->wave_p = value;
+    } else if (k == "wave_min_n") {
+      require(value >= 16, "set_option: wave_min_n >= 16");
+      c->wave_min_n = value;
     } else if (k == "batch_graph") {
       c->batch_graph = value ? 1 : 0;
     } else if (k == "batch_autotune") {

@@ -88,7 +88,7 @@ def test_relax_bitexact(ctx, dt, n, nrelax, iopt):
 
 
 @pytest.mark.parametrize("dt", DT)
-@pytest.mark.parametrize("n,ng,sigma", [(16, 16, 1.0), (32, 32, 2.0), (32, 64, 1.0), (64, 64, 2.0)])
+@pytest.mark.parametrize("n,ng,sigma", [(16, 16, 1.0), (32, 32, 2.0), (32, 64, 1.0), (64, 64, 2.0), (128, 128, 2.0)])
 def test_multigrid_bitexact(ctx, dt, n, ng, sigma):
     print("case multigrid n=%d ng=%d %s" % (n, ng, dt.__name__), flush=True)
     rho = field(n, sigma, 7, dt)
@@ -259,3 +259,61 @@ def test_sublattice_sweep_equals_column_major(ctx, dt, n):
         out.append(host(phi))
     ctx.set_option("sweep_impl", 0)
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n,ns", [(128, 2), (128, 3), (256, 2)])
+def test_wavefront_sweep_equals_colour_kernels(ctx, dt, n, ns):
+    """The whole-sweep wavefront kernel (sweep_impl 0 at n >= 128) against the colour-per-launch
+    kernels (sweep_impl 2) on the same sub-lattice arrays: bit-identical after ns sweeps (an odd
+    count ends with one colour-by-colour sweep)."""
+    print("case wavefront sweep n=%d ns=%d %s" % (n, ns, dt.__name__), flush=True)
+    rho = field(n, 2.0, 3, dt)
+    u = aos(rho)
+    d = smooth_def(n, 2.0, 4, dt)
+    Gd = dev(np.zeros(6 * n ** 3, dt))
+    ctx.metric(Gd, dev(d), dev(u), 1.0, 0, nu=5)
+    rhs = dev((1 - rho).astype(dt))
+    phi0 = smooth_def(n, 0.5, 9, dt)
+    out = []
+    ctx.set_option("wave", 1)
+    try:
+        for impl in (0, 2):
+            ctx.set_option("sweep_impl", impl)
+            phi = dev(phi0)
+            ctx.time_sweeps(phi, rhs, Gd, 1.0, ns)
+            out.append(host(phi))
+    finally:
+        ctx.set_option("sweep_impl", 0)
+        ctx.set_option("wave", 0)
+    assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("dt", DT)
+def test_wavefront_vcycle_bitexact(ctx, dt):
+    """With the wavefront kernel switched on, a 128^3 V-cycle (forward sweeps at 128^3) and a
+    symmetric-preconditioner PCG solve (reverse sweeps) give the colour kernels' results."""
+    n = 128
+    print("case wavefront V-cycle/PCG n=%d %s" % (n, dt.__name__), flush=True)
+    rho = field(n, 2.0, 7, dt)
+    u = aos(rho)
+    d = smooth_def(n, 2.0, 4, dt)
+    rhs = (1 - rho).astype(dt)
+    R.or_nsmooth(rhs, n)
+    res = []
+    for wave in (0, 1):
+        ctx.set_option("wave", wave)
+        try:
+            ad, bd = dev(np.zeros(n ** 3, dt)), dev(rhs)
+            ctx.multigrid(ad, bd, dev(d), dev(u), 1.0, 2, n, nu=5)
+            res.append(host(ad))
+            if dt == np.float64:
+                phi, b = dev(np.zeros(n ** 3)), dev(rhs)
+                ctx.solve(phi, b, dev(d), dev(u), tol=1e-8, max_cycles=100, schedule=1, nu_pre=2, nu_post=2,
+                          nu_coarse=8, nu=5, krylov=True)
+                res.append(host(phi))
+        finally:
+            ctx.set_option("wave", 0)
+    half = len(res) // 2
+    for a, b in zip(res[:half], res[half:]):
+        assert np.array_equal(a, b)
