@@ -173,6 +173,13 @@ int mmh_reconstruct_d(mmh_ctx*, double* u, double* def, double* defp, int n, int
 int mmh_reconstruct_f(mmh_ctx*, float* u, float* def, float* defp, int n, int nfluid, int niter, float dtold,
                       float dtaumesh, float dt, float compressmax, mmh_iter_stats* st);
 
+/* Density of ng^3 cells from np particle positions xp[3 p + d] (device, in [0, nf) per axis after
+ * a periodic wrap), as haloden.f90:49-195 assigns it: each particle's unit mass is shared equally
+ * between its nearest-grid-point cell and the empty cells whose nearest particle (searched in
+ * growing cubes, haloden.f90:119-167) it is; den (device, ng^3, Fortran order) is then scaled to
+ * mean 1 (haloden.f90:187).  wp_out (device, np, may be NULL): the cell count of every particle. */
+int mmh_particle_density_f(mmh_ctx*, const float* xp, long long np, int ng, float nf, float* den, int* wp_out);
+
 /* Binned power spectrum of a real N^3 field (no reference counterpart: the reference reads P(k)
  * from text files, mmh/calcfisherinfo.f90:26-36).  P_b = sum over modes k in bin b of |F(k)|^2 /
  * (N_b N^3), F the unnormalised DFT, N_b the mode count; nbins bins of |k| (grid units, k = 2 pi m / N)

@@ -23,6 +23,9 @@
 #include "kernels_lim.hpp"
 #include "kernels_adj.hpp"
 #include "kernels_wave.hpp"
+#include "kernels_pd.hpp"
+
+#include <hipcub/hipcub.hpp>
 
 using namespace mmh;
 
@@ -1377,6 +1380,46 @@ int mmh_solve_batch_d(mmh_ctx* c, int count, const int* n, double* const* u, dou
                       double* const* defpot, double* const* ubig, int nu, const mmh_solve_opts* opts,
                       mmh_batch_stats* st) {
   return guarded(c, [&] { solve_batch_dev(c, count, n, u, rhs, defpot, ubig, nu, opts, st); });
+}
+
+int mmh_particle_density_f(mmh_ctx* c, const float* xp, long long np, int ng, float nf, float* den, int* wp_out) {
+  return guarded(c, [&] {
+    require(xp && den && np >= 1 && np < (1LL << 31), "particle_density: 1 <= np < 2^31 particles required");
+    require(ng >= 2 && ng <= 1024 && nf > 0, "particle_density: 2 <= ng <= 1024, nf > 0");
+    const i64 ncell = cube(ng);
+    float* xs = wsT<float>(c, "pd_xs", 3 * np);
+    int* cell = (int*)ws_get(c, "pd_cell", sizeof(int) * np);
+    int* idx = (int*)ws_get(c, "pd_idx", sizeof(int) * np);
+    int* key = (int*)ws_get(c, "pd_key", sizeof(int) * np);
+    int* order = (int*)ws_get(c, "pd_order", sizeof(int) * np);
+    int* start = (int*)ws_get(c, "pd_start", sizeof(int) * ncell);
+    int* end = (int*)ws_get(c, "pd_end", sizeof(int) * ncell);
+    int* near = (int*)ws_get(c, "pd_near", sizeof(int) * ncell);
+    int* wp = (int*)ws_get(c, "pd_wp", sizeof(int) * np);
+    k_pd_cells<<<nblk(np), 256, 0, c->stream>>>(xs, cell, idx, xp, np, ng, nf);
+    LAUNCHCHK();
+    // particles sorted by cell, stably, so each cell lists its particles by increasing index
+    int bits = 1;
+    while ((1LL << bits) < ncell) ++bits;
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, cell, key, idx, order, (int)np, 0, bits, c->stream));
+    void* tmp = ws_get(c, "pd_sort_tmp", tmp_bytes + 16);
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, cell, key, idx, order, (int)np, 0, bits, c->stream));
+    k_fill<int><<<nblk(ncell), 256, 0, c->stream>>>(start, 0, ncell);
+    k_fill<int><<<nblk(ncell), 256, 0, c->stream>>>(end, 0, ncell);
+    k_fill<int><<<nblk(np), 256, 0, c->stream>>>(wp, 1, np);
+    k_pd_ranges<<<nblk(np), 256, 0, c->stream>>>(start, end, key, np);
+    k_pd_nearest<<<nblk(ncell), 256, 0, c->stream>>>(near, wp, start, end, order, xs, ng, ng / 2 + 1);
+    k_pd_density<<<nblk(ncell), 256, 0, c->stream>>>(den, near, wp, start, end, order, ncell);
+    LAUNCHCHK();
+    double* sum = wsT<double>(c, "pd_sum", 1);
+    reduce<float>(c, den, ncell, 0, sum);
+    const float fng = (float)ng;
+    k_pd_normalise<<<nblk(ncell), 256, 0, c->stream>>>(den, sum, fng * fng * fng, ncell);
+    LAUNCHCHK();
+    if (wp_out) HIPCHK(hipMemcpyAsync(wp_out, wp, sizeof(int) * np, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
 }
 
 int mmh_power_d(mmh_ctx* c, const double* field, int n, int nbins, double* pk, double* nmodes) {

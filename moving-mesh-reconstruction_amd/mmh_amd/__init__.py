@@ -226,6 +226,12 @@ class Context:
         nu = nu or u.numel() // phi.numel()
         self.call("mmh_residual_d", _ptr(r), _ptr(phi), _ptr(rhs), _ptr(def_), _ptr(u), nu, n, C.c_double(dx))
 
+    def particle_density(self, xp, ng, nf, den, wp=None):
+        """haloden.f90's density assignment: xp device float32 (np, 3) positions in [0, nf);
+        den device float32 ng^3 (Fortran order) receives the mean-1 density."""
+        npart = xp.numel() // 3
+        self.call("mmh_particle_density_f", _ptr(xp), C.c_longlong(npart), ng, C.c_float(nf), _ptr(den), _ptr(wp))
+
     def power(self, field, nbins=16):
         """Binned power spectrum of an N^3 FP64 device field: (P_b, mode counts) as numpy arrays."""
         import numpy as np

@@ -1123,6 +1123,73 @@ void lognormal(double* rho, int n, double sigma, double slope, double ks_frac, u
 
 // ================================================================================================
 // C entry points (ctypes).  Suffix _d = double, _f = float.
+// ------------------------------------------------------------------------------------------------
+// Particle to grid density assignment (haloden.f90:84-187), with the reference's 1-based linked
+// lists: hoc_g/ll_p (particles of a cell, newest first), hoc_p/ll_g (cells of a particle).
+// xp: 3 np positions (x, y, z per particle); den: ng^3 (Fortran order); wp: np.
+static void particle_density(const float* xp_in, long np, int ng, float nf, float* den, int* wp) {
+  std::vector<float> xp(3 * (size_t)np);
+  for (long p = 0; p < np; ++p)
+    for (int d = 0; d < 3; ++d) {
+      float x = xp_in[3 * p + d];
+      x = x - std::floor(x / nf) * nf;          // modulo(xp, real(nf)), haloden.f90:86-88
+      xp[3 * p + d] = x * (float)ng / nf;       // xp = xp * ng / nf, haloden.f90:92
+    }
+  const long nc = (long)ng * ng * ng;
+  std::vector<int> hoc_g(nc + 1, 0), ll_p(np + 1, 0), hoc_p(np + 1, 0), ll_g(nc + 1, 0);
+  auto cid = [&](int i, int j, int k) { return (long)ng * ng * (k - 1) + (long)ng * (j - 1) + i; };  // 1-based
+  auto clampc = [&](float x) { int c = (int)std::floor(x) + 1; return c < 1 ? 1 : (c > ng ? ng : c); };
+  for (long ip = 1; ip <= np; ++ip) {  // haloden.f90:96-107
+    const int i = clampc(xp[3 * (ip - 1)]), j = clampc(xp[3 * (ip - 1) + 1]), k = clampc(xp[3 * (ip - 1) + 2]);
+    ll_p[ip] = hoc_g[cid(i, j, k)];
+    hoc_g[cid(i, j, k)] = (int)ip;
+  }
+  for (long ip = 1; ip <= np; ++ip) wp[ip - 1] = 1;
+  for (int k = 1; k <= ng; ++k)  // non-empty cells head their particles' chains, haloden.f90:111-122
+    for (int j = 1; j <= ng; ++j)
+      for (int i = 1; i <= ng; ++i) {
+        const long ig = cid(i, j, k);
+        for (int pp = hoc_g[ig]; pp != 0; pp = ll_p[pp]) hoc_p[pp] = (int)ig;
+      }
+  for (int k = 1; k <= ng; ++k)  // empty cells join their nearest particle, haloden.f90:125-167
+    for (int j = 1; j <= ng; ++j)
+      for (int i = 1; i <= ng; ++i) {
+        const long ig = cid(i, j, k);
+        if (hoc_g[ig] != 0) continue;
+        const float gpos[3] = {(float)i - 0.5f, (float)j - 0.5f, (float)k - 0.5f};
+        float r2min = (float)(1000 * ng * ng);
+        int pp_s = 0;
+        bool search = true;
+        for (int r = 1; search; ++r)
+          for (int ks = k - r; ks <= k + r; ++ks)
+            for (int js = j - r; js <= j + r; ++js)
+              for (int is = i - r; is <= i + r; ++is) {
+                const int kk = ((ks - 1) % ng + ng) % ng + 1, jj = ((js - 1) % ng + ng) % ng + 1,
+                          ii = ((is - 1) % ng + ng) % ng + 1;
+                for (int pp = hoc_g[cid(ii, jj, kk)]; pp != 0; pp = ll_p[pp]) {
+                  search = false;
+                  float dpos[3];
+                  for (int a = 0; a < 3; ++a) {
+                    const float t = (xp[3 * (pp - 1) + a] - gpos[a]) + (float)(ng / 2);
+                    dpos[a] = (t - std::floor(t / (float)ng) * (float)ng) - (float)(ng / 2);
+                  }
+                  const float r2 = dpos[0] * dpos[0] + dpos[1] * dpos[1] + dpos[2] * dpos[2];
+                  if (r2 < r2min) { r2min = r2; pp_s = pp; }
+                }
+              }
+        ll_g[ig] = hoc_p[pp_s];
+        hoc_p[pp_s] = (int)ig;
+        wp[pp_s - 1] += 1;
+      }
+  for (long c = 0; c < nc; ++c) den[c] = 0.f;
+  for (long ip = 1; ip <= np; ++ip)  // haloden.f90:170-181
+    for (int ig = hoc_p[ip]; ig != 0; ig = ll_g[ig]) den[ig - 1] = den[ig - 1] + 1.f / (float)wp[ip - 1];
+  double sum = 0.0;  // haloden.f90:187, sequential FP64 sum
+  for (long c = 0; c < nc; ++c) sum += (double)den[c];
+  const float fng = (float)ng;
+  for (long c = 0; c < nc; ++c) den[c] = (float)((double)den[c] / sum * (double)(fng * fng * fng));
+}
+
 extern "C" {
 
 #define OR_BOTH(X) X(double, d) X(float, f)
@@ -1176,6 +1243,9 @@ double or_compression_d(double c11, double c12, double c13, double c22, double c
   double comp;
   eigen_limits(c11, c12, c13, c22, c23, c33, 3.14159265358979323846264338328, comp);
   return comp;
+}
+void or_particle_density(const float* xp, long np, int ng, float nf, float* den, int* wp) {
+  particle_density(xp, np, ng, nf, den, wp);
 }
 int or_mgstats_size(void) { return (int)sizeof(MgStats); }
 int or_limstats_size(void) { return (int)sizeof(LimStats); }

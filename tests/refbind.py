@@ -273,3 +273,13 @@ def or_limiter_rhs(defn, u, n, dtold=1.0, dtaumesh=1.0, nfluid=5, compressmax=10
                                    C.c_double(compressmax))
     assert rc == 0, rc
     return tmp
+
+
+def or_particle_density(xp, ng, nf):
+    """Oracle of haloden.f90's density assignment: xp (np, 3) float32 -> (den ng^3, wp np)."""
+    xp = np.ascontiguousarray(xp, dtype=np.float32)
+    npart = xp.shape[0]
+    den = np.zeros(ng ** 3, np.float32)
+    wp = np.zeros(npart, np.int32)
+    oracle().or_particle_density(ptr(xp), C.c_long(npart), ng, C.c_float(nf), ptr(den), ptr(wp))
+    return den, wp
