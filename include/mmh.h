@@ -180,6 +180,17 @@ int mmh_reconstruct_f(mmh_ctx*, float* u, float* def, float* defp, int n, int nf
  * mean 1 (haloden.f90:187).  wp_out (device, np, may be NULL): the cell count of every particle. */
 int mmh_particle_density_f(mmh_ctx*, const float* xp, long long np, int ng, float nf, float* den, int* wp_out);
 
+/* Downsampling (from1024to512.f90:17-23, coarsen 54-66): `levels` rounds of 2x2x2 block sums of
+ * den (n^3) into out ((n >> levels)^3), then scaled to mean 1 with an FP64 total (the reference
+ * divides by an FP32 SUM).  Device arrays, Fortran order. */
+int mmh_coarsen_f(mmh_ctx*, const float* den, int n, int levels, float* out);
+
+/* Column density through the deformed mesh (column_density.f90:14-66): den is scaled to mean 1
+ * (FP64 total), and proj (ng^2) is den integrated along z over the deformed column of the cells
+ * ng/2 .. ng/2 + nplot - 1, the column's ends moved by the mesh's z displacement from def.
+ * Device arrays, Fortran order. */
+int mmh_column_density_f(mmh_ctx*, const float* den, const float* def, int ng, int nplot, float* proj);
+
 /* Binned power spectrum of a real N^3 field (no reference counterpart: the reference reads P(k)
  * from text files, mmh/calcfisherinfo.f90:26-36).  P_b = sum over modes k in bin b of |F(k)|^2 /
  * (N_b N^3), F the unnormalised DFT, N_b the mode count; nbins bins of |k| (grid units, k = 2 pi m / N)

@@ -24,6 +24,7 @@
 #include "kernels_adj.hpp"
 #include "kernels_wave.hpp"
 #include "kernels_pd.hpp"
+#include "kernels_vis.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -1418,6 +1419,42 @@ int mmh_particle_density_f(mmh_ctx* c, const float* xp, long long np, int ng, fl
     k_pd_normalise<<<nblk(ncell), 256, 0, c->stream>>>(den, sum, fng * fng * fng, ncell);
     LAUNCHCHK();
     if (wp_out) HIPCHK(hipMemcpyAsync(wp_out, wp, sizeof(int) * np, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_coarsen_f(mmh_ctx* c, const float* den, int n, int levels, float* out) {
+  return guarded(c, [&] {
+    require(den && out && is_pow2(n) && levels >= 1 && (n >> levels) >= 1, "coarsen: n a power of two, 1 <= levels");
+    const float* src = den;
+    int m = n;
+    for (int l = 0; l < levels; ++l) {
+      const int h = m / 2;
+      float* dst = l == levels - 1 ? out : wsT<float>(c, "crs" + std::to_string(l % 2), cube(h));
+      k_coarsen2<<<nblk(cube(h)), 256, 0, c->stream>>>(dst, src, h);
+      LAUNCHCHK();
+      src = dst;
+      m = h;
+    }
+    double* sum = wsT<double>(c, "crs_sum", 1);
+    reduce<float>(c, out, cube(m), 0, sum);
+    k_scale_mean1<<<nblk(cube(m)), 256, 0, c->stream>>>(out, sum, cube(m));
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_column_density_f(mmh_ctx* c, const float* den, const float* def, int ng, int nplot, float* proj) {
+  return guarded(c, [&] {
+    require(den && def && proj && ng >= 2 && nplot >= 1 && nplot <= ng, "column_density: bad arguments");
+    const i64 n3 = cube(ng);
+    float* d = wsT<float>(c, "cd_den", n3);
+    HIPCHK(hipMemcpyAsync(d, den, sizeof(float) * n3, hipMemcpyDeviceToDevice, c->stream));
+    double* sum = wsT<double>(c, "cd_sum", 1);
+    reduce<float>(c, d, n3, 0, sum);
+    k_scale_mean1<<<nblk(n3), 256, 0, c->stream>>>(d, sum, n3);
+    k_column_density<<<nblk((i64)ng * ng), 256, 0, c->stream>>>(proj, d, def, ng, nplot);
+    LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }

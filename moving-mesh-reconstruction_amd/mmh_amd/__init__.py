@@ -232,6 +232,14 @@ class Context:
         npart = xp.numel() // 3
         self.call("mmh_particle_density_f", _ptr(xp), C.c_longlong(npart), ng, C.c_float(nf), _ptr(den), _ptr(wp))
 
+    def coarsen(self, den, n, levels, out):
+        """from1024to512.f90: levels rounds of 2x2x2 block sums, then mean 1 (float32 device)."""
+        self.call("mmh_coarsen_f", _ptr(den), n, levels, _ptr(out))
+
+    def column_density(self, den, def_, ng, nplot, proj):
+        """column_density.f90: density integrated over the deformed column (float32 device)."""
+        self.call("mmh_column_density_f", _ptr(den), _ptr(def_), ng, nplot, _ptr(proj))
+
     def power(self, field, nbins=16):
         """Binned power spectrum of an N^3 FP64 device field: (P_b, mode counts) as numpy arrays."""
         import numpy as np

@@ -1190,6 +1190,65 @@ static void particle_density(const float* xp_in, long np, int ng, float nf, floa
   for (long c = 0; c < nc; ++c) den[c] = (float)((double)den[c] / sum * (double)(fng * fng * fng));
 }
 
+// ------------------------------------------------------------------------------------------------
+// Downsampling (from1024to512.f90:17-23, 54-66) and the column-density projection
+// (column_density.f90:35-66); mean-1 scaling with a sequential FP64 total.
+static void scale_mean1(float* x, long n3) {
+  double s = 0.0;
+  for (long p = 0; p < n3; ++p) s += (double)x[p];
+  for (long p = 0; p < n3; ++p) x[p] = (float)((double)x[p] / s * (double)n3);
+}
+
+static void coarsen(const float* den, int n, int levels, float* out) {
+  std::vector<float> a(den, den + (size_t)n * n * n);
+  int m = n;
+  for (int l = 0; l < levels; ++l) {
+    const int h = m / 2;
+    std::vector<float> b((size_t)h * h * h);
+    for (int k = 0; k < h; ++k)
+      for (int j = 0; j < h; ++j)
+        for (int i = 0; i < h; ++i) {
+          float s = 0.f;  // sum(den(2i-1:2i, 2j-1:2j, 2k-1:2k)), i fastest
+          for (int c = 0; c < 2; ++c)
+            for (int bb = 0; bb < 2; ++bb)
+              for (int aa = 0; aa < 2; ++aa) s = s + a[(2 * i + aa) + (size_t)m * ((2 * j + bb) + (size_t)m * (2 * k + c))];
+          b[i + (size_t)h * (j + (size_t)h * k)] = s;
+        }
+    a.swap(b);
+    m = h;
+  }
+  std::copy(a.begin(), a.end(), out);
+  scale_mean1(out, (long)m * m * m);
+}
+
+static void column_density(const float* den_in, const float* def, int ng, int nplot, float* proj) {
+  const long n3 = (long)ng * ng * ng;
+  std::vector<float> den(den_in, den_in + n3);
+  scale_mean1(den.data(), n3);
+  auto w = [&](int a) { return ((a - 1) % ng + ng) % ng; };
+  auto D = [&](int i, int j, int k) { return def[w(i) + (size_t)ng * (w(j) + (size_t)ng * w(k))]; };
+  auto dspz = [&](int i, int j, int k) {
+    const int ip = i % ng + 1, jp = j % ng + 1, kp = k % ng + 1;
+    float v = D(i, j, kp) + D(ip, j, kp) + D(i, jp, kp) + D(ip, jp, kp) - D(i, j, k) - D(ip, j, k) - D(i, jp, k) -
+              D(ip, jp, k);
+    return v / 4.f;
+  };
+  const int k1 = ng / 2, k2 = k1 + nplot - 1;
+  for (int j = 1; j <= ng; ++j)
+    for (int i = 1; i <= ng; ++i) {
+      auto dn = [&](int k) { return den[(i - 1) + (size_t)ng * ((j - 1) + (size_t)ng * w(k))]; };
+      const int jm = ((j - 2) % ng + ng) % ng + 1, im = ((i - 2) % ng + ng) % ng + 1, k1m = ((k1 - 2) % ng + ng) % ng + 1;
+      const float kk1 = (float)(k1 - 1) + (dspz(im, jm, k1m) + dspz(i, jm, k1m) + dspz(im, j, k1m) + dspz(i, j, k1m)) / 4.f;
+      const float kk2 = (float)k2 + (dspz(im, jm, k2) + dspz(i, jm, k2) + dspz(im, j, k2) + dspz(i, j, k2)) / 4.f;
+      const int nk1 = (int)std::floor(kk1) + 1, nk2 = (int)std::floor(kk2) + 1;
+      const float r1 = (float)nk1 - kk1, r2 = kk2 - (float)nk2 + 1.f;
+      float s1 = 0.f, s2 = 0.f;
+      for (int c = nk1 + 1; c <= nk2 - 1; ++c) s1 = s1 + dn(c);
+      for (int c = nk2; c <= nk1; ++c) s2 = s2 + dn(c);
+      proj[(i - 1) + (size_t)ng * (j - 1)] = (r1 * dn(nk1) + s1 + r2 * dn(nk2) - s2) / (float)nplot;
+    }
+}
+
 extern "C" {
 
 #define OR_BOTH(X) X(double, d) X(float, f)
@@ -1246,6 +1305,10 @@ double or_compression_d(double c11, double c12, double c13, double c22, double c
 }
 void or_particle_density(const float* xp, long np, int ng, float nf, float* den, int* wp) {
   particle_density(xp, np, ng, nf, den, wp);
+}
+void or_coarsen(const float* den, int n, int levels, float* out) { coarsen(den, n, levels, out); }
+void or_column_density(const float* den, const float* def, int ng, int nplot, float* proj) {
+  column_density(den, def, ng, nplot, proj);
 }
 int or_mgstats_size(void) { return (int)sizeof(MgStats); }
 int or_limstats_size(void) { return (int)sizeof(LimStats); }

@@ -283,3 +283,18 @@ def or_particle_density(xp, ng, nf):
     wp = np.zeros(npart, np.int32)
     oracle().or_particle_density(ptr(xp), C.c_long(npart), ng, C.c_float(nf), ptr(den), ptr(wp))
     return den, wp
+
+
+def or_coarsen(den, n, levels):
+    den = np.ascontiguousarray(den, dtype=np.float32)
+    out = np.zeros((n >> levels) ** 3, np.float32)
+    oracle().or_coarsen(ptr(den), n, levels, ptr(out))
+    return out
+
+
+def or_column_density(den, defn, ng, nplot):
+    den = np.ascontiguousarray(den, dtype=np.float32)
+    defn = np.ascontiguousarray(defn, dtype=np.float32)
+    proj = np.zeros(ng * ng, np.float32)
+    oracle().or_column_density(ptr(den), ptr(defn), ng, nplot, ptr(proj))
+    return proj

@@ -26,3 +26,30 @@ def test_particle_density_matches_oracle(ctx, ng, npart, seed):
     d = den.cpu().numpy()
     assert np.abs(d - den_o).max() <= np.spacing(den_o).max()
     assert np.allclose(d, den_o, rtol=1.2e-7, atol=0)
+
+
+@pytest.mark.parametrize("n,levels", [(32, 1), (64, 3)])
+def test_coarsen_matches_oracle(ctx, n, levels):
+    print("case coarsen n=%d levels=%d" % (n, levels), flush=True)
+    x = (np.random.default_rng(n).random(n ** 3) + 0.5).astype(np.float32)
+    ref = R.or_coarsen(x, n, levels)
+    out = torch.zeros((n >> levels) ** 3, dtype=torch.float32, device="cuda")
+    ctx.coarsen(torch.from_numpy(x).cuda(), n, levels, out)
+    got = out.cpu().numpy()
+    # block sums are bit-identical; the mean-1 scaling uses an FP64 total summed in another order
+    assert np.abs(got - ref).max() <= np.spacing(ref).max()
+
+
+@pytest.mark.parametrize("ng,nplot", [(32, 4), (64, 1)])
+def test_column_density_matches_oracle(ctx, ng, nplot):
+    print("case column density ng=%d nplot=%d" % (ng, nplot), flush=True)
+    rng = np.random.default_rng(ng)
+    den = (rng.random(ng ** 3) + 0.5).astype(np.float32)
+    d = rng.standard_normal(ng ** 3)
+    for _ in range(3):
+        R.or_nsmooth(d, ng)
+    defn = (d * 3.0).astype(np.float32)
+    ref = R.or_column_density(den, defn, ng, nplot)
+    proj = torch.zeros(ng * ng, dtype=torch.float32, device="cuda")
+    ctx.column_density(torch.from_numpy(den).cuda(), torch.from_numpy(defn).cuda(), ng, nplot, proj)
+    assert np.allclose(proj.cpu().numpy(), ref, rtol=1e-6, atol=0)
