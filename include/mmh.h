@@ -60,7 +60,9 @@ typedef struct {
   int schedule;      /* 0: reference sweeps nu = 4*ng/N per level, 32+32 on 4^3; 1: fixed */
   int nu_pre, nu_post, nu_coarse;  /* schedule 1: sweeps per level and on 4^3 */
   int ng;            /* schedule 0: the reference's NG (top size if 0) */
-  int mixed;         /* 0: FP64 V-cycles; 1: FP32 V-cycles inside FP64 defect correction */
+  int mixed;         /* 0: FP64; 1: FP32 V-cycles inside FP64 defect correction, or (krylov) as the
+                        preconditioner of the FP64 PCG; 2 (krylov): FP32 PCG solves of the
+                        correction inside FP64 iterative refinement */
   int krylov;        /* 1: flexible preconditioned CG, one symmetric V-cycle per iteration as the
                         preconditioner (schedule 1 with nu_pre == nu_post; no divergence guard) */
 } mmh_solve_opts;

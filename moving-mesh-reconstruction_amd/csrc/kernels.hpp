@@ -177,8 +177,10 @@ __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __re
 }
 
 // The 19-point symmetric curvilinear operator at one point (mmh/gauss1.f90:954-978), on arrays in
-// layout L.
-template <class T, int L>
+// layout L.  DELTA = true (the FP32 preconditioner of the mixed solve only) returns in t1 the face
+// sum over (phi_nb - phi_c) instead of over phi_nb, so that t1 + tdiag phi_c is formed from
+// differences: in FP32 the update then loses precision to |grad phi| rather than to |phi|.
+template <class T, int L, bool DELTA = false>
 __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __restrict__ G, i64 n3, int n, int i, int j,
                                           int k, T& t1, T& t2, T& tdiag) {
   const T* G1 = G;
@@ -198,8 +200,14 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __
   T b33p = (G6[X_(i, j, kp)] + G6[c]) / T(2);
   T b33m = (G6[X_(i, j, km)] + G6[c]) / T(2);
   tdiag = -(b11p + b11m + b22p + b22m + b33p + b33m);
-  t1 = (b11p * phi[X_(ip, j, k)] + b11m * phi[X_(im, j, k)] + b22p * phi[X_(i, jp, k)] +
-        b22m * phi[X_(i, jm, k)] + b33p * phi[X_(i, j, kp)] + b33m * phi[X_(i, j, km)]);
+  if (DELTA) {
+    const T pc = phi[c];
+    t1 = (b11p * (phi[X_(ip, j, k)] - pc) + b11m * (phi[X_(im, j, k)] - pc) + b22p * (phi[X_(i, jp, k)] - pc) +
+          b22m * (phi[X_(i, jm, k)] - pc) + b33p * (phi[X_(i, j, kp)] - pc) + b33m * (phi[X_(i, j, km)] - pc));
+  } else {
+    t1 = (b11p * phi[X_(ip, j, k)] + b11m * phi[X_(im, j, k)] + b22p * phi[X_(i, jp, k)] +
+          b22m * phi[X_(i, jm, k)] + b33p * phi[X_(i, j, kp)] + b33m * phi[X_(i, j, km)]);
+  }
   t2 = ((phi[X_(ip, jp, k)] - phi[X_(im, jp, k)]) * G2[X_(i, jp, k)]
       - (phi[X_(ip, jm, k)] - phi[X_(im, jm, k)]) * G2[X_(i, jm, k)]
       + (phi[X_(ip, jp, k)] - phi[X_(ip, jm, k)]) * G2[X_(ip, j, k)]
@@ -221,7 +229,7 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __
 // unit-stride.
 // Batched launches: gridDim.y = B grids of the same size stored one after another (bstride = n^3
 // values; G holds 6 n^3 per grid).  remap = 0 keeps the hardware's block order (a tuning choice).
-template <class T, int L>
+template <class T, int L, bool DELTA = false>
 __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G, int n,
                                                 T dx2, int kpar, int io, int jo, i64 bstride = 0, int remap = 1) {
   const int h = n >> 1;
@@ -234,15 +242,16 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
   const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
   const i64 n3 = (i64)n * n * n;
   T t1, t2, tdiag;
-  stencil19<T, L>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+  stencil19<T, L, DELTA>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
   const i64 c = lx<L>(i, j, k, n);
-  phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
+  if (DELTA) phi[c] = phi[c] + (rhs[c] * dx2 - t1 - t2) / tdiag;
+  else phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
 }
 
 // Residual r = rhs - L[phi] (calcresid, mmh/gauss1.f90:683-709); optional store; per-block max|r|.
 // In layout 1 a thread owns one 2x2x2 cell and evaluates its 8 points, one per sub-lattice, so
 // the 8 class arrays and their neighbours are read in lockstep at unit stride.
-template <class T, int L>
+template <class T, int L, bool DELTA = false>
 __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restrict__ bmax, const T* __restrict__ phi,
                                               const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2,
                                               int batched = 0) {
@@ -269,8 +278,8 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
       const int i = 2 * ci + (c & 1), j = 2 * cj + ((c >> 1) & 1), k = 2 * ck + (c >> 2);
       const i64 p = lx<L>(i, j, k, n);
       T t1, t2, tdiag;
-      stencil19<T, L>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
-      T rv = rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
+      stencil19<T, L, DELTA>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+      T rv = DELTA ? rhs[p] - (t1 + t2) * di2 : rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
       if (r) r[p] = rv;
       a = dmax(a, (double)fabs(rv));
     }
@@ -295,7 +304,7 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
 
 // q = L[p] (the operator of k_resid: (t1 + t2 + tdiag*p)/dx^2) and per-block sums of p*q in FP64,
 // block b covering elements [b*RB, (b+1)*RB) combined by the halving tree (deterministic).
-template <class T, int L>
+template <class T, int L, bool DELTA = false>
 __global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __restrict__ bsum, const T* __restrict__ p,
                                                   const T* __restrict__ G, int n, T di2, int batched = 0) {
   __shared__ double s[RB];
@@ -320,8 +329,8 @@ __global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __r
       const int i = 2 * ci + (cc & 1), j = 2 * cj + ((cc >> 1) & 1), k = 2 * ck + (cc >> 2);
       const i64 c = lx<L>(i, j, k, n);
       T t1, t2, tdiag;
-      stencil19<T, L>(p, G, n3, n, i, j, k, t1, t2, tdiag);
-      T v = (t1 + t2 + tdiag * p[c]) * di2;
+      stencil19<T, L, DELTA>(p, G, n3, n, i, j, k, t1, t2, tdiag);
+      T v = DELTA ? (t1 + t2) * di2 : (t1 + t2 + tdiag * p[c]) * di2;
       q[c] = v;
       a = a + (double)p[c] * (double)v;
     }
@@ -391,6 +400,38 @@ __global__ void k_cg_dir(double* __restrict__ p, const double* __restrict__ z, d
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   p[i] = z[i] + beta * p[i];
+}
+
+// CG updates in the working precision T with FP64 scalars (the FP32 Krylov solver).
+template <class T>
+__global__ __launch_bounds__(RB) void k_cg_update_t(T* __restrict__ x, T* __restrict__ r, T* __restrict__ rold,
+                                                    const T* __restrict__ p, const T* __restrict__ q, double alpha,
+                                                    double* __restrict__ bmax, i64 n) {
+  __shared__ double s[RB];
+  const i64 i = (i64)blockIdx.x * RB + threadIdx.x;
+  const T al = (T)alpha;
+  double a = 0.0;
+  if (i < n) {
+    x[i] = x[i] + al * p[i];
+    const T ro = r[i];
+    rold[i] = ro;
+    const T rn = ro - al * q[i];
+    r[i] = rn;
+    a = fabs((double)rn);
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = RB / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bmax[blockIdx.x] = s[0];
+}
+
+template <class T> __global__ void k_cg_dir_t(T* __restrict__ p, const T* __restrict__ z, double beta, i64 n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = z[i] + (T)beta * p[i];
 }
 
 // Batched CG updates: grid y uses alpha[y] / beta[y] (0 once it has converged, which freezes x and r).

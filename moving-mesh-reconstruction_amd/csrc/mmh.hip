@@ -67,7 +67,12 @@ inline void require(bool c, const char* msg) {
 }
 inline bool is_pow2(int n) { return n >= 4 && n <= 4096 && (n & (n - 1)) == 0; }
 __host__ __device__ inline i64 cube(int n) { return (i64)n * n * n; }
-inline unsigned nblk(i64 n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+// Blocks for n work-items.  A dispatch carries its grid size in work-items in 32 bits, so larger
+// launches are split by the caller; reaching this check means one was not.
+inline unsigned nblk(i64 n, int b = 256) {
+  if (n > (i64)0xFFFFFFFFLL - b) throw MmhError(MMH_EINVAL, "internal: launch of more than 2^32 work-items");
+  return (unsigned)((n + b - 1) / b);
+}
 
 void* ws_get(mmh_ctx* c, const std::string& key, size_t bytes) {
   auto it = c->ws.find(key);
@@ -87,7 +92,11 @@ template <class T> T* wsT(mmh_ctx* c, const std::string& key, i64 count) {
 }
 
 template <class T> void fill(mmh_ctx* c, T* a, T v, i64 n) {
-  k_fill<T><<<nblk(n), 256, 0, c->stream>>>(a, v, n);
+  constexpr i64 CH = (i64)1 << 30;
+  for (i64 o = 0; o < n; o += CH) {
+    const i64 m = std::min(CH, n - o);
+    k_fill<T><<<nblk(m), 256, 0, c->stream>>>(a + o, v, m);
+  }
   LAUNCHCHK();
 }
 
@@ -140,6 +149,7 @@ struct Sched {
   int nu_pre = 2, nu_post = 2, nu_coarse = 32;
   bool guard = true;       // the reference's divergence guard on coarse levels
   bool symmetric = false;  // post-smoothing in reverse colour order (symmetric V-cycle)
+  bool delta = false;      // difference form of the FP32 stencil (mixed preconditioner only)
   int pre(int n) const { return kind == 0 ? 4 * ng / n : nu_pre; }
   int post(int n) const { return kind == 0 ? 4 * ng / n : nu_post; }
   int coarse() const { return kind == 0 ? 32 : nu_coarse; }
@@ -196,8 +206,12 @@ inline unsigned stencil_grid(int n, int layout) { return layout == 1 ? nblk(cube
 
 template <class T> void relayout(mmh_ctx* c, T* dst, const T* src, int n, int comps, int to_sl) {
   const i64 cnt = cube(n) * comps;
-  if (to_sl) k_relayout<T, 1><<<nblk(cnt), 256, 0, c->stream>>>(dst, src, n, comps);
-  else k_relayout<T, 0><<<nblk(cnt), 256, 0, c->stream>>>(dst, src, n, comps);
+  (void)cnt;
+  const i64 n3 = cube(n);
+  for (int f = 0; f < comps; ++f) {  // one field per launch (n^3 <= 2^30 work-items)
+    if (to_sl) k_relayout<T, 1><<<nblk(n3), 256, 0, c->stream>>>(dst + f * n3, src + f * n3, n, 1);
+    else k_relayout<T, 0><<<nblk(n3), 256, 0, c->stream>>>(dst + f * n3, src + f * n3, n, 1);
+  }
   LAUNCHCHK();
 }
 
@@ -234,11 +248,12 @@ template <class T> void wave_sweep(mmh_ctx* c, T* out, const T* in, const T* rhs
 }
 
 template <class T>
-void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, int layout = 1, bool reverse = false) {
+void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, int layout = 1, bool reverse = false,
+            bool delta = false) {
   const T dx2 = dx * dx;
   const int h = n / 2;
   const unsigned nb = nblk(cube(h));
-  if (layout == 1 && ns >= 2 && wave_fits<T>(c, n)) {
+  if (layout == 1 && ns >= 2 && !delta && wave_fits<T>(c, n)) {
     // pairs of sweeps through a second buffer; an odd last sweep goes colour by colour below
     T* tmp = wsT<T>(c, "wave_tmp" + std::to_string(n), cube(n));
     const int pairs = ns / 2;
@@ -252,7 +267,8 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, i
     for (int cc = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
       const int kp = col >> 2, io = (col >> 1) & 1, jo = col & 1;
-      if (layout == 1) k_colour<T, 1><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+      if (layout == 1 && delta) k_colour<T, 1, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+      else if (layout == 1) k_colour<T, 1><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
       else k_colour<T, 0><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
     }
   LAUNCHCHK();
@@ -262,12 +278,13 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, i
 // rc != null), max-norm into *rinf_dev, and the calcresid restriction into rc.
 template <class T>
 void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n, T dx, int nrelax, int want_norm, T* rc,
-               double* rinf_dev, bool reverse = false) {
-  sweeps(c, phi, rhs, G, n, dx, nrelax, 1, reverse);
+               double* rinf_dev, bool reverse = false, bool delta = false) {
+  sweeps(c, phi, rhs, G, n, dx, nrelax, 1, reverse, delta);
   if (!want_norm && !rc) return;
   const unsigned nb = stencil_grid(n, 1);
   double* bmax = (double*)ws_get(c, "bmax", sizeof(double) * nb);
-  k_resid<T, 1><<<nb, RB, 0, c->stream>>>(rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx));
+  if (delta) k_resid<T, 1, true><<<nb, RB, 0, c->stream>>>(rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx));
+  else k_resid<T, 1><<<nb, RB, 0, c->stream>>>(rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx));
   LAUNCHCHK();
   if (want_norm) reduce<double>(c, bmax, nb, 1, rinf_dev);
   if (rc) {
@@ -773,12 +790,14 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   }
   const i64 n3 = cube(n);
   double* sc = Hf.scal + 3 * l;
-  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1], sc + 0);
+  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1], sc + 0, false,
+            S.delta);
   fill<float>(c, Hf.u[l + 1], 0.f, cube(n / 2));
   vcycle_mixed(c, Hf, Hd, l + 1, ng_top, S);
   k_prolong<float, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
   LAUNCHCHK();
-  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1, S.symmetric);
+  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1, S.symmetric,
+            S.delta);
   reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
   k_zerosum_guard<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
                                                           Hf.guard + l);
@@ -798,7 +817,9 @@ void build_fp32(mmh_ctx* c, Hier<float>& F, Hier<double>& H) {
     F.r[l] = wsT<float>(c, t + "r", cube(m));
     F.G[l] = wsT<float>(c, t + "G", 6 * cube(m));
     F.def[l] = nullptr; F.u1[l] = nullptr;
-    k_convert<double, float><<<nblk(6 * cube(m)), 256, 0, c->stream>>>(F.G[l], H.G[l], 6 * cube(m));
+    for (int f = 0; f < 6; ++f)  // one coefficient field per launch (n^3 <= 2^30 work-items)
+      k_convert<double, float><<<nblk(cube(m)), 256, 0, c->stream>>>(F.G[l] + f * cube(m), H.G[l] + f * cube(m),
+                                                                     cube(m));
     LAUNCHCHK();
   }
 }
@@ -879,6 +900,84 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   LAUNCHCHK();
 }
 
+// Mixed precision by iterative refinement: the FP64 residual r = b - L x of the current iterate
+// (sub-lattice FP64), an FP32 flexible PCG on L e = r (FP32 operator, vectors and symmetric FP32
+// V-cycle preconditioner with the FP64 4^3 level; FP64 dot products) to a relative reduction of
+// 1e-4, x += e in FP64; until the FP64 residual meets the tolerance.
+template <class T> double dot_t(mmh_ctx* c, const T* a, const T* b, i64 n, double* dst) {
+  const unsigned nb = nblk(n, RB);
+  double* bs = (double*)ws_get(c, "dot_bs", sizeof(double) * nb);
+  k_dot<T><<<nb, RB, 0, c->stream>>>(bs, a, b, n);
+  LAUNCHCHK();
+  reduce<double>(c, bs, nb, 0, dst);
+  return read_scalar<double>(c, dst);
+}
+
+void refine_fp32(mmh_ctx* c, Hier<double>& H, Hier<float>& F, double* x, double* b, int n, const Sched& S,
+                 const mmh_solve_opts* o, mmh_solve_stats* st, double rden, double* sums) {
+  const i64 n3 = cube(n);
+  const double inner_tol = 1e-4;
+  double* r64 = H.r[0];
+  float* e = wsT<float>(c, "rf_e", n3);
+  float* r = wsT<float>(c, "rf_r", n3);
+  float* ro = wsT<float>(c, "rf_ro", n3);
+  float* p = wsT<float>(c, "rf_p", n3);
+  float* q = wsT<float>(c, "rf_q", n3);
+  float* z = wsT<float>(c, "rf_z", n3);
+  const unsigned nb = nblk(n3, RB), ns = stencil_grid(n, 1);
+  double* bm = (double*)ws_get(c, "rf_bm", sizeof(double) * nb);
+  double* bs = (double*)ws_get(c, "rf_bs", sizeof(double) * nb);
+  Hier<float> P = F;
+  P.u[0] = z;
+  P.rhs[0] = r;
+  auto precond = [&] {
+    fill<float>(c, z, 0.f, n3);
+    vcycle_mixed(c, P, H, 0, n, S);
+  };
+  int total = 0, outer = 0;
+  for (;; ++outer) {
+    k_resid<double, 1><<<ns, RB, 0, c->stream>>>(r64, bm, x, b, H.G[0], n, 1.0);
+    LAUNCHCHK();
+    reduce<double>(c, bm, ns, 1, sums + 6);
+    const double rmax = read_scalar<double>(c, sums + 6);
+    const double rr = rmax / rden;
+    if (st && total > 0) st->relres[total - 1] = rr;
+    if (rr <= o->tol) break;
+    if (total >= o->max_cycles) throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
+    k_convert<double, float><<<nblk(n3), 256, 0, c->stream>>>(r, r64, n3);
+    fill<float>(c, e, 0.f, n3);
+    LAUNCHCHK();
+    precond();
+    double rz = dot_t<float>(c, r, z, n3, sums + 4);
+    HIPCHK(hipMemcpyAsync(p, z, sizeof(float) * n3, hipMemcpyDeviceToDevice, c->stream));
+    while (total < o->max_cycles) {
+      k_apply_dot<float, 1, true><<<ns, RB, 0, c->stream>>>(q, bs, p, F.G[0], n, 1.f);
+      LAUNCHCHK();
+      reduce<double>(c, bs, ns, 0, sums + 5);
+      const double alpha = rz / read_scalar<double>(c, sums + 5);
+      k_cg_update_t<float><<<nb, RB, 0, c->stream>>>(e, r, ro, p, q, alpha, bm, n3);
+      LAUNCHCHK();
+      reduce<double>(c, bm, nb, 1, sums + 7);
+      const double rin = read_scalar<double>(c, sums + 7) / rmax;
+      ++total;
+      if (st) { st->relres[total - 1] = rin * rr; st->cycles = total; }
+      if (rin <= inner_tol) break;
+      precond();
+      const double rz_new = dot_t<float>(c, z, r, n3, sums + 4);
+      const double zro = dot_t<float>(c, z, ro, n3, sums + 4);
+      const double beta = (rz_new - zro) / rz;
+      k_cg_dir_t<float><<<nblk(n3), 256, 0, c->stream>>>(p, z, beta, n3);
+      LAUNCHCHK();
+      rz = rz_new;
+    }
+    k_add_f2d<<<nblk(n3), 256, 0, c->stream>>>(x, e, n3);
+    LAUNCHCHK();
+  }
+  reduce<double>(c, x, n3, 0, sums + 2);
+  k_zerosum_guard<double><<<nblk(n3), 256, 0, c->stream>>>(x, sums + 2, H.scal, n3, 0, nullptr);
+  LAUNCHCHK();
+}
+
 void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
                const mmh_solve_opts* o, mmh_solve_stats* st) {
   require(is_pow2(n) && n <= 1024, "solve: n must be a power of two in [4, 1024]");
@@ -895,6 +994,7 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
     S.symmetric = true;
     S.ng = n;
   }
+  S.delta = o->mixed != 0;
   const i64 n3 = cube(n);
   double* us = wsT<double>(c, "sd_us", n3);
   double* rs = wsT<double>(c, "sd_rs", n3);
@@ -922,7 +1022,8 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
     Hier<float> F;
     if (o->mixed) build_fp32(c, F, H);
     try {
-      pcg(c, H, o->mixed ? &F : nullptr, us, rs, n, S, o, st, rden, sums);
+      if (o->mixed == 2) refine_fp32(c, H, F, us, rs, n, S, o, st, rden, sums);
+      else pcg(c, H, o->mixed ? &F : nullptr, us, rs, n, S, o, st, rden, sums);
     } catch (...) {
       finish();
       throw;

@@ -177,7 +177,7 @@ class Context:
               ng=0, mixed=False, nu=None, krylov=False):
         n = _cube_side(u)
         nu = nu or ubig.numel() // u.numel()
-        o = SolveOpts(tol, max_cycles, schedule, nu_pre, nu_post, nu_coarse, ng, 1 if mixed else 0, 1 if krylov else 0)
+        o = SolveOpts(tol, max_cycles, schedule, nu_pre, nu_post, nu_coarse, ng, int(mixed), 1 if krylov else 0)
         st = SolveStats()
         rc = lib().mmh_solve_d(self._c, _ptr(u), _ptr(rhs), _ptr(defpot), _ptr(ubig), n, nu, C.byref(o), C.byref(st))
         if rc not in (MMH_OK, MMH_ENOCONV):

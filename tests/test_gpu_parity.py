@@ -317,3 +317,27 @@ def test_wavefront_vcycle_bitexact(ctx, dt):
     half = len(res) // 2
     for a, b in zip(res[:half], res[half:]):
         assert np.array_equal(a, b)
+
+
+def test_mixed_preconditioner_1024(ctx):
+    """At 1024^3 the FP32 copy of the metric is 6 * 2^30 values, more than one dispatch can cover;
+    three PCG iterations with the FP32 preconditioner must track the FP64 ones (rel 1e-6)."""
+    n = 1024
+    print("case mixed preconditioner n=%d" % n, flush=True)
+    f64 = dict(dtype=torch.float64, device="cuda:0")
+    rho = torch.zeros(n ** 3, **f64)
+    ctx.lognormal(rho, 2.0, 42)
+    d = torch.zeros(n ** 3, **f64)
+    rhs0 = (1.0 - rho) / 30.0
+    out = []
+    for mixed in (0, 1):
+        phi, rhs = torch.zeros(n ** 3, **f64), rhs0.clone()
+        st, ok = ctx.solve(phi, rhs, d, rho, tol=1e-30, max_cycles=3, schedule=1, nu_pre=2, nu_post=2, nu_coarse=8,
+                           mixed=mixed, nu=1, krylov=True)
+        out.append((phi, [st.relres[i] for i in range(st.cycles)]))
+        del rhs
+    ctx.release_workspace()
+    (a, ra), (b, rb) = out
+    print("relres fp64 %s mixed %s" % (ra, rb), flush=True)
+    assert float((a - b).abs().max() / a.abs().max()) <= 1e-6
+    assert all(abs(x - y) <= 1e-5 * abs(x) for x, y in zip(ra, rb))
