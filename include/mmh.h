@@ -107,6 +107,8 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 (it measures slower than the colour kernels on MI355X; kept as an option)
  *   "wave_min_n"  smallest level size for the wavefront kernel (default 128)
  *   "wave_p"      1 (default), 2 or 4 points per thread in flight per colour phase
+ *   "mixed_delta" 0 (default) / 1: the FP32 preconditioner's stencil sums neighbour differences
+ *                 (changes the mixed path's FP32 rounding, not its converged result)
  *   "batch_graph", "batch_autotune": see mmh_solve_batch_d */
 int mmh_set_option(mmh_ctx* ctx, const char* key, int value);
 

@@ -37,6 +37,7 @@ struct mmh_ctx {
   std::set<std::string> uploaded;  // constant tables already copied to the device
   std::string err;
   int sweep_impl = 0;  // mmh_time_sweeps layout: 0 sub-lattice (the V-cycle's), 1 column-major
+  int mixed_delta = 0;     // FP32 preconditioner stencil in difference form (option "mixed_delta")
   int wave = 0;            // whole-sweep wavefront kernel for levels with n >= wave_min_n (opt-in)
   int wave_min_n = 128;
   int wave_p = 1;          // points per thread in flight per colour phase of the wavefront kernel
@@ -994,7 +995,7 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
     S.symmetric = true;
     S.ng = n;
   }
-  S.delta = o->mixed != 0;
+  S.delta = o->mixed != 0 && c->mixed_delta;
   const i64 n3 = cube(n);
   double* us = wsT<double>(c, "sd_us", n3);
   double* rs = wsT<double>(c, "sd_rs", n3);
@@ -1338,6 +1339,8 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
     if (k == "sweep_impl") {
       require(value >= 0 && value <= 2, "set_option: sweep_impl is 0, 1 or 2");
       c->sweep_impl = value;
+    } else if (k == "mixed_delta") {
+      c->mixed_delta = value ? 1 : 0;
     } else if (k == "wave") {
       c->wave = value ? 1 : 0;
     } else if (k == "wave_p") {
