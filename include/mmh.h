@@ -109,6 +109,10 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *   "wave_p"      1 (default), 2 or 4 points per thread in flight per colour phase
  *   "mixed_delta" 0 (default) / 1: the FP32 preconditioner's stencil sums neighbour differences
  *                 (changes the mixed path's FP32 rounding, not its converged result)
+ *   "pcg_graph"   1 (default) / 0: the Krylov solve (mmh_solve_d, krylov = 1) captures its
+ *                 preconditioning V-cycle as a HIP graph on its second call and replays it after;
+ *                 on the null stream the solve then runs on a stream of its own, ordered after
+ *                 the work already queued
  *   "batch_graph", "batch_autotune": see mmh_solve_batch_d */
 int mmh_set_option(mmh_ctx* ctx, const char* key, int value);
 

@@ -270,26 +270,8 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
   S.kind = 1; S.nu_pre = o->nu_pre; S.nu_post = o->nu_post; S.nu_coarse = o->nu_coarse;
   S.guard = false; S.symmetric = true;
   require(S.nu_pre >= 1 && S.nu_coarse >= 1, "solve_batch: sweep counts >= 1");
-  // Graph capture needs a stream other than the null stream: the batch runs on its own stream
-  // (default priority), ordered after the work already queued on the context's stream.
-  struct StreamSwap {
-    mmh_ctx* c;
-    hipStream_t saved, own = nullptr;
-    hipEvent_t ev = nullptr;
-    explicit StreamSwap(mmh_ctx* cc) : c(cc), saved(cc->stream) {
-      HIPCHK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
-      HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      HIPCHK(hipEventRecord(ev, saved));
-      HIPCHK(hipStreamWaitEvent(own, ev, 0));
-      c->stream = own;
-    }
-    ~StreamSwap() {
-      (void)hipStreamSynchronize(own);
-      c->stream = saved;
-      (void)hipEventDestroy(ev);
-      (void)hipStreamDestroy(own);
-    }
-  } swap_guard(c);
+  // Graph capture needs a stream other than the null stream: the batch runs on its own stream.
+  StreamSwap swap_guard(c);
   std::map<int, std::vector<int>> bysize;
   for (int i = 0; i < count; ++i) {
     require(is_pow2(ns[i]) && ns[i] >= 8 && ns[i] <= 1024, "solve_batch: sizes must be powers of two in [8, 1024]");
@@ -335,27 +317,20 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
         if (v) return true;
       return false;
     };
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     int it = 0;
     if (any_active()) {
       // the first iteration runs eagerly (it also sizes every workspace), the rest replay its graph
+      Replay graph(c);
       batch_iteration(c, g, S, o->tol);
       ++it;
-      if (c->batch_graph && it < o->max_cycles && any_active()) {
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        batch_iteration(c, g, S, o->tol);
-        HIPCHK(hipStreamEndCapture(c->stream, &graph));
-        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-      }
+      if (c->batch_graph && it < o->max_cycles && any_active())
+        graph.capture([&] { batch_iteration(c, g, S, o->tol); });
       while (it < o->max_cycles && any_active()) {
-        if (exec) HIPCHK(hipGraphLaunch(exec, c->stream));
+        if (graph.exec) graph.launch();
         else batch_iteration(c, g, S, o->tol);
         ++it;
       }
     }
-    if (exec) HIPCHK(hipGraphExecDestroy(exec));
-    if (graph) HIPCHK(hipGraphDestroy(graph));
     const bool group_ok = !any_active();
     all_ok = all_ok && group_ok;
     // true residual of the final iterates, mean removal, back to the caller's layout

@@ -371,10 +371,41 @@ __global__ __launch_bounds__(RB) void k_dot(double* __restrict__ bsum, const T* 
   if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
 }
 
-// x += alpha p; r -= alpha q (the previous r kept in rold); per-block max|r|.
+// Per-block sums of z*r and z*ro in FP64 (z widened exactly from its storage type), with the
+// blocking of k_dot: the two dot products of one flexible-CG iteration in one pass over z.
+template <class TZ>
+__global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __restrict__ b2, const TZ* __restrict__ z,
+                                             const double* __restrict__ r, const double* __restrict__ ro, i64 n) {
+  __shared__ double s1[RB], s2[RB];
+  const int t = threadIdx.x;
+  const i64 p = (i64)blockIdx.x * RB + t;
+  double a1 = 0.0, a2 = 0.0;
+  if (p < n) {
+    const double zz = (double)z[p];
+    a1 = zz * r[p];
+    a2 = zz * ro[p];
+  }
+  s1[t] = a1;
+  s2[t] = a2;
+  __syncthreads();
+  for (int h = RB / 2; h > 0; h >>= 1) {
+    if (t < h) {
+      s1[t] = s1[t] + s1[t + h];
+      s2[t] = s2[t] + s2[t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    b1[blockIdx.x] = s1[0];
+    b2[blockIdx.x] = s2[0];
+  }
+}
+
+// x += alpha p; r -= alpha q (the previous r kept in rold); per-block max|r|.  With r32, the new r
+// is also stored rounded to FP32 (the right-hand side of an FP32 preconditioner).
 __global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ x, double* __restrict__ r, double* __restrict__ rold,
                                                   const double* __restrict__ p, const double* __restrict__ q, double alpha,
-                                                  double* __restrict__ bmax, i64 n) {
+                                                  double* __restrict__ bmax, i64 n, float* __restrict__ r32 = nullptr) {
   __shared__ double s[RB];
   const i64 i = (i64)blockIdx.x * RB + threadIdx.x;
   double a = 0.0;
@@ -384,6 +415,7 @@ __global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ x, double
     rold[i] = ro;
     const double rn = ro - alpha * q[i];
     r[i] = rn;
+    if (r32) r32[i] = (float)rn;
     a = fabs(rn);
   }
   s[threadIdx.x] = a;
@@ -396,10 +428,10 @@ __global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ x, double
 }
 
 // p = z + beta p.
-__global__ void k_cg_dir(double* __restrict__ p, const double* __restrict__ z, double beta, i64 n) {
+template <class TZ> __global__ void k_cg_dir(double* __restrict__ p, const TZ* __restrict__ z, double beta, i64 n) {
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  p[i] = z[i] + beta * p[i];
+  p[i] = (double)z[i] + beta * p[i];
 }
 
 // CG updates in the working precision T with FP64 scalars (the FP32 Krylov solver).

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <set>
 #include <stdexcept>
@@ -41,7 +42,9 @@ struct mmh_ctx {
   int wave = 0;            // whole-sweep wavefront kernel for levels with n >= wave_min_n (opt-in)
   int wave_min_n = 128;
   int wave_p = 1;          // points per thread in flight per colour phase of the wavefront kernel
+  int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
+  int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
   int batch_autotune = 1;  // batched solve: pick the colour kernel's launch shape by timing
   std::map<std::string, std::pair<int, int>> tune;  // (grid size, batch) -> (block size, XCD remap)
 };
@@ -79,6 +82,8 @@ void* ws_get(mmh_ctx* c, const std::string& key, size_t bytes) {
   auto it = c->ws.find(key);
   if (it != c->ws.end() && it->second.second >= bytes) return it->second.first;
   if (it != c->ws.end()) {
+    // a captured graph keeps the old address in its kernel arguments
+    if (c->graphs_live) throw MmhError(MMH_EDEVICE, "internal: workspace '" + key + "' grows while a graph is live");
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipFree(it->second.first));
     c->ws.erase(it);
@@ -100,6 +105,60 @@ template <class T> void fill(mmh_ctx* c, T* a, T v, i64 n) {
   }
   LAUNCHCHK();
 }
+
+// Graph capture needs a stream other than the null stream: work that is captured runs on a stream
+// of the library's own (default priority), ordered after the work already queued on the context's
+// stream; the context's stream is restored, after a synchronisation, when the guard goes.
+struct StreamSwap {
+  mmh_ctx* c;
+  hipStream_t saved, own = nullptr;
+  hipEvent_t ev = nullptr;
+  explicit StreamSwap(mmh_ctx* cc) : c(cc), saved(cc->stream) {
+    HIPCHK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev, saved));
+    HIPCHK(hipStreamWaitEvent(own, ev, 0));
+    c->stream = own;
+  }
+  ~StreamSwap() {
+    (void)hipStreamSynchronize(own);
+    c->stream = saved;
+    (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(own);
+  }
+};
+
+// A sequence of launches captured once on the context's stream and then replayed.  While it
+// exists, workspace buffers may be added but not moved (ws_get refuses to grow one).
+struct Replay {
+  mmh_ctx* c;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  explicit Replay(mmh_ctx* cc) : c(cc) {}
+  template <class F> void capture(F&& work) {
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      work();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(c->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(c->stream, &graph));
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    ++c->graphs_live;
+  }
+  void launch() { HIPCHK(hipGraphLaunch(exec, c->stream)); }
+  ~Replay() {
+    if (exec) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipGraphExecDestroy(exec);
+      --c->graphs_live;
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
 
 // FP64 sum (op 0) or max|x| (op 1) of x[0..n) into the device double *dst, blocked tree order.
 template <class T> void reduce(mmh_ctx* c, const T* x, i64 n, int op, double* dst) {
@@ -845,38 +904,72 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   double* ro = wsT<double>(c, "cg_ro", n3);
   double* p = wsT<double>(c, "cg_p", n3);
   double* q = wsT<double>(c, "cg_q", n3);
-  double* z = wsT<double>(c, "cg_z", n3);
+  // z = M r: FP64, or with the FP32 preconditioner the FP32 V-cycle's own solution array, read
+  // (widened exactly) by the dot products and the direction update; r reaches the FP32 V-cycle as
+  // its right-hand side, rounded by the r update itself
+  double* z = F ? nullptr : wsT<double>(c, "cg_z", n3);
+  float* z32 = F ? F->u[0] : nullptr;
+  float* r32 = F ? F->rhs[0] : nullptr;
   const unsigned nb = nblk(n3, RB), ns = stencil_grid(n, 1);
   double* bm = (double*)ws_get(c, "cg_bm", sizeof(double) * nb);
   double* bs = (double*)ws_get(c, "cg_bs", sizeof(double) * nb);
+  double* bs2 = (double*)ws_get(c, "cg_bs2", sizeof(double) * nb);
+  // The preconditioner is device work only: several hundred launches per V-cycle, most of them on
+  // the 4^3..32^3 levels.  The first call runs eagerly (it sizes every workspace it uses); the
+  // second is captured and every later call replays the graph (option "pcg_graph").
+  std::unique_ptr<StreamSwap> own_stream;
+  if (c->pcg_graph && c->stream == nullptr) own_stream.reset(new StreamSwap(c));
+  Replay graph(c);
+  int ncalls = 0;
   Hier<double> P = H;
   P.u[0] = z;
   P.rhs[0] = r;
-  auto precond = [&] {
+  auto precond_work = [&] {
     if (!F) {
       fill<double>(c, z, 0.0, n3);
       vcycle<double>(c, P, 0, n, S);
     } else {
-      k_convert<double, float><<<nblk(n3), 256, 0, c->stream>>>(F->rhs[0], r, n3);
-      fill<float>(c, F->u[0], 0.f, n3);
-      LAUNCHCHK();
+      fill<float>(c, z32, 0.f, n3);
       vcycle_mixed(c, *F, H, 0, n, S);
-      k_convert<float, double><<<nblk(n3), 256, 0, c->stream>>>(z, F->u[0], n3);
-      LAUNCHCHK();
     }
+  };
+  auto precond = [&] {
+    if (!c->pcg_graph || ncalls++ == 0) {
+      precond_work();
+      return;
+    }
+    if (!graph.exec) graph.capture(precond_work);
+    graph.launch();
+  };
+  // z.r and z.ro of the current z (deterministic block sums, one read-back for both)
+  auto dots = [&](const double* rr, const double* rro, double& zr, double& zro) {
+    if (F) k_dot2<float><<<nb, RB, 0, c->stream>>>(bs, bs2, z32, rr, rro, n3);
+    else k_dot2<double><<<nb, RB, 0, c->stream>>>(bs, bs2, z, rr, rro, n3);
+    LAUNCHCHK();
+    reduce<double>(c, bs, nb, 0, sums + 8);
+    reduce<double>(c, bs2, nb, 0, sums + 9);
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, sums + 8, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    zr = h[0];
+    zro = h[1];
   };
   k_resid<double, 1><<<ns, RB, 0, c->stream>>>(r, bm, x, b, H.G[0], n, 1.0);
   LAUNCHCHK();
+  if (F) k_convert<double, float><<<nblk(n3), 256, 0, c->stream>>>(r32, r, n3);
   precond();
-  double rz = dot_dev(c, r, z, n3, sums + 4);
-  HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  double rz, unused;
+  dots(r, r, rz, unused);
+  if (F) k_convert<float, double><<<nblk(n3), 256, 0, c->stream>>>(p, z32, n3);
+  else HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  LAUNCHCHK();
   for (int it = 0; it < o->max_cycles; ++it) {
     k_apply_dot<double, 1><<<ns, RB, 0, c->stream>>>(q, bs, p, H.G[0], n, 1.0);
     LAUNCHCHK();
     reduce<double>(c, bs, ns, 0, sums + 5);
     const double pq = read_scalar<double>(c, sums + 5);
     const double alpha = rz / pq;
-    k_cg_update<<<nb, RB, 0, c->stream>>>(x, r, ro, p, q, alpha, bm, n3);
+    k_cg_update<<<nb, RB, 0, c->stream>>>(x, r, ro, p, q, alpha, bm, n3, r32);
     LAUNCHCHK();
     reduce<double>(c, bm, nb, 1, sums + 6);
     const double rr = read_scalar<double>(c, sums + 6) / rden;
@@ -884,10 +977,11 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
     if (rr <= o->tol) break;
     if (it + 1 == o->max_cycles) throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
     precond();
-    const double rz_new = dot_dev(c, z, r, n3, sums + 4);
-    const double zro = dot_dev(c, z, ro, n3, sums + 4);
+    double rz_new, zro;
+    dots(r, ro, rz_new, zro);
     const double beta = (rz_new - zro) / rz;
-    k_cg_dir<<<nblk(n3), 256, 0, c->stream>>>(p, z, beta, n3);
+    if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(p, z32, beta, n3);
+    else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(p, z, beta, n3);
     LAUNCHCHK();
     rz = rz_new;
   }
@@ -1339,6 +1433,8 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
     if (k == "sweep_impl") {
       require(value >= 0 && value <= 2, "set_option: sweep_impl is 0, 1 or 2");
       c->sweep_impl = value;
+    } else if (k == "pcg_graph") {
+      c->pcg_graph = value ? 1 : 0;
     } else if (k == "mixed_delta") {
       c->mixed_delta = value ? 1 : 0;
     } else if (k == "wave") {

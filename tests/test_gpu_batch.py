@@ -88,3 +88,27 @@ def test_batch_zero_rhs_item(ctx, probs):
     st, ok = ctx.solve_batch(items, **SOLVE)
     assert ok and st[0].cycles == 0 and st[0].converged
     assert float(items[0][0].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_single_solve_graph_replay_equals_eager(ctx, probs, mixed):
+    """The single solve replays its PCG preconditioner (one V-cycle, FP64 or FP32) as a captured
+    graph from the second call on, tens of times per solve with eager CG work in between; every
+    iterate must equal the all-eager solve's, so the residual histories and solutions match."""
+    print("case single-solve graph vs eager, mixed=%d" % mixed, flush=True)
+    for (u, r, d), case in zip(probs, CASES):
+        out = []
+        for graph in (1, 0):
+            ctx.set_option("pcg_graph", graph)
+            try:
+                phi = torch.zeros_like(r)
+                st, ok = ctx.solve(phi, r.clone(), d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
+                                   nu_coarse=8, mixed=mixed, nu=5, krylov=True)
+            finally:
+                ctx.set_option("pcg_graph", 1)
+            assert ok, case
+            out.append((phi.cpu().numpy(), [st.relres[i] for i in range(st.cycles)]))
+        (xg, rg), (xe, re) = out
+        assert len(rg) > 3, case
+        assert rg == re, case
+        assert np.array_equal(xg, xe), case
