@@ -94,8 +94,7 @@ void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, co
   }
   const int pre = S.pre(m);
   batch_sweeps(c, g, u, rh, g.G[l], l, pre, false, shape);
-  const unsigned ns = stencil_grid(m, 1);
-  k_resid<double, 1><<<dim3(ns, g.B), RB, 0, c->stream>>>(g.r[l], g.bm, u, rh, g.G[l], m, 1.0 / (g.dx[l] * g.dx[l]), 1);
+  op_resid<double>(c, g.r[l], g.bm, u, rh, g.G[l], m, 1.0 / (g.dx[l] * g.dx[l]), false, g.B, 1);
   const int nrmax = std::min(4, std::max(1, m / 4));
   const int nrl = pre > nrmax ? (pre % nrmax ? pre % nrmax : nrmax) : pre;
   k_restrict_resid<double, 1><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(g.rhs[l + 1], g.r[l], m,
@@ -129,9 +128,8 @@ void batch_precond(mmh_ctx* c, BatchGroup& g, const Sched& S) {
 // One flexible-PCG iteration for every grid of the batch (the loop body of pcg()).
 void batch_iteration(mmh_ctx* c, BatchGroup& g, const Sched& S, double tol) {
   const i64 n3 = cube(g.n);
-  const unsigned ns = stencil_grid(g.n, 1), nb = nblk(n3, RB), sb = nblk(g.B, 64);
-  k_apply_dot<double, 1><<<dim3(ns, g.B), RB, 0, c->stream>>>(g.q, g.bs, g.p, g.G[0], g.n, 1.0, 1);
-  LAUNCHCHK();
+  const unsigned nb = nblk(n3, RB), sb = nblk(g.B, 64);
+  const unsigned ns = op_apply<double>(c, g.q, g.bs, g.p, g.G[0], g.n, 1.0, g.B, 1);
   batch_reduce(c, g, g.bs, ns, 0, g.pq);
   k_cg_scalars<<<sb, 64, 0, c->stream>>>(0, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
                                          g.active, g.iters, g.relres);
@@ -303,9 +301,7 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     }
     if (c->batch_autotune) batch_autotune(c, g);
     // r = b - L x, z = M r, rz = <r, z>, p = z
-    const unsigned nsg = stencil_grid(g.n, 1);
-    k_resid<double, 1><<<dim3(nsg, g.B), RB, 0, c->stream>>>(g.cr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, 1);
-    LAUNCHCHK();
+    op_resid<double>(c, g.cr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, false, g.B, 1);
     batch_precond(c, g, S);
     batch_dot(c, g, g.cr, g.z, g.rz);
     HIPCHK(hipMemcpyAsync(g.p, g.z, sizeof(double) * g.B * n3, hipMemcpyDeviceToDevice, c->stream));
@@ -334,9 +330,8 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     const bool group_ok = !any_active();
     all_ok = all_ok && group_ok;
     // true residual of the final iterates, mean removal, back to the caller's layout
-    k_resid<double, 1><<<dim3(nsg, g.B), RB, 0, c->stream>>>(nullptr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, 1);
-    LAUNCHCHK();
-    batch_reduce(c, g, g.bm, nsg, 1, g.rmax);
+    const unsigned nsr = op_resid<double>(c, (double*)nullptr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, false, g.B, 1);
+    batch_reduce(c, g, g.bm, nsr, 1, g.rmax);
     batch_reduce(c, g, g.x, n3, 0, g.sums);
     k_zerosum_guard<double><<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.sums, nullptr, n3, 0, nullptr, 1);
     LAUNCHCHK();

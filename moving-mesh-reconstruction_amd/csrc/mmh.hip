@@ -26,6 +26,7 @@
 #include "kernels_wave.hpp"
 #include "kernels_pd.hpp"
 #include "kernels_vis.hpp"
+#include "kernels_march.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -42,6 +43,7 @@ struct mmh_ctx {
   int wave = 0;            // whole-sweep wavefront kernel for levels with n >= wave_min_n (opt-in)
   int wave_min_n = 128;
   int wave_p = 1;          // points per thread in flight per colour phase of the wavefront kernel
+  int march_op = 1;        // residual/operator of large levels by k-marching workgroups (option "march_op")
   int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
@@ -264,6 +266,59 @@ void build_hier(mmh_ctx* c, Hier<T>& H, const std::string& tag, T* u, T* rhs, co
 // one per 2x2x2 cell in layout 1.
 inline unsigned stencil_grid(int n, int layout) { return layout == 1 ? nblk(cube(n / 2), RB) : nblk(cube(n), RB); }
 
+// Tiles of the k-marching kernels: 256 threads, rows of 32 FP64 (256 B) or 64 FP32 values.
+template <class T> struct MarchTile;
+template <> struct MarchTile<double> { static constexpr int TI = 32, TJ = 8; };
+template <> struct MarchTile<float> { static constexpr int TI = 64, TJ = 4; };
+
+// Planes per workgroup of k_op_march for a level of size n: the largest of 32, 16, 8 leaving at
+// least 2048 workgroups (8 at least, so the block results never outnumber stencil_grid's); 0 where
+// the marching kernel does not apply (small levels, the option off).
+template <class T> int march_ks(const mmh_ctx* c, int n) {
+  constexpr int TI = MarchTile<T>::TI, TJ = MarchTile<T>::TJ;
+  if (!c->march_op || n < 64 || n % TI || n % TJ || n % 8) return 0;
+  for (int ks = 32; ks > 8; ks >>= 1)
+    if (n % ks == 0 && cube(n) / ((i64)TI * TJ * ks) >= 2048) return ks;
+  return 8;
+}
+
+// r = rhs - L phi on a sub-lattice level (r may be null) with per-block max|r| into bres, for B
+// grids stored one after another when batched; returns the number of block results per grid.
+template <class T>
+unsigned op_resid(mmh_ctx* c, T* r, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2,
+                  bool delta = false, int B = 1, int batched = 0) {
+  constexpr int TI = MarchTile<T>::TI, TJ = MarchTile<T>::TJ;
+  const int ks = delta ? 0 : march_ks<T>(c, n);
+  unsigned nb;
+  if (ks) {
+    nb = (unsigned)(cube(n) / ((i64)TI * TJ * ks));
+    k_op_march<T, 0, TI, TJ><<<dim3(nb, B), TI * TJ, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, ks, batched);
+  } else {
+    nb = stencil_grid(n, 1);
+    if (delta) k_resid<T, 1, true><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched);
+    else k_resid<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched);
+  }
+  LAUNCHCHK();
+  return nb;
+}
+
+// q = L p with per-block sums of p*q into bres (same conventions as op_resid).
+template <class T>
+unsigned op_apply(mmh_ctx* c, T* q, double* bres, const T* p, const T* G, int n, T di2, int B = 1, int batched = 0) {
+  constexpr int TI = MarchTile<T>::TI, TJ = MarchTile<T>::TJ;
+  const int ks = march_ks<T>(c, n);
+  unsigned nb;
+  if (ks) {
+    nb = (unsigned)(cube(n) / ((i64)TI * TJ * ks));
+    k_op_march<T, 1, TI, TJ><<<dim3(nb, B), TI * TJ, 0, c->stream>>>(q, bres, p, nullptr, G, n, di2, ks, batched);
+  } else {
+    nb = stencil_grid(n, 1);
+    k_apply_dot<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(q, bres, p, G, n, di2, batched);
+  }
+  LAUNCHCHK();
+  return nb;
+}
+
 template <class T> void relayout(mmh_ctx* c, T* dst, const T* src, int n, int comps, int to_sl) {
   const i64 cnt = cube(n) * comps;
   (void)cnt;
@@ -341,11 +396,8 @@ void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n,
                double* rinf_dev, bool reverse = false, bool delta = false) {
   sweeps(c, phi, rhs, G, n, dx, nrelax, 1, reverse, delta);
   if (!want_norm && !rc) return;
-  const unsigned nb = stencil_grid(n, 1);
-  double* bmax = (double*)ws_get(c, "bmax", sizeof(double) * nb);
-  if (delta) k_resid<T, 1, true><<<nb, RB, 0, c->stream>>>(rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx));
-  else k_resid<T, 1><<<nb, RB, 0, c->stream>>>(rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx));
-  LAUNCHCHK();
+  double* bmax = (double*)ws_get(c, "bmax", sizeof(double) * stencil_grid(n, 1));
+  const unsigned nb = op_resid<T>(c, rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx), delta);
   if (want_norm) reduce<double>(c, bmax, nb, 1, rinf_dev);
   if (rc) {
     const int nrmax = std::min(4, std::max(1, n / 4));
@@ -910,7 +962,7 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   double* z = F ? nullptr : wsT<double>(c, "cg_z", n3);
   float* z32 = F ? F->u[0] : nullptr;
   float* r32 = F ? F->rhs[0] : nullptr;
-  const unsigned nb = nblk(n3, RB), ns = stencil_grid(n, 1);
+  const unsigned nb = nblk(n3, RB);
   double* bm = (double*)ws_get(c, "cg_bm", sizeof(double) * nb);
   double* bs = (double*)ws_get(c, "cg_bs", sizeof(double) * nb);
   double* bs2 = (double*)ws_get(c, "cg_bs2", sizeof(double) * nb);
@@ -954,8 +1006,7 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
     zr = h[0];
     zro = h[1];
   };
-  k_resid<double, 1><<<ns, RB, 0, c->stream>>>(r, bm, x, b, H.G[0], n, 1.0);
-  LAUNCHCHK();
+  op_resid<double>(c, r, bm, x, b, H.G[0], n, 1.0);
   if (F) k_convert<double, float><<<nblk(n3), 256, 0, c->stream>>>(r32, r, n3);
   precond();
   double rz, unused;
@@ -964,9 +1015,8 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   else HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
   LAUNCHCHK();
   for (int it = 0; it < o->max_cycles; ++it) {
-    k_apply_dot<double, 1><<<ns, RB, 0, c->stream>>>(q, bs, p, H.G[0], n, 1.0);
-    LAUNCHCHK();
-    reduce<double>(c, bs, ns, 0, sums + 5);
+    const unsigned nq = op_apply<double>(c, q, bs, p, H.G[0], n, 1.0);
+    reduce<double>(c, bs, nq, 0, sums + 5);
     const double pq = read_scalar<double>(c, sums + 5);
     const double alpha = rz / pq;
     k_cg_update<<<nb, RB, 0, c->stream>>>(x, r, ro, p, q, alpha, bm, n3, r32);
@@ -986,9 +1036,8 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
     rz = rz_new;
   }
   // report the true residual of the final iterate and remove its mean
-  k_resid<double, 1><<<ns, RB, 0, c->stream>>>(nullptr, bm, x, b, H.G[0], n, 1.0);
-  LAUNCHCHK();
-  reduce<double>(c, bm, ns, 1, sums + 6);
+  const unsigned nr = op_resid<double>(c, (double*)nullptr, bm, x, b, H.G[0], n, 1.0);
+  reduce<double>(c, bm, nr, 1, sums + 6);
   if (st && st->cycles > 0) st->relres[st->cycles - 1] = read_scalar<double>(c, sums + 6) / rden;
   reduce<double>(c, x, n3, 0, sums + 2);
   k_zerosum_guard<double><<<nblk(n3), 256, 0, c->stream>>>(x, sums + 2, H.scal, n3, 0, nullptr);
@@ -1142,17 +1191,15 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
   const unsigned nb = stencil_grid(n, 1);
   double* bmax = (double*)ws_get(c, "sbmax", sizeof(double) * nb);
   for (int cyc = 0; cyc < o->max_cycles; ++cyc) {
-    k_resid<double, 1><<<nb, RB, 0, c->stream>>>(r64, bmax, us, rs, H.G[0], n, 1.0);
-    LAUNCHCHK();
+    op_resid<double>(c, r64, bmax, us, rs, H.G[0], n, 1.0);
     k_convert<double, float><<<nblk(n3), 256, 0, c->stream>>>(F.rhs[0], r64, n3);
     fill<float>(c, F.u[0], 0.f, n3);
     LAUNCHCHK();
     vcycle_mixed(c, F, H, 0, S.ng, S);
     k_add_f2d<<<nblk(n3), 256, 0, c->stream>>>(us, F.u[0], n3);
     LAUNCHCHK();
-    k_resid<double, 1><<<nb, RB, 0, c->stream>>>(nullptr, bmax, us, rs, H.G[0], n, 1.0);
-    LAUNCHCHK();
-    reduce<double>(c, bmax, nb, 1, sums + 3);
+    const unsigned nr = op_resid<double>(c, (double*)nullptr, bmax, us, rs, H.G[0], n, 1.0);
+    reduce<double>(c, bmax, nr, 1, sums + 3);
     const double rr = read_scalar<double>(c, sums + 3) / rden;
     if (st) { st->relres[cyc] = rr; st->cycles = cyc + 1; }
     if (rr <= o->tol) {
@@ -1433,6 +1480,8 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
     if (k == "sweep_impl") {
       require(value >= 0 && value <= 2, "set_option: sweep_impl is 0, 1 or 2");
       c->sweep_impl = value;
+    } else if (k == "march_op") {
+      c->march_op = value ? 1 : 0;
     } else if (k == "pcg_graph") {
       c->pcg_graph = value ? 1 : 0;
     } else if (k == "mixed_delta") {
