@@ -6,10 +6,12 @@
 // need.  Here a workgroup owns a TI x TJ column of points and walks KS consecutive planes of it:
 //   - phi of four consecutive planes (with a one-point halo in i and j) sits in an LDS ring, and
 //     each step loads one new plane, so phi is fetched from HBM about once;
-//   - G3, G5 and G6 at the thread's own point of planes k-1, k, k+1 are carried in registers
-//     (each step loads plane k+1), so the k-direction metric reads are not repeated;
-//   - the in-plane neighbours' metric values are the ones the neighbouring threads load in the
-//     same step, so those reads hit the caches.
+//   - G3, G5 and G6 at the thread's own point of planes k-1, k, k+1 are carried in registers,
+//     so the k-direction metric reads are not repeated;
+//   - G1..G5 of the current plane (own points plus the halo values the in-plane neighbours need)
+//     are staged in LDS, so each metric value is loaded once;
+//   - loads run one plane ahead of the arithmetic (software pipeline), so a load has a whole
+//     step to arrive.
 // Every point evaluates the expression of stencil19 term for term (the same operands in the same
 // order), so r and q are bit-identical to the cell-lockstep kernels'.  The per-block results
 // (max |r|, or the sum of p*q over the thread's points in plane order and then the halving tree)
@@ -31,6 +33,7 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
   constexpr int NT = TI * TJ, RI = TI + 2, RJ = TJ + 2, PL = RI * RJ, NH = PL - NT;
   static_assert(NH <= NT, "one halo point per thread at most");
   __shared__ T ring[4][PL];
+  __shared__ T gs[2][5 * PL];  // G1..G5 of the current plane, double-buffered
   __shared__ double red[NT];
   const i64 n3 = (i64)n * n * n;
   if (batched) {
@@ -56,8 +59,12 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
     z &= m;
     return (i64)(4 * (z & 1)) * h3 + hh * (z >> 1);
   };
-  const i64 oc = IJ(i, j), oip = IJ((i + 1) & m, j), oim = IJ((i - 1) & m, j), ojp = IJ(i, (j + 1) & m),
-            ojm = IJ(i, (j - 1) & m);
+  // Loads and stores address a plane-uniform base (array + K(k), scalar registers) plus this
+  // thread's 32-bit byte offset of the i/j part: IJ < 4 (n/2)^3, so IJ * sizeof(T) < 2^32 for
+  // n <= 1024.
+  auto BO = [&](i64 e) -> uint32_t { return (uint32_t)(e * (i64)sizeof(T)); };
+  const uint32_t oc = BO(IJ(i, j)), oip = BO(IJ((i + 1) & m, j)), oim = BO(IJ((i - 1) & m, j)),
+                 ojp = BO(IJ(i, (j + 1) & m)), ojm = BO(IJ(i, (j - 1) & m));
   // this thread's halo point (tid < NH): top row, bottom row, then the left/right columns
   int hx = 0, hy = 0;
   if (tid < NH) {
@@ -73,7 +80,8 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
       hx = (r & 1) ? TI : -1;
     }
   }
-  const i64 oh = IJ((i0 + hx) & m, (j0 + hy) & m);
+  const uint32_t oh = BO(IJ((i0 + hx) & m, (j0 + hy) & m));
+  auto LD = [](const T* base, uint32_t off) -> T { return *(const T*)((const char*)base + off); };
   const int own = (ty + 1) * RI + tx + 1, hslot = (hy + 1) * RI + hx + 1;
   const T* G1 = G;
   const T* G2 = G + n3;
@@ -82,43 +90,96 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
   const T* G5 = G + 4 * n3;
   const T* G6 = G + 5 * n3;
 
-  // prologue: phi of planes k0-1, k0, k0+1; G3, G5, G6 of planes k0-1 and k0 at the own point
+  // halo points in the left/right columns supply G1, G2, G3 (i-neighbours); in the top/bottom rows
+  // G2, G4, G5 (j-neighbours); corners no metric value
+  const bool hcol = tid < NH && hy >= 0 && hy < TJ;
+  const bool hrow = tid < NH && !hcol && hx >= 0 && hx < TI;
+
+  // Software pipeline, one plane ahead: step k writes the G1..G5 plane k and the phi plane k+1
+  // that step k-1 loaded into registers, then loads G of plane k+1, phi of plane k+2 and G3/G5/G6
+  // of plane k+2 at the own point, so a load has a whole step to arrive.
+  T nphi = T(0), nphih = T(0);                      // phi of the plane to be stored next
+  T n1 = T(0), n2 = T(0), n4 = T(0), nh0 = T(0), nh1 = T(0), nh2 = T(0);  // G of the next plane
 #pragma unroll 1
-  for (int z = k0 - 1; z <= k0 + 1; ++z) {
-    const i64 kz = K(z);
+  for (int z = k0 - 1; z <= k0; ++z) {
+    const T* pz = phi + K(z);
     T* s = ring[z & 3];
-    s[own] = phi[oc + kz];
-    if (tid < NH) s[hslot] = phi[oh + kz];
+    s[own] = LD(pz, oc);
+    if (tid < NH) s[hslot] = LD(pz, oh);
   }
-  const i64 km0 = K(k0 - 1), kc0 = K(k0);
-  T g3m = G3[oc + km0], g5m = G5[oc + km0], g6m = G6[oc + km0];
-  T g3c = G3[oc + kc0], g5c = G5[oc + kc0], g6c = G6[oc + kc0];
-  __syncthreads();
+  auto load_next = [&](int kn, int kphi) {  // G1/G2/G4 and halo G of plane kn, phi of plane kphi
+    const i64 q = K(kn);
+    n1 = LD(G1 + q, oc);
+    n2 = LD(G2 + q, oc);
+    n4 = LD(G4 + q, oc);
+    if (hcol) {
+      nh0 = LD(G1 + q, oh);
+      nh1 = LD(G2 + q, oh);
+      nh2 = LD(G3 + q, oh);
+    } else if (hrow) {
+      nh0 = LD(G2 + q, oh);
+      nh1 = LD(G4 + q, oh);
+      nh2 = LD(G5 + q, oh);
+    }
+    const T* pz = phi + K(kphi);
+    nphi = LD(pz, oc);
+    if (tid < NH) nphih = LD(pz, oh);
+  };
+  load_next(k0, k0 + 1);
+  const i64 km0 = K(k0 - 1), kc0 = K(k0), kp0 = K(k0 + 1);
+  T g3m = LD(G3 + km0, oc), g5m = LD(G5 + km0, oc), g6m = LD(G6 + km0, oc);
+  T g3c = LD(G3 + kc0, oc), g5c = LD(G5 + kc0, oc), g6c = LD(G6 + kc0, oc);
+  T g3p = LD(G3 + kp0, oc), g5p = LD(G5 + kp0, oc), g6p = LD(G6 + kp0, oc);
 
   double acc = 0.0;
 #pragma unroll 1
   for (int t = 0; t < KS; ++t) {
     const int k = k0 + t;
-    const i64 kc = K(k), kp = K(k + 1);
-    // the phi plane two ahead, stored after this step's reads of the ring
-    T nxo = T(0), nxh = T(0);
-    const bool more = t + 1 < KS;
-    if (more) {
-      const i64 k2 = K(k + 2);
-      nxo = phi[oc + k2];
-      if (tid < NH) nxh = phi[oh + k2];
+    const i64 kc = K(k);
+    // plane k of G1..G5 into buffer k & 1 (last read in step k - 2); phi plane k + 1 into ring slot
+    // (k + 1) & 3 (last read in step k - 2)
+    T* gb = &gs[k & 1][0];
+    gb[0 * PL + own] = n1;
+    gb[1 * PL + own] = n2;
+    gb[2 * PL + own] = g3c;
+    gb[3 * PL + own] = n4;
+    gb[4 * PL + own] = g5c;
+    if (hcol) {
+      gb[0 * PL + hslot] = nh0;
+      gb[1 * PL + hslot] = nh1;
+      gb[2 * PL + hslot] = nh2;
+    } else if (hrow) {
+      gb[1 * PL + hslot] = nh0;
+      gb[3 * PL + hslot] = nh1;
+      gb[4 * PL + hslot] = nh2;
     }
-    const T g3p = G3[oc + kp], g5p = G5[oc + kp], g6p = G6[oc + kp];
-    const T g1c = G1[oc + kc], g4c = G4[oc + kc];
-    const T g1ip = G1[oip + kc], g1im = G1[oim + kc];
-    const T g4jp = G4[ojp + kc], g4jm = G4[ojm + kc];
-    const T g2jp = G2[ojp + kc], g2jm = G2[ojm + kc], g2ip = G2[oip + kc], g2im = G2[oim + kc];
-    const T g5jp = G5[ojp + kc], g5jm = G5[ojm + kc];
-    const T g3ip = G3[oip + kc], g3im = G3[oim + kc];
+    {
+      T* s = ring[(k + 1) & 3];
+      s[own] = nphi;
+      if (tid < NH) s[hslot] = nphih;
+    }
+    const T g1c = n1, g4c = n4;
+    // loads for the next step
+    T g3n = T(0), g5n = T(0), g6n = T(0);
+    if (t + 1 < KS) {
+      load_next(k + 1, k + 2);
+      const i64 k2 = K(k + 2);
+      g3n = LD(G3 + k2, oc);
+      g5n = LD(G5 + k2, oc);
+      g6n = LD(G6 + k2, oc);
+    }
+    __syncthreads();
     const T* sm = ring[(k - 1) & 3];
     const T* sc = ring[k & 3];
     const T* sp = ring[(k + 1) & 3];
 #define PH_(s, dx, dy) s[own + (dy) * RI + (dx)]
+#define GG_(comp, dx, dy) gb[(comp) * PL + own + (dy) * RI + (dx)]
+    const T g1ip = GG_(0, 1, 0), g1im = GG_(0, -1, 0);
+    const T g4jp = GG_(3, 0, 1), g4jm = GG_(3, 0, -1);
+    const T g2jp = GG_(1, 0, 1), g2jm = GG_(1, 0, -1), g2ip = GG_(1, 1, 0), g2im = GG_(1, -1, 0);
+    const T g5jp = GG_(4, 0, 1), g5jm = GG_(4, 0, -1);
+    const T g3ip = GG_(2, 1, 0), g3im = GG_(2, -1, 0);
+#undef GG_
     // stencil19 (mmh/gauss1.f90:954-978), operands and order unchanged
     T b11p = (g1ip + g1c) / T(2);
     T b11m = (g1im + g1c) / T(2);
@@ -144,25 +205,19 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
     const T pc = PH_(sc, 0, 0);
 #undef PH_
     if (MODE == 0) {
-      const T rv = rhs[oc + kc] - (t1 + t2 + tdiag * pc) * di2;
-      if (out) out[oc + kc] = rv;
+      const T rv = LD(rhs + kc, oc) - (t1 + t2 + tdiag * pc) * di2;
+      if (out) *(T*)((char*)(out + kc) + oc) = rv;
       acc = dmax(acc, (double)fabs(rv));
     } else {
       const T v = (t1 + t2 + tdiag * pc) * di2;
-      out[oc + kc] = v;
+      *(T*)((char*)(out + kc) + oc) = v;
       acc = acc + (double)pc * (double)v;
     }
-    // slot (k + 2) & 3 was last read in step k - 1, which every thread has left
-    if (more) {
-      T* s = ring[(k + 2) & 3];
-      s[own] = nxo;
-      if (tid < NH) s[hslot] = nxh;
-    }
-    g3m = g3c; g3c = g3p;
-    g5m = g5c; g5c = g5p;
-    g6m = g6c; g6c = g6p;
-    __syncthreads();
+    g3m = g3c; g3c = g3p; g3p = g3n;
+    g5m = g5c; g5c = g5p; g5p = g5n;
+    g6m = g6c; g6c = g6p; g6p = g6n;
   }
+  __syncthreads();
   red[tid] = acc;
   __syncthreads();
   for (int s = NT / 2; s > 0; s >>= 1) {

@@ -44,6 +44,7 @@ struct mmh_ctx {
   int wave_min_n = 128;
   int wave_p = 1;          // points per thread in flight per colour phase of the wavefront kernel
   int march_op = 1;        // residual/operator of large levels by k-marching workgroups (option "march_op")
+  int march_tile = 0;      // tile shape of the k-marching kernels (option "march_tile": 0, 1, 2)
   int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
@@ -266,20 +267,49 @@ void build_hier(mmh_ctx* c, Hier<T>& H, const std::string& tag, T* u, T* rhs, co
 // one per 2x2x2 cell in layout 1.
 inline unsigned stencil_grid(int n, int layout) { return layout == 1 ? nblk(cube(n / 2), RB) : nblk(cube(n), RB); }
 
-// Tiles of the k-marching kernels: 256 threads, rows of 32 FP64 (256 B) or 64 FP32 values.
-template <class T> struct MarchTile;
-template <> struct MarchTile<double> { static constexpr int TI = 32, TJ = 8; };
-template <> struct MarchTile<float> { static constexpr int TI = 64, TJ = 4; };
+// Tiles (TI x TJ points) of the k-marching kernels, by option "march_tile".  0 (default): the
+// fastest measured at 512^3 on MI355X, 64 x 8 (512 threads) in FP64 and 64 x 4 (256 threads) in
+// FP32; 1 and 2 are the alternatives that were timed against them.
+template <class T, int V> struct MarchTile;
+template <> struct MarchTile<double, 0> { static constexpr int TI = 64, TJ = 8; };
+template <> struct MarchTile<double, 1> { static constexpr int TI = 32, TJ = 8; };
+template <> struct MarchTile<double, 2> { static constexpr int TI = 32, TJ = 16; };
+template <> struct MarchTile<float, 0> { static constexpr int TI = 64, TJ = 4; };
+template <> struct MarchTile<float, 1> { static constexpr int TI = 64, TJ = 8; };
+template <> struct MarchTile<float, 2> { static constexpr int TI = 128, TJ = 4; };
 
 // Planes per workgroup of k_op_march for a level of size n: the largest of 32, 16, 8 leaving at
 // least 2048 workgroups (8 at least, so the block results never outnumber stencil_grid's); 0 where
 // the marching kernel does not apply (small levels, the option off).
-template <class T> int march_ks(const mmh_ctx* c, int n) {
-  constexpr int TI = MarchTile<T>::TI, TJ = MarchTile<T>::TJ;
-  if (!c->march_op || n < 64 || n % TI || n % TJ || n % 8) return 0;
+template <class T, int V> int march_ks(const mmh_ctx* c, int n) {
+  constexpr int TI = MarchTile<T, V>::TI, TJ = MarchTile<T, V>::TJ;
+  if (!c->march_op || n < 64 || n > 1024 || n % TI || n % TJ || n % 8) return 0;  // 32-bit offsets up to 1024^3
   for (int ks = 32; ks > 8; ks >>= 1)
     if (n % ks == 0 && cube(n) / ((i64)TI * TJ * ks) >= 2048) return ks;
   return 8;
+}
+
+// One k_op_march launch (MODE 0 residual, 1 operator) if the level qualifies; returns the number
+// of block results, 0 when it does not apply.
+template <class T, int MODE, int V>
+unsigned march_launch(mmh_ctx* c, T* out, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2, int B,
+                      int batched) {
+  constexpr int TI = MarchTile<T, V>::TI, TJ = MarchTile<T, V>::TJ;
+  const int ks = march_ks<T, V>(c, n);
+  if (!ks) return 0;
+  const unsigned nb = (unsigned)(cube(n) / ((i64)TI * TJ * ks));
+  k_op_march<T, MODE, TI, TJ><<<dim3(nb, B), TI * TJ, 0, c->stream>>>(out, bres, phi, rhs, G, n, di2, ks, batched);
+  LAUNCHCHK();
+  return nb;
+}
+template <class T, int MODE>
+unsigned march_op(mmh_ctx* c, T* out, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2, int B,
+                  int batched) {
+  switch (c->march_tile) {
+    case 1: return march_launch<T, MODE, 1>(c, out, bres, phi, rhs, G, n, di2, B, batched);
+    case 2: return march_launch<T, MODE, 2>(c, out, bres, phi, rhs, G, n, di2, B, batched);
+    default: return march_launch<T, MODE, 0>(c, out, bres, phi, rhs, G, n, di2, B, batched);
+  }
 }
 
 // r = rhs - L phi on a sub-lattice level (r may be null) with per-block max|r| into bres, for B
@@ -287,17 +317,11 @@ template <class T> int march_ks(const mmh_ctx* c, int n) {
 template <class T>
 unsigned op_resid(mmh_ctx* c, T* r, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2,
                   bool delta = false, int B = 1, int batched = 0) {
-  constexpr int TI = MarchTile<T>::TI, TJ = MarchTile<T>::TJ;
-  const int ks = delta ? 0 : march_ks<T>(c, n);
-  unsigned nb;
-  if (ks) {
-    nb = (unsigned)(cube(n) / ((i64)TI * TJ * ks));
-    k_op_march<T, 0, TI, TJ><<<dim3(nb, B), TI * TJ, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, ks, batched);
-  } else {
-    nb = stencil_grid(n, 1);
-    if (delta) k_resid<T, 1, true><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched);
-    else k_resid<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched);
-  }
+  if (!delta)
+    if (const unsigned nb = march_op<T, 0>(c, r, bres, phi, rhs, G, n, di2, B, batched)) return nb;
+  const unsigned nb = stencil_grid(n, 1);
+  if (delta) k_resid<T, 1, true><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched);
+  else k_resid<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched);
   LAUNCHCHK();
   return nb;
 }
@@ -305,16 +329,9 @@ unsigned op_resid(mmh_ctx* c, T* r, double* bres, const T* phi, const T* rhs, co
 // q = L p with per-block sums of p*q into bres (same conventions as op_resid).
 template <class T>
 unsigned op_apply(mmh_ctx* c, T* q, double* bres, const T* p, const T* G, int n, T di2, int B = 1, int batched = 0) {
-  constexpr int TI = MarchTile<T>::TI, TJ = MarchTile<T>::TJ;
-  const int ks = march_ks<T>(c, n);
-  unsigned nb;
-  if (ks) {
-    nb = (unsigned)(cube(n) / ((i64)TI * TJ * ks));
-    k_op_march<T, 1, TI, TJ><<<dim3(nb, B), TI * TJ, 0, c->stream>>>(q, bres, p, nullptr, G, n, di2, ks, batched);
-  } else {
-    nb = stencil_grid(n, 1);
-    k_apply_dot<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(q, bres, p, G, n, di2, batched);
-  }
+  if (const unsigned nb = march_op<T, 1>(c, q, bres, p, (const T*)nullptr, G, n, di2, B, batched)) return nb;
+  const unsigned nb = stencil_grid(n, 1);
+  k_apply_dot<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(q, bres, p, G, n, di2, batched);
   LAUNCHCHK();
   return nb;
 }
@@ -1482,6 +1499,9 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->sweep_impl = value;
     } else if (k == "march_op") {
       c->march_op = value ? 1 : 0;
+    } else if (k == "march_tile") {
+      require(value >= 0 && value <= 2, "set_option: march_tile is 0, 1 or 2");
+      c->march_tile = value;
     } else if (k == "pcg_graph") {
       c->pcg_graph = value ? 1 : 0;
     } else if (k == "mixed_delta") {
