@@ -107,6 +107,24 @@ def smoother_only(ctx, torch, defn, u, rhs0, n, sweeps):
     print("smoother-only: %d^3 FP64 %.4f ms/sweep (%d sweeps); copy 1 GiB x3 done" % (n, ms, sweeps), flush=True)
 
 
+def krylov_probe(ctx, torch, defn, u, rhs0, n, iters=2):
+    """A few PCG iterations in FP64 and in mixed precision (no tolerance), for counter runs over the
+    residual/operator kernels (k_op_march), then the same 1 GiB calibration copy."""
+    n3 = n ** 3
+    for mixed in (0, 1):
+        phi = torch.zeros(n3, dtype=torch.float64, device="cuda")
+        rhs = rhs0.clone()
+        st, _ = ctx.solve(phi, rhs, defn, u, tol=1e-30, max_cycles=iters, schedule=1, nu_pre=2, nu_post=2,
+                          nu_coarse=8, mixed=mixed, nu=1, krylov=True)
+        del phi, rhs
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("krylov-probe: %d^3, %d PCG iterations per precision; copy 1 GiB x3 done" % (n, iters), flush=True)
+
+
 def batched_leg(ctx, torch):
     """Batched uneven realisations: 40 x 32^3, 40 x 64^3 and 20 x 128^3 lognormal fields, sigma
     uniform in [0.5, 2.0] (seeded), seeds 0..99, each solved from phi = 0 to 1e-8 (FP64 PCG).
@@ -177,6 +195,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweeps", type=int, default=20, help="sweeps timed for the roofline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-realisations leg")
+    ap.add_argument("--krylov-probe", action="store_true",
+                    help="only two PCG iterations per precision plus the 1 GiB copy (counter runs)")
     ap.add_argument("--smoother-only", action="store_true",
                     help="only the smoother timing plus a 1 GiB device copy (the counter-calibration run)")
     args = ap.parse_args()
@@ -210,6 +230,9 @@ def main():
     print("bench: setup done", file=sys.stderr, flush=True)
     if args.smoother_only:
         smoother_only(ctx, torch, defn, u, rhs0, n, args.sweeps)
+        return
+    if args.krylov_probe:
+        krylov_probe(ctx, torch, defn, u, rhs0, n)
         return
 
     def step(mixed):
