@@ -223,6 +223,21 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __
 #undef X_
 }
 
+// Gauss-Seidel update of point q (0 <= q < (n/2)^3) of the colour class (kpar, io, jo), in place
+// (relaxplane, mmh/gauss1.f90:954-980).
+template <class T, int L, bool DELTA = false>
+__device__ __forceinline__ void colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G,
+                                             int n, T dx2, int kpar, int io, int jo, i64 q) {
+  const int h = n >> 1;
+  const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
+  const i64 n3 = (i64)n * n * n;
+  T t1, t2, tdiag;
+  stencil19<T, L, DELTA>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+  const i64 c = lx<L>(i, j, k, n);
+  if (DELTA) phi[c] = phi[c] + (rhs[c] * dx2 - t1 - t2) / tdiag;
+  else phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
+}
+
 // One colour class of the 8-colour Gauss-Seidel sweep: points with (k, i, j) parity (kpar, io, jo)
 // (relaxplane, mmh/gauss1.f90:910-980).  One thread per point of the colour; in layout 1 the
 // threads of a workgroup own consecutive points of the class and every stencil access is
@@ -239,13 +254,18 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
   const unsigned blk = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const i64 q = (i64)blk * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
-  const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
+  colour_point<T, L, DELTA>(phi, rhs, G, n, dx2, kpar, io, jo, q);
+}
+
+// Residual rhs - L[phi] at point (i, j, k) (calcresid, mmh/gauss1.f90:683-709).
+template <class T, int L, bool DELTA = false>
+__device__ __forceinline__ T resid_point(const T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G,
+                                        int n, T di2, int i, int j, int k) {
   const i64 n3 = (i64)n * n * n;
+  const i64 p = lx<L>(i, j, k, n);
   T t1, t2, tdiag;
   stencil19<T, L, DELTA>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
-  const i64 c = lx<L>(i, j, k, n);
-  if (DELTA) phi[c] = phi[c] + (rhs[c] * dx2 - t1 - t2) / tdiag;
-  else phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
+  return DELTA ? rhs[p] - (t1 + t2) * di2 : rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
 }
 
 // Residual r = rhs - L[phi] (calcresid, mmh/gauss1.f90:683-709); optional store; per-block max|r|.
@@ -277,9 +297,7 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
     for (int c = 0; c < 8; ++c) {
       const int i = 2 * ci + (c & 1), j = 2 * cj + ((c >> 1) & 1), k = 2 * ck + (c >> 2);
       const i64 p = lx<L>(i, j, k, n);
-      T t1, t2, tdiag;
-      stencil19<T, L, DELTA>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
-      T rv = DELTA ? rhs[p] - (t1 + t2) * di2 : rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
+      const T rv = resid_point<T, L, DELTA>(phi, rhs, G, n, di2, i, j, k);
       if (r) r[p] = rv;
       a = dmax(a, (double)fabs(rv));
     }
@@ -287,9 +305,7 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
     int i, j, k;
     point_of<L>(pp, n, i, j, k);
     const i64 p = lx<L>(i, j, k, n);
-    T t1, t2, tdiag;
-    stencil19<T, L>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
-    T rv = rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
+    const T rv = resid_point<T, L>(phi, rhs, G, n, di2, i, j, k);
     if (r) r[p] = rv;
     a = (double)fabs(rv);
   }
@@ -533,16 +549,10 @@ __global__ void k_cg_scalars(int step, int B, int it, double tol, double* __rest
 // (mmh/gauss1.f90:819-859): coarse plane ck gets a main part from fine planes 2ck, 2ck+1 and a
 // share from fine plane 2ck-1; plane ck0 accumulates main then share, every other plane share
 // then main (the order in which r1lax's plane pipeline reaches them).  r and rc in layout L.
+// restrict_point computes coarse point q (column-major order of the coarse grid).
 template <class T, int L>
-__global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0,
-                                                        int batched = 0) {
+__device__ __forceinline__ void restrict_point(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0, i64 q) {
   const int h = n >> 1;
-  if (batched) {
-    rc += (i64)blockIdx.y * h * h * h;
-    r += (i64)blockIdx.y * n * n * n;
-  }
-  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (i64)h * h * h) return;
   // coarse points in column-major order: in layout 1 the 27 fine reads of consecutive threads are
   // then consecutive within each fine class array
   int ci, cj, ck;
@@ -571,6 +581,19 @@ __global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, cons
   }
 #undef R_
   rc[lx<L>(ci, cj, ck, h)] = acc;
+}
+
+template <class T, int L>
+__global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0,
+                                                        int batched = 0) {
+  const int h = n >> 1;
+  if (batched) {
+    rc += (i64)blockIdx.y * h * h * h;
+    r += (i64)blockIdx.y * n * n * n;
+  }
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (i64)h * h * h) return;
+  restrict_point<T, L>(rc, r, n, ck0, q);
 }
 
 // 27-point weights 1/8, 1/16, 1/32, 1/64 around fine point (i2,j2,k2): restrict/restrictu
@@ -607,6 +630,30 @@ __global__ __launch_bounds__(256) void k_smooth27(T* __restrict__ out, const T* 
 
 // Trilinear prolongation (mmh/mgutil.f90:38-71 inject adds; 2-34 prolong overwrites); u (n^3) and
 // uh ((n/2)^3) in layout L.
+// Trilinear prolongation of fine 2x2x2 cell pp (column-major cell order) in sub-lattice layout:
+// its 8 points interpolate from the coarse block {ci, ci+1} x {cj, cj+1} x {ck, ck+1}; each
+// point's 8-term sum keeps the order below.  add: u += s, else u = s.
+template <class T>
+__device__ __forceinline__ void prolong_cell(T* __restrict__ u, const T* __restrict__ uh, int n, int add, i64 pp) {
+  const int h = n >> 1;
+  int ci, cj, ck;
+  point_of<0>(pp, h, ci, cj, ck);
+  const int c1 = (ci + 1) % h, d1 = (cj + 1) % h, e1 = (ck + 1) % h;
+  T v[2][2][2];
+  v[0][0][0] = uh[lx<1>(ci, cj, ck, h)]; v[0][0][1] = uh[lx<1>(ci, cj, e1, h)];
+  v[0][1][0] = uh[lx<1>(ci, d1, ck, h)]; v[0][1][1] = uh[lx<1>(ci, d1, e1, h)];
+  v[1][0][0] = uh[lx<1>(c1, cj, ck, h)]; v[1][0][1] = uh[lx<1>(c1, cj, e1, h)];
+  v[1][1][0] = uh[lx<1>(c1, d1, ck, h)]; v[1][1][1] = uh[lx<1>(c1, d1, e1, h)];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int a = c & 1, b = (c >> 1) & 1, e = c >> 2;  // i = 2ci+a: low index ci, high index ci+a
+    const T s = (v[0][0][0] + v[0][0][e] + v[0][b][0] + v[0][b][e]
+                 + v[a][b][0] + v[a][b][e] + v[a][0][0] + v[a][0][e]) / T(8);
+    const i64 p = lx<1>(2 * ci + a, 2 * cj + b, 2 * ck + e, n);
+    u[p] = add ? u[p] + s : s;
+  }
+}
+
 template <class T, int L>
 __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __restrict__ uh, int n, int add,
                                                  int batched = 0) {
@@ -617,25 +664,8 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __r
     uh += (i64)blockIdx.y * h * h * h;
   }
   if (L == 1) {
-    // one thread per fine 2x2x2 cell (ci, cj, ck): its 8 points interpolate from the coarse block
-    // {ci, ci+1} x {cj, cj+1} x {ck, ck+1}; each point's 8-term sum keeps the order below
     if (pp >= (i64)h * h * h) return;
-    int ci, cj, ck;
-    point_of<0>(pp, h, ci, cj, ck);
-    const int c1 = (ci + 1) % h, d1 = (cj + 1) % h, e1 = (ck + 1) % h;
-    T v[2][2][2];
-    v[0][0][0] = uh[lx<1>(ci, cj, ck, h)]; v[0][0][1] = uh[lx<1>(ci, cj, e1, h)];
-    v[0][1][0] = uh[lx<1>(ci, d1, ck, h)]; v[0][1][1] = uh[lx<1>(ci, d1, e1, h)];
-    v[1][0][0] = uh[lx<1>(c1, cj, ck, h)]; v[1][0][1] = uh[lx<1>(c1, cj, e1, h)];
-    v[1][1][0] = uh[lx<1>(c1, d1, ck, h)]; v[1][1][1] = uh[lx<1>(c1, d1, e1, h)];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int a = c & 1, b = (c >> 1) & 1, e = c >> 2;  // i = 2ci+a: low index ci, high index ci+a
-      const T s = (v[0][0][0] + v[0][0][e] + v[0][b][0] + v[0][b][e]
-                   + v[a][b][0] + v[a][b][e] + v[a][0][0] + v[a][0][e]) / T(8);
-      const i64 p = lx<1>(2 * ci + a, 2 * cj + b, 2 * ck + e, n);
-      u[p] = add ? u[p] + s : s;
-    }
+    prolong_cell<T>(u, uh, n, add, pp);
     return;
   }
   if (pp >= (i64)n * n * n) return;

@@ -1,0 +1,184 @@
+// kernels_coarse.hpp -- the coarse end of the symmetric preconditioning V-cycle in one workgroup.
+//
+// Below 64^3 a V-cycle level is a few thousand points or fewer, and each of its ~40 launches
+// (8 colours per sweep, residual, restriction, fill, prolongation, sums, mean removal) takes a few
+// microseconds plus the dispatch gap: the 4^3..32^3 levels cost about as much as the 256^3 level.
+// Here one workgroup of 1024 threads runs the whole cycle from the top coarse level down to 4^3
+// and back, with a barrier between dependent phases.  The level arrays stay in global memory
+// (they fit in the L2).  Every phase applies the per-point functions of the multi-launch kernels
+// (colour_point, resid_point, restrict_point, prolong_cell) and the sums follow reduce()'s
+// blocked tree order exactly, so the result is bit-identical to the launch-by-launch cycle.
+// Used for the preconditioner of the Krylov solve only (no divergence guard: the mean removal is
+// always mode 0).
+#pragma once
+#include "kernels.hpp"
+
+namespace mmh {
+
+constexpr int CW = 1024;       // threads of the coarse-cycle workgroup
+constexpr int CLEV_MAX = 8;    // levels one coarse cycle may hold
+constexpr int CTOP_MAX = 32;   // largest level run this way (its sums have <= 16 blocks)
+
+__device__ __forceinline__ i64 cube_d(int n) { return (i64)n * n * n; }
+
+// Levels 0..nl-1 of the coarse cycle (level 0 the finest of them, level nl-1 the 4^3 level), in
+// sub-lattice layout.  With mixed, T = float down to 8^3 and the 4^3 level runs in FP64 on the d*
+// arrays (its T arrays receive the converted result), as in the multi-launch mixed cycle.
+template <class T> struct CoarseCycle {
+  T* u[CLEV_MAX];
+  T* rhs[CLEV_MAX];
+  T* r[CLEV_MAX];
+  const T* G[CLEV_MAX];
+  double* scal[CLEV_MAX];  // per level: [0..2] as the multi-launch cycle's scalars (sum in [2])
+  int* guard[CLEV_MAX];
+  T dx[CLEV_MAX];
+  int n[CLEV_MAX];
+  int nl;
+  int pre, post, coarse;  // sweeps per level (pre == post) and each way on 4^3
+  int symmetric;          // post-smoothing in reverse colour order
+  int mixed;
+  double* du;             // FP64 4^3 level (mixed)
+  double* drhs;
+  const double* dG;
+  double* dscal;
+  int* dguard;
+  double ddx;
+};
+
+// One stage of reduce() (k_reduce_stage, op 0) over x[0..n) by the whole workgroup: block b's
+// value, over elements [b*RB*RPT, (b+1)*RB*RPT), is formed by one group of RB threads exactly as
+// k_reduce_stage forms it, four blocks at a time.  Block values go to out[b] (shared memory).
+template <class S> __device__ void cw_stage(double* out, const S* x, i64 n, double (*s)[RB]) {
+  const int t = threadIdx.x % RB, g = threadIdx.x / RB;
+  constexpr int NG = CW / RB;
+  const i64 ch = (i64)RB * RPT;
+  const int nb = (int)((n + ch - 1) / ch);
+  for (int b0 = 0; b0 < nb; b0 += NG) {
+    const int b = b0 + g;
+    double a = 0.0;
+    if (b < nb) {
+      const i64 base = (i64)b * ch;
+#pragma unroll
+      for (int m = 0; m < RPT; ++m) {
+        const i64 q = base + (i64)m * RB + t;
+        if (q < n) {
+          const double v = (double)x[q];
+          a = a + v;
+        }
+      }
+    }
+    s[g][t] = a;
+    __syncthreads();
+    for (int h = RB / 2; h > 0; h >>= 1) {
+      if (t < h) s[g][t] = s[g][t] + s[g][t + h];
+      __syncthreads();
+    }
+    if (t == 0 && b < nb) out[b] = s[g][0];
+    __syncthreads();
+  }
+}
+
+// reduce(x, n, op 0): the stages until one value is left; returns it (every thread).
+template <class S> __device__ double cw_sum(const S* x, i64 n, double* blk, double* blk2, double (*s)[RB]) {
+  const i64 ch = (i64)RB * RPT;
+  i64 nb = (n + ch - 1) / ch;
+  cw_stage<S>(blk, x, n, s);
+  double* a = blk;
+  double* b = blk2;
+  while (nb > 1) {
+    const i64 nb2 = (nb + ch - 1) / ch;
+    cw_stage<double>(b, a, nb, s);
+    double* tmp = a;
+    a = b;
+    b = tmp;
+    nb = nb2;
+  }
+  return a[0];
+}
+
+// ns Gauss-Seidel sweeps of a level (colour order reversed when rev), one barrier per colour.
+template <class T> __device__ void cw_sweeps(T* u, const T* rhs, const T* G, int n, T dx, int ns, bool rev) {
+  const T dx2 = dx * dx;
+  const i64 h3 = cube_d(n / 2);
+  for (int sw = 0; sw < ns; ++sw)
+    for (int cc = 0; cc < 8; ++cc) {
+      const int col = rev ? 7 - cc : cc;
+      const int kp = col >> 2, io = (col >> 1) & 1, jo = col & 1;
+      for (i64 q = threadIdx.x; q < h3; q += CW) colour_point<T, 1>(u, rhs, G, n, dx2, kp, io, jo, q);
+      __syncthreads();
+    }
+}
+
+// u -= sum(u)/n^3 (k_zerosum_guard mode 0) with the sum in reduce()'s order; the sum is also
+// stored where the multi-launch cycle keeps it.
+template <class T>
+__device__ void cw_zerosum(T* u, int n, double* scal, int* guard, double* blk, double* blk2, double (*s)[RB]) {
+  const i64 n3 = cube_d(n);
+  const double sum = cw_sum<T>(u, n3, blk, blk2, s);
+  if (threadIdx.x == 0) {
+    scal[2] = sum;
+    if (guard) *guard = 0;
+  }
+  const double m = sum / (double)n3;
+  for (i64 p = threadIdx.x; p < n3; p += CW) u[p] = (T)((double)u[p] - m);
+  __syncthreads();
+}
+
+// The 4^3 level: coarse sweeps forward, then forward or reversed, then the mean removal.
+template <class T>
+__device__ void cw_bottom(T* u, const T* rhs, const T* G, T dx, int nc, bool rev, double* scal, int* guard,
+                          double* blk, double* blk2, double (*s)[RB]) {
+  cw_sweeps<T>(u, rhs, G, 4, dx, nc, false);
+  cw_sweeps<T>(u, rhs, G, 4, dx, nc, rev);
+  cw_zerosum<T>(u, 4, scal, guard, blk, blk2, s);
+}
+
+template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCycle<T> C) {
+  __shared__ double s[CW / RB][RB];
+  __shared__ double blk[64], blk2[64];  // block values of a level's sums (32^3: 16 blocks)
+  // down: pre-smoothing, residual, restriction into the next level's rhs, zero next level's u
+  for (int l = 0; l + 1 < C.nl; ++l) {
+    const int n = C.n[l];
+    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], n, C.dx[l], C.pre, false);
+    const T di2 = T(1) / (C.dx[l] * C.dx[l]);
+    const i64 n3 = cube_d(n), h3 = cube_d(n / 2);
+    for (i64 p = threadIdx.x; p < n3; p += CW) {
+      int i, j, k;
+      point_of<1>(p, n, i, j, k);
+      C.r[l][p] = resid_point<T, 1>(C.u[l], C.rhs[l], C.G[l], n, di2, i, j, k);
+    }
+    __syncthreads();
+    const int nrmax = min(4, max(1, n / 4));
+    const int nrl = C.pre > nrmax ? (C.pre % nrmax ? C.pre % nrmax : nrmax) : C.pre;
+    for (i64 q = threadIdx.x; q < h3; q += CW) {
+      restrict_point<T, 1>(C.rhs[l + 1], C.r[l], n, nrl % (n / 2), q);
+      C.u[l + 1][q] = T(0);
+    }
+    __syncthreads();
+  }
+  // the 4^3 level
+  const int lb = C.nl - 1;
+  if (C.mixed) {
+    for (int p = threadIdx.x; p < 64; p += CW) {
+      C.drhs[p] = (double)C.rhs[lb][p];
+      C.du[p] = 0.0;
+    }
+    __syncthreads();
+    cw_bottom<double>(C.du, C.drhs, C.dG, C.ddx, C.coarse, C.symmetric, C.dscal, C.dguard, blk, blk2, s);
+    for (int p = threadIdx.x; p < 64; p += CW) C.u[lb][p] = (T)C.du[p];
+    __syncthreads();
+  } else {
+    cw_bottom<T>(C.u[lb], C.rhs[lb], C.G[lb], C.dx[lb], C.coarse, C.symmetric, C.scal[lb], C.guard[lb], blk, blk2, s);
+  }
+  // up: prolongation, post-smoothing, mean removal
+  for (int l = C.nl - 2; l >= 0; --l) {
+    const int n = C.n[l];
+    const i64 h3 = cube_d(n / 2);
+    for (i64 pp = threadIdx.x; pp < h3; pp += CW) prolong_cell<T>(C.u[l], C.u[l + 1], n, 1, pp);
+    __syncthreads();
+    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], n, C.dx[l], C.post, C.symmetric != 0);
+    cw_zerosum<T>(C.u[l], n, C.scal[l], C.guard[l], blk, blk2, s);
+  }
+}
+
+}  // namespace mmh

@@ -27,6 +27,7 @@
 #include "kernels_pd.hpp"
 #include "kernels_vis.hpp"
 #include "kernels_march.hpp"
+#include "kernels_coarse.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -45,6 +46,7 @@ struct mmh_ctx {
   int wave_p = 1;          // points per thread in flight per colour phase of the wavefront kernel
   int march_op = 1;        // residual/operator of large levels by k-marching workgroups (option "march_op")
   int march_tile = 0;      // tile shape of the k-marching kernels (option "march_tile": 0, 1, 2)
+  int coarse_fused = 16;   // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
   int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
@@ -424,11 +426,45 @@ void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n,
   }
 }
 
+// The levels l.. of a hierarchy as one coarse cycle (k_coarse_cycle), when the schedule allows it:
+// the Krylov preconditioner's fixed symmetric schedule without divergence guard, levels up to the
+// size set by option "coarse_fused" (16 by default: one workgroup is faster than ~40 launches at
+// 16^3 and below, slower at 32^3 where a level's sweeps move megabytes through one CU).
+template <class T> bool coarse_cycle_applies(const mmh_ctx* c, const Hier<T>& H, int l, const Sched& S) {
+  return c->coarse_fused && !S.guard && !S.delta && S.kind == 1 && S.nu_pre == S.nu_post &&
+         H.n[l] <= std::min(c->coarse_fused, CTOP_MAX) && H.nlev - l <= CLEV_MAX;
+}
+template <class T> CoarseCycle<T> coarse_cycle_of(Hier<T>& H, int l, const Sched& S) {
+  CoarseCycle<T> C{};
+  C.nl = H.nlev - l;
+  for (int q = 0; q < C.nl; ++q) {
+    C.u[q] = H.u[l + q];
+    C.rhs[q] = H.rhs[l + q];
+    C.r[q] = H.r[l + q];
+    C.G[q] = H.G[l + q];
+    C.scal[q] = H.scal + 3 * (l + q);
+    C.guard[q] = H.guard + l + q;
+    C.dx[q] = H.dx[l + q];
+    C.n[q] = H.n[l + q];
+  }
+  C.pre = S.nu_pre;
+  C.post = S.nu_post;
+  C.coarse = S.nu_coarse;
+  C.symmetric = S.symmetric ? 1 : 0;
+  return C;
+}
+
 // One V-cycle from level l (mgtemplate.f90 / mg*.f90; mg4 at multigrid.f90:94-121).
 template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const Sched& S) {
   const int n = H.n[l];
   const i64 n3 = cube(n);
   double* sc = H.scal + 3 * l;
+  if (coarse_cycle_applies(c, H, l, S)) {
+    CoarseCycle<T> C = coarse_cycle_of(H, l, S);
+    k_coarse_cycle<T><<<1, CW, 0, c->stream>>>(C);
+    LAUNCHCHK();
+    return;
+  }
   if (n == 4) {
     const int nc = S.coarse();
     relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 0);
@@ -907,6 +943,20 @@ template <class T> double read_scalar(mmh_ctx* c, const double* d) {
 // FP32 V-cycle whose 4^3 level runs in FP64 (the coarsest solve stays FP64).
 void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_top, const Sched& S) {
   const int n = Hf.n[l];
+  if (coarse_cycle_applies(c, Hf, l, S)) {
+    CoarseCycle<float> C = coarse_cycle_of(Hf, l, S);
+    const int lb = Hd.nlev - 1;
+    C.mixed = 1;
+    C.du = Hd.u[lb];
+    C.drhs = Hd.rhs[lb];
+    C.dG = Hd.G[lb];
+    C.dscal = Hd.scal + 3 * lb;
+    C.dguard = Hd.guard + lb;
+    C.ddx = Hd.dx[lb];
+    k_coarse_cycle<float><<<1, CW, 0, c->stream>>>(C);
+    LAUNCHCHK();
+    return;
+  }
   if (n == 4) {
     const i64 n3 = 64;
     k_convert<float, double><<<1, 64, 0, c->stream>>>(Hd.rhs[l], Hf.rhs[l], n3);
@@ -1502,6 +1552,10 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
     } else if (k == "march_tile") {
       require(value >= 0 && value <= 2, "set_option: march_tile is 0, 1 or 2");
       c->march_tile = value;
+    } else if (k == "coarse_fused") {
+      require(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
+              "set_option: coarse_fused is 0 (off) or the largest fused level size, 4 to 32");
+      c->coarse_fused = value;
     } else if (k == "pcg_graph") {
       c->pcg_graph = value ? 1 : 0;
     } else if (k == "mixed_delta") {

@@ -112,3 +112,27 @@ def test_single_solve_graph_replay_equals_eager(ctx, probs, mixed):
         assert len(rg) > 3, case
         assert rg == re, case
         assert np.array_equal(xg, xe), case
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_single_solve_coarse_cycle_equals_launches(ctx, probs, mixed):
+    """The preconditioner's coarse levels run in one workgroup (k_coarse_cycle, levels up to 32^3 by
+    option); every iterate must equal the launch-by-launch cycle's (the same per-point functions,
+    the same sum order)."""
+    print("case single-solve fused coarse cycle vs launches, mixed=%d" % mixed, flush=True)
+    for (u, r, d), case in zip(probs, CASES):
+        out = []
+        for fused in (32, 16, 4, 0):
+            ctx.set_option("coarse_fused", fused)
+            try:
+                phi = torch.zeros_like(r)
+                st, ok = ctx.solve(phi, r.clone(), d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
+                                   nu_coarse=8, mixed=mixed, nu=5, krylov=True)
+            finally:
+                ctx.set_option("coarse_fused", 16)
+            assert ok, case
+            out.append((phi.cpu().numpy(), [st.relres[i] for i in range(st.cycles)]))
+        xl, rl = out[-1]
+        for xf, rf in out[:-1]:
+            assert rf == rl, case
+            assert np.array_equal(xf, xl), case
