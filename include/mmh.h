@@ -109,6 +109,9 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *   "wave_p"      1 (default), 2 or 4 points per thread in flight per colour phase
  *   "mixed_delta" 0 (default) / 1: the FP32 preconditioner's stencil sums neighbour differences
  *                 (changes the mixed path's FP32 rounding, not its converged result)
+ *   "galerkin"    1 (default) / 0: Galerkin coarse operators (mmh_galerkin_d) on the levels of
+ *                 the Krylov solve's preconditioner; 0 rebuilds them from restricted fields as the
+ *                 reference's V-cycle does (changes the preconditioner, not the converged result)
  *   "pcg_graph"   1 (default) / 0: the Krylov solve (mmh_solve_d, krylov = 1) captures its
  *                 preconditioning V-cycle as a HIP graph on its second call and replays it after;
  *                 on the null stream the solve then runs on a stream of its own, ordered after
@@ -139,6 +142,15 @@ int mmh_solve_d(mmh_ctx*, double* u, double* rhs, const double* defpot, const do
  * point; the reference only returns its norm and restriction), with the metric from def and u(1). */
 int mmh_residual_d(mmh_ctx*, double* r, const double* phi, const double* rhs, const double* def, const double* u,
                    int nu, int n, double dx);
+
+/* Galerkin coarse operator A_level = R A_(level-1) P of the Krylov solve's preconditioner (A_0 the
+ * fine 19-point operator of def and u(1), P the V-cycle's trilinear prolongation, R = P^T / 8 its
+ * full weighting): 27 fields of (n >> level)^3 values, column-major, field (ex+1) + 3(ey+1) +
+ * 9(ez+1) holding the coefficient of offset (ex, ey, ez).  No counterpart in the reference, whose
+ * coarse levels rebuild the metric from restricted fields (mgtemplate.f90); exposed to check the
+ * operators the preconditioner uses. */
+int mmh_galerkin_d(mmh_ctx*, double* A_out, const double* defpot, const double* ubig, int nu, int n, double dx,
+                   int level);
 
 /* Per-realisation record of mmh_solve_batch_d. */
 typedef struct {

@@ -12,6 +12,7 @@
 // always mode 0).
 #pragma once
 #include "kernels.hpp"
+#include "kernels_gal.hpp"
 
 namespace mmh {
 
@@ -29,6 +30,7 @@ template <class T> struct CoarseCycle {
   T* rhs[CLEV_MAX];
   T* r[CLEV_MAX];
   const T* G[CLEV_MAX];
+  const T* A[CLEV_MAX];    // Galerkin operator of the level, or null (19-point operator from G)
   double* scal[CLEV_MAX];  // per level: [0..2] as the multi-launch cycle's scalars (sum in [2])
   int* guard[CLEV_MAX];
   T dx[CLEV_MAX];
@@ -40,6 +42,7 @@ template <class T> struct CoarseCycle {
   double* du;             // FP64 4^3 level (mixed)
   double* drhs;
   const double* dG;
+  const double* dA;
   double* dscal;
   int* dguard;
   double ddx;
@@ -97,14 +100,18 @@ template <class S> __device__ double cw_sum(const S* x, i64 n, double* blk, doub
 }
 
 // ns Gauss-Seidel sweeps of a level (colour order reversed when rev), one barrier per colour.
-template <class T> __device__ void cw_sweeps(T* u, const T* rhs, const T* G, int n, T dx, int ns, bool rev) {
+template <class T>
+__device__ void cw_sweeps(T* u, const T* rhs, const T* G, const T* A, int n, T dx, int ns, bool rev) {
   const T dx2 = dx * dx;
   const i64 h3 = cube_d(n / 2);
   for (int sw = 0; sw < ns; ++sw)
     for (int cc = 0; cc < 8; ++cc) {
       const int col = rev ? 7 - cc : cc;
       const int kp = col >> 2, io = (col >> 1) & 1, jo = col & 1;
-      for (i64 q = threadIdx.x; q < h3; q += CW) colour_point<T, 1>(u, rhs, G, n, dx2, kp, io, jo, q);
+      if (A)
+        for (i64 q = threadIdx.x; q < h3; q += CW) gal_colour_point<T>(u, rhs, A, n, kp, io, jo, q);
+      else
+        for (i64 q = threadIdx.x; q < h3; q += CW) colour_point<T, 1>(u, rhs, G, n, dx2, kp, io, jo, q);
       __syncthreads();
     }
 }
@@ -126,10 +133,10 @@ __device__ void cw_zerosum(T* u, int n, double* scal, int* guard, double* blk, d
 
 // The 4^3 level: coarse sweeps forward, then forward or reversed, then the mean removal.
 template <class T>
-__device__ void cw_bottom(T* u, const T* rhs, const T* G, T dx, int nc, bool rev, double* scal, int* guard,
+__device__ void cw_bottom(T* u, const T* rhs, const T* G, const T* A, T dx, int nc, bool rev, double* scal, int* guard,
                           double* blk, double* blk2, double (*s)[RB]) {
-  cw_sweeps<T>(u, rhs, G, 4, dx, nc, false);
-  cw_sweeps<T>(u, rhs, G, 4, dx, nc, rev);
+  cw_sweeps<T>(u, rhs, G, A, 4, dx, nc, false);
+  cw_sweeps<T>(u, rhs, G, A, 4, dx, nc, rev);
   cw_zerosum<T>(u, 4, scal, guard, blk, blk2, s);
 }
 
@@ -139,13 +146,14 @@ template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCy
   // down: pre-smoothing, residual, restriction into the next level's rhs, zero next level's u
   for (int l = 0; l + 1 < C.nl; ++l) {
     const int n = C.n[l];
-    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], n, C.dx[l], C.pre, false);
+    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], C.A[l], n, C.dx[l], C.pre, false);
     const T di2 = T(1) / (C.dx[l] * C.dx[l]);
     const i64 n3 = cube_d(n), h3 = cube_d(n / 2);
     for (i64 p = threadIdx.x; p < n3; p += CW) {
       int i, j, k;
       point_of<1>(p, n, i, j, k);
-      C.r[l][p] = resid_point<T, 1>(C.u[l], C.rhs[l], C.G[l], n, di2, i, j, k);
+      C.r[l][p] = C.A[l] ? gal_resid_point<T>(C.u[l], C.rhs[l], C.A[l], n, i, j, k)
+                         : resid_point<T, 1>(C.u[l], C.rhs[l], C.G[l], n, di2, i, j, k);
     }
     __syncthreads();
     const int nrmax = min(4, max(1, n / 4));
@@ -164,11 +172,12 @@ template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCy
       C.du[p] = 0.0;
     }
     __syncthreads();
-    cw_bottom<double>(C.du, C.drhs, C.dG, C.ddx, C.coarse, C.symmetric, C.dscal, C.dguard, blk, blk2, s);
+    cw_bottom<double>(C.du, C.drhs, C.dG, C.dA, C.ddx, C.coarse, C.symmetric, C.dscal, C.dguard, blk, blk2, s);
     for (int p = threadIdx.x; p < 64; p += CW) C.u[lb][p] = (T)C.du[p];
     __syncthreads();
   } else {
-    cw_bottom<T>(C.u[lb], C.rhs[lb], C.G[lb], C.dx[lb], C.coarse, C.symmetric, C.scal[lb], C.guard[lb], blk, blk2, s);
+    cw_bottom<T>(C.u[lb], C.rhs[lb], C.G[lb], C.A[lb], C.dx[lb], C.coarse, C.symmetric, C.scal[lb], C.guard[lb], blk,
+                 blk2, s);
   }
   // up: prolongation, post-smoothing, mean removal
   for (int l = C.nl - 2; l >= 0; --l) {
@@ -176,7 +185,7 @@ template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCy
     const i64 h3 = cube_d(n / 2);
     for (i64 pp = threadIdx.x; pp < h3; pp += CW) prolong_cell<T>(C.u[l], C.u[l + 1], n, 1, pp);
     __syncthreads();
-    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], n, C.dx[l], C.post, C.symmetric != 0);
+    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], C.A[l], n, C.dx[l], C.post, C.symmetric != 0);
     cw_zerosum<T>(C.u[l], n, C.scal[l], C.guard[l], blk, blk2, s);
   }
 }

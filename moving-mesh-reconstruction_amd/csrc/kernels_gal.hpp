@@ -1,0 +1,221 @@
+// kernels_gal.hpp -- Galerkin coarse operators for the Krylov solve's preconditioner.
+//
+// The reference rebuilds each coarse level's operator from restricted fields (metric of the
+// restricted deformation and density).  With a lognormal density the coefficient contrast is
+// large and that operator is a poor model of the fine one on coarse modes: the PCG iteration count
+// grows with the grid (37 at 256^3, 46 at 512^3, 56 at 1024^3 for sigma = 2).  The Galerkin
+// operator A_{l+1} = R A_l P, with P the trilinear prolongation of the V-cycle (k_prolong) and
+// R = P^T / 8 its full-weighting restriction (k_restrict_resid), is the coarse operator the
+// fine one induces; with it the iteration count is about half and no longer grows.  It is a
+// 27-point stencil: 27 coefficient fields per level, field e = (ex+1) + 3(ey+1) + 9(ez+1) holding
+// the coefficient of offset (ex, ey, ez), each field in the level's sub-lattice layout.
+//
+// Building is local: a coarse row sums, over the 27 fine points f of the full-weighting box and the
+// stencil offsets d of the fine operator at f, R(C, f) a(f, d) P(f + d, C + e).  Per axis the
+// fine offset o = delta + d from 2C feeds coarse offset e with weight 1 when o = 2e and 1/2 when
+// |o - 2e| = 1.  The loops run over compile-time offsets, so the 27 sums stay in registers.
+#pragma once
+#include "kernels.hpp"
+
+namespace mmh {
+
+// Coefficients of the fine 19-point operator L phi = (t1 + t2 + tdiag phi) * di2 (stencil19,
+// mmh/gauss1.f90:954-978) at point (i, j, k), read off the expression: faces from the face
+// averages of G11/G22/G33, edges from the cross terms of G12/G23/G13.
+template <class T>
+__device__ __forceinline__ void coef19(const T* __restrict__ G, int n, int i, int j, int k, double di2, double a[19]) {
+  const i64 n3 = (i64)n * n * n;
+  const int m = n - 1;
+  auto g = [&](int comp, int di, int dj, int dk) -> double {
+    return (double)G[(i64)comp * n3 + lx<1>((i + di) & m, (j + dj) & m, (k + dk) & m, n)];
+  };
+  const double g1 = g(0, 0, 0, 0), g4 = g(3, 0, 0, 0), g6 = g(5, 0, 0, 0);
+  const double b11p = (g(0, 1, 0, 0) + g1) / 2, b11m = (g(0, -1, 0, 0) + g1) / 2;
+  const double b22p = (g(3, 0, 1, 0) + g4) / 2, b22m = (g(3, 0, -1, 0) + g4) / 2;
+  const double b33p = (g(5, 0, 0, 1) + g6) / 2, b33m = (g(5, 0, 0, -1) + g6) / 2;
+  a[0] = -(b11p + b11m + b22p + b22m + b33p + b33m);
+  a[1] = b11p; a[2] = b11m; a[3] = b22p; a[4] = b22m; a[5] = b33p; a[6] = b33m;
+  const double g2jp = g(1, 0, 1, 0), g2jm = g(1, 0, -1, 0), g2ip = g(1, 1, 0, 0), g2im = g(1, -1, 0, 0);
+  a[7] = (g2jp + g2ip) / 4;    // (+1, +1, 0)
+  a[8] = -(g2jp + g2im) / 4;   // (-1, +1, 0)
+  a[9] = -(g2jm + g2ip) / 4;   // (+1, -1, 0)
+  a[10] = (g2jm + g2im) / 4;   // (-1, -1, 0)
+  const double g5kp = g(4, 0, 0, 1), g5km = g(4, 0, 0, -1), g5jp = g(4, 0, 1, 0), g5jm = g(4, 0, -1, 0);
+  a[11] = (g5kp + g5jp) / 4;   // (0, +1, +1)
+  a[12] = -(g5kp + g5jm) / 4;  // (0, -1, +1)
+  a[13] = -(g5km + g5jp) / 4;  // (0, +1, -1)
+  a[14] = (g5km + g5jm) / 4;   // (0, -1, -1)
+  const double g3kp = g(2, 0, 0, 1), g3km = g(2, 0, 0, -1), g3ip = g(2, 1, 0, 0), g3im = g(2, -1, 0, 0);
+  a[15] = (g3kp + g3ip) / 4;   // (+1, 0, +1)
+  a[16] = -(g3kp + g3im) / 4;  // (-1, 0, +1)
+  a[17] = -(g3km + g3ip) / 4;  // (+1, 0, -1)
+  a[18] = (g3km + g3im) / 4;   // (-1, 0, -1)
+#pragma unroll
+  for (int q = 0; q < 19; ++q) a[q] *= di2;
+}
+
+__device__ __forceinline__ constexpr int st27_index(int ex, int ey, int ez) { return (ex + 1) + 3 * (ey + 1) + 9 * (ez + 1); }
+
+// Adds R(C, f) a(f, d) P(f + d, C + e) for one fine point f = 2C + delta and one stencil offset d,
+// whose fine offset from 2C is o = delta + d in [-2, 2]^3 (compile-time).  Per axis: o even ->
+// e = o/2 (weight 1); o odd -> e = (o-1)/2 and (o+1)/2 (weight 1/2 each).
+template <int OX, int OY, int OZ> __device__ __forceinline__ void rap_add(double (&acc)[27], double w) {
+  constexpr int ex0 = (OX - (OX & 1)) / 2, ey0 = (OY - (OY & 1)) / 2, ez0 = (OZ - (OZ & 1)) / 2;
+  constexpr int nx = (OX & 1) ? 2 : 1, ny = (OY & 1) ? 2 : 1, nz = (OZ & 1) ? 2 : 1;
+  constexpr double ww = ((OX & 1) ? 0.5 : 1.0) * ((OY & 1) ? 0.5 : 1.0) * ((OZ & 1) ? 0.5 : 1.0);
+  const double v = w * ww;
+  acc[st27_index(ex0, ey0, ez0)] += v;
+  if constexpr (nx == 2) acc[st27_index(ex0 + 1, ey0, ez0)] += v;
+  if constexpr (ny == 2) acc[st27_index(ex0, ey0 + 1, ez0)] += v;
+  if constexpr (nz == 2) acc[st27_index(ex0, ey0, ez0 + 1)] += v;
+  if constexpr (nx == 2 && ny == 2) acc[st27_index(ex0 + 1, ey0 + 1, ez0)] += v;
+  if constexpr (nx == 2 && nz == 2) acc[st27_index(ex0 + 1, ey0, ez0 + 1)] += v;
+  if constexpr (ny == 2 && nz == 2) acc[st27_index(ex0, ey0 + 1, ez0 + 1)] += v;
+  if constexpr (nx == 2 && ny == 2 && nz == 2) acc[st27_index(ex0 + 1, ey0 + 1, ez0 + 1)] += v;
+}
+
+// the offsets of the 19-point stencil as compile-time triples, in the order of coef19
+template <int S> struct St19;
+#define ST19_(S, X, Y, Z) \
+  template <> struct St19<S> { static constexpr int x = X, y = Y, z = Z; };
+ST19_(0, 0, 0, 0) ST19_(1, 1, 0, 0) ST19_(2, -1, 0, 0) ST19_(3, 0, 1, 0) ST19_(4, 0, -1, 0) ST19_(5, 0, 0, 1)
+ST19_(6, 0, 0, -1) ST19_(7, 1, 1, 0) ST19_(8, -1, 1, 0) ST19_(9, 1, -1, 0) ST19_(10, -1, -1, 0) ST19_(11, 0, 1, 1)
+ST19_(12, 0, -1, 1) ST19_(13, 0, 1, -1) ST19_(14, 0, -1, -1) ST19_(15, 1, 0, 1) ST19_(16, -1, 0, 1)
+ST19_(17, 1, 0, -1) ST19_(18, -1, 0, -1)
+#undef ST19_
+
+// one fine point fd (delta = (fd%3-1, fd/3%3-1, fd/9-1)) of a level-1 row: its 19 coefficients
+template <int FD, int S>
+__device__ __forceinline__ void rap19_terms(double (&acc)[27], const double (&a)[19], double wr) {
+  if constexpr (S < 19) {
+    rap_add<FD % 3 - 1 + St19<S>::x, (FD / 3) % 3 - 1 + St19<S>::y, FD / 9 - 1 + St19<S>::z>(acc, wr * a[S]);
+    rap19_terms<FD, S + 1>(acc, a, wr);
+  }
+}
+template <class T, int FD>
+__device__ __forceinline__ void rap19_points(double (&acc)[27], const T* __restrict__ G, int n, int ci, int cj, int ck,
+                                             double di2) {
+  if constexpr (FD < 27) {
+    constexpr int dx = FD % 3 - 1, dy = (FD / 3) % 3 - 1, dz = FD / 9 - 1;
+    constexpr double wr = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0) / 8.0;
+    const int m = n - 1;
+    double a[19];
+    coef19<T>(G, n, (2 * ci + dx) & m, (2 * cj + dy) & m, (2 * ck + dz) & m, di2, a);
+    rap19_terms<FD, 0>(acc, a, wr);
+    rap19_points<T, FD + 1>(acc, G, n, ci, cj, ck, di2);
+  }
+}
+
+// Level 1 from the fine 19-point operator: one thread per coarse point q (column-major coarse
+// order), A1 (27 fields of (n/2)^3, sub-lattice layout of the coarse level).
+template <class T>
+__global__ __launch_bounds__(256) void k_rap19(double* __restrict__ A1, const T* __restrict__ G, int n, double di2) {
+  const int h = n >> 1;
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 h3 = (i64)h * h * h;
+  if (q >= h3) return;
+  int ci, cj, ck;
+  point_of<0>(q, h, ci, cj, ck);
+  double acc[27];
+#pragma unroll
+  for (int e = 0; e < 27; ++e) acc[e] = 0.0;
+  rap19_points<T, 0>(acc, G, n, ci, cj, ck, di2);
+  const i64 p = lx<1>(ci, cj, ck, h);
+#pragma unroll
+  for (int e = 0; e < 27; ++e) A1[(i64)e * h3 + p] = acc[e];
+}
+
+template <int FD, int S>
+__device__ __forceinline__ void rap27_terms(double (&acc)[27], const double* __restrict__ A, i64 n3, i64 f, double wr) {
+  if constexpr (S < 27) {
+    rap_add<FD % 3 - 1 + S % 3 - 1, (FD / 3) % 3 - 1 + (S / 3) % 3 - 1, FD / 9 - 1 + S / 9 - 1>(acc, wr * A[(i64)S * n3 + f]);
+    rap27_terms<FD, S + 1>(acc, A, n3, f, wr);
+  }
+}
+template <int FD>
+__device__ __forceinline__ void rap27_points(double (&acc)[27], const double* __restrict__ A, int n, int ci, int cj, int ck) {
+  if constexpr (FD < 27) {
+    constexpr int dx = FD % 3 - 1, dy = (FD / 3) % 3 - 1, dz = FD / 9 - 1;
+    constexpr double wr = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0) / 8.0;
+    const int m = n - 1;
+    const i64 f = lx<1>((2 * ci + dx) & m, (2 * cj + dy) & m, (2 * ck + dz) & m, n);
+    rap27_terms<FD, 0>(acc, A, (i64)n * n * n, f, wr);
+    rap27_points<FD + 1>(acc, A, n, ci, cj, ck);
+  }
+}
+
+// Level l+1 from the 27-point level l (size n): one thread per coarse point.
+__global__ __launch_bounds__(256) void k_rap27(double* __restrict__ Ac, const double* __restrict__ A, int n) {
+  const int h = n >> 1;
+  const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const i64 h3 = (i64)h * h * h;
+  if (q >= h3) return;
+  int ci, cj, ck;
+  point_of<0>(q, h, ci, cj, ck);
+  double acc[27];
+#pragma unroll
+  for (int e = 0; e < 27; ++e) acc[e] = 0.0;
+  rap27_points<0>(acc, A, n, ci, cj, ck);
+  const i64 p = lx<1>(ci, cj, ck, h);
+#pragma unroll
+  for (int e = 0; e < 27; ++e) Ac[(i64)e * h3 + p] = acc[e];
+}
+
+// sum over the 26 off-centre offsets of A(p, e) phi(p + e) at point (i, j, k) of a 27-point level
+template <class T>
+__device__ __forceinline__ T gal_offsum(const T* __restrict__ phi, const T* __restrict__ A, int n, int i, int j, int k,
+                                       i64 p) {
+  const int m = n - 1;
+  const i64 n3 = (i64)n * n * n;
+  T s = T(0);
+#pragma unroll
+  for (int e = 0; e < 27; ++e) {
+    if (e == 13) continue;
+    const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
+    s = s + A[(i64)e * n3 + p] * phi[lx<1>((i + ex) & m, (j + ey) & m, (k + ez) & m, n)];
+  }
+  return s;
+}
+
+// Gauss-Seidel update of point q of colour class (kpar, io, jo) on a 27-point level (the 8
+// classes decouple the 27-point stencil as they do the 19-point one).
+template <class T>
+__device__ __forceinline__ void gal_colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
+                                                 int n, int kpar, int io, int jo, i64 q) {
+  const int h = n >> 1;
+  const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
+  const i64 p = lx<1>(i, j, k, n);
+  const i64 n3 = (i64)n * n * n;
+  phi[p] = (rhs[p] - gal_offsum<T>(phi, A, n, i, j, k, p)) / A[13 * n3 + p];
+}
+
+template <class T>
+__device__ __forceinline__ T gal_resid_point(const T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
+                                            int n, int i, int j, int k) {
+  const i64 p = lx<1>(i, j, k, n);
+  const i64 n3 = (i64)n * n * n;
+  return rhs[p] - (gal_offsum<T>(phi, A, n, i, j, k, p) + A[13 * n3 + p] * phi[p]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
+                                                    int n, int kpar, int io, int jo) {
+  const int h = n >> 1;
+  const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
+  const i64 q = (i64)blk * blockDim.x + threadIdx.x;
+  if (q >= (i64)h * h * h) return;
+  gal_colour_point<T>(phi, rhs, A, n, kpar, io, jo, q);
+}
+
+// r = rhs - A phi at every point (one thread per point, sub-lattice order); r stored.
+template <class T>
+__global__ __launch_bounds__(256) void k_gal_resid(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
+                                                   const T* __restrict__ A, int n) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (i64)n * n * n) return;
+  int i, j, k;
+  point_of<1>(p, n, i, j, k);
+  r[p] = gal_resid_point<T>(phi, rhs, A, n, i, j, k);
+}
+
+}  // namespace mmh

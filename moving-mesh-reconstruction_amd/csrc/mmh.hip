@@ -28,6 +28,7 @@
 #include "kernels_vis.hpp"
 #include "kernels_march.hpp"
 #include "kernels_coarse.hpp"
+#include "kernels_gal.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -46,6 +47,7 @@ struct mmh_ctx {
   int wave_p = 1;          // points per thread in flight per colour phase of the wavefront kernel
   int march_op = 1;        // residual/operator of large levels by k-marching workgroups (option "march_op")
   int march_tile = 0;      // tile shape of the k-marching kernels (option "march_tile": 0, 1, 2)
+  int galerkin = 1;        // Krylov preconditioner: Galerkin coarse operators (option "galerkin")
   int coarse_fused = 16;   // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
   int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
@@ -98,6 +100,21 @@ void* ws_get(mmh_ctx* c, const std::string& key, size_t bytes) {
   c->ws[key] = {p, bytes};
   return p;
 }
+// Frees one cached buffer (after the queued work that may use it).
+void ws_free(mmh_ctx* c, const std::string& key) {
+  auto it = c->ws.find(key);
+  if (it == c->ws.end()) return;
+  if (c->graphs_live) throw MmhError(MMH_EDEVICE, "internal: workspace '" + key + "' freed while a graph is live");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(it->second.first));
+  c->ws.erase(it);
+}
+// Bytes a wsT request would newly allocate (0 if the key already holds enough).
+size_t ws_need(const mmh_ctx* c, const std::string& key, size_t bytes) {
+  auto it = c->ws.find(key);
+  return (it != c->ws.end() && it->second.second >= bytes) ? 0 : bytes;
+}
+
 template <class T> T* wsT(mmh_ctx* c, const std::string& key, i64 count) {
   return (T*)ws_get(c, key + ":" + std::to_string(sizeof(T)), sizeof(T) * (size_t)count);
 }
@@ -204,6 +221,7 @@ template <class T> struct Hier {
   const T* u1[MMH_MAXLEV];
   T* G[MMH_MAXLEV];
   T* r[MMH_MAXLEV];
+  T* A[MMH_MAXLEV] = {};  // Galerkin coarse operators (27 fields per level), null where not built
   double* scal;  // per level: [0] r1, [1] relaxerr, [2] sum; then [3*MAXLEV] rhs sums
   int* guard;    // per level, plus entry flag at [MAXLEV]
 };
@@ -262,6 +280,42 @@ void build_hier(mmh_ctx* c, Hier<T>& H, const std::string& tag, T* u, T* rhs, co
       k_metric<T, 1><<<nblk(cube(m)), 256, 0, c->stream>>>(H.G[l], H.def[l], H.u1[l], H.nu[l], m, H.dx[l], lsquare);
       LAUNCHCHK();
     }
+  }
+}
+
+// The Galerkin operators take 27 values per coarse point (about 3.9 fine-grid words, plus half
+// that again in FP32 for the mixed preconditioner).  They are built only where they fit, with the
+// rest of the Krylov solve's workspace (its vectors, the FP32 hierarchy) and 4 GiB to spare, beside
+// the memory already held; otherwise the rebuilt coarse operators are used.
+bool galerkin_fits(mmh_ctx* c, const Hier<double>& H, const std::string& tag, bool mixed) {
+  size_t need = 0;
+  const size_t n3 = (size_t)cube(H.n[0]);
+  for (const char* k : {"cg_r:8", "cg_ro:8", "cg_p:8", "cg_q:8", "det:8"}) need += ws_need(c, k, n3 * sizeof(double));
+  if (!mixed) need += ws_need(c, "cg_z:8", n3 * sizeof(double));
+  for (int l = 0; l < H.nlev; ++l) {
+    const size_t m3 = (size_t)cube(H.n[l]);
+    const std::string t = "sf" + std::to_string(l);
+    if (l > 0) need += ws_need(c, tag + std::to_string(l) + "A:8", 27 * m3 * sizeof(double));
+    if (!mixed) continue;
+    if (l > 0) need += ws_need(c, t + "A:4", 27 * m3 * sizeof(float));
+    need += ws_need(c, t + "u:4", m3 * sizeof(float)) + ws_need(c, t + "rhs:4", m3 * sizeof(float)) +
+            ws_need(c, t + "r:4", m3 * sizeof(float)) + ws_need(c, t + "G:4", 6 * m3 * sizeof(float));
+  }
+  size_t freeb = 0, total = 0;
+  HIPCHK(hipMemGetInfo(&freeb, &total));
+  return need + ((size_t)4 << 30) <= freeb;
+}
+
+// Galerkin operators A_l = R A_{l-1} P of levels 1.. (FP64), the first from the fine metric.
+void build_galerkin(mmh_ctx* c, Hier<double>& H, const std::string& tag) {
+  for (int l = 1; l < H.nlev; ++l) {
+    const int m = H.n[l];
+    H.A[l] = wsT<double>(c, tag + std::to_string(l) + "A", 27 * cube(m));
+    if (l == 1)
+      k_rap19<double><<<nblk(cube(m)), 256, 0, c->stream>>>(H.A[1], H.G[0], H.n[0], 1.0 / (H.dx[0] * H.dx[0]));
+    else
+      k_rap27<<<nblk(cube(m)), 256, 0, c->stream>>>(H.A[l], H.A[l - 1], H.n[l - 1]);
+    LAUNCHCHK();
   }
 }
 
@@ -426,6 +480,26 @@ void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n,
   }
 }
 
+// Sweeps and relax of a Galerkin (27-point) level: the colour order of the 19-point sweeps, the
+// residual stored in r and restricted into rc with calcresid's plane order rule.
+template <class T> void gal_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* A, int n, int ns, bool reverse) {
+  const unsigned nb = nblk(cube(n / 2));
+  for (int s = 0; s < ns; ++s)
+    for (int cc = 0; cc < 8; ++cc) {
+      const int col = reverse ? 7 - cc : cc;
+      k_gal_colour<T><<<nb, 256, 0, c->stream>>>(phi, rhs, A, n, col >> 2, (col >> 1) & 1, col & 1);
+    }
+  LAUNCHCHK();
+}
+template <class T> void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const T* A, T* r, int n, int nrelax, T* rc) {
+  gal_sweeps<T>(c, phi, rhs, A, n, nrelax, false);
+  k_gal_resid<T><<<nblk(cube(n)), 256, 0, c->stream>>>(r, phi, rhs, A, n);
+  const int nrmax = std::min(4, std::max(1, n / 4));
+  const int nrl = nrelax > nrmax ? (nrelax % nrmax ? nrelax % nrmax : nrmax) : nrelax;
+  k_restrict_resid<T, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(rc, r, n, nrl % (n / 2));
+  LAUNCHCHK();
+}
+
 // The levels l.. of a hierarchy as one coarse cycle (k_coarse_cycle), when the schedule allows it:
 // the Krylov preconditioner's fixed symmetric schedule without divergence guard, levels up to the
 // size set by option "coarse_fused" (16 by default: one workgroup is faster than ~40 launches at
@@ -446,6 +520,7 @@ template <class T> CoarseCycle<T> coarse_cycle_of(Hier<T>& H, int l, const Sched
     C.guard[q] = H.guard + l + q;
     C.dx[q] = H.dx[l + q];
     C.n[q] = H.n[l + q];
+    C.A[q] = H.A[l + q];
   }
   C.pre = S.nu_pre;
   C.post = S.nu_post;
@@ -467,19 +542,26 @@ template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const 
   }
   if (n == 4) {
     const int nc = S.coarse();
-    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 0);
-    relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 1, S.symmetric);
+    if (H.A[l]) {
+      gal_sweeps<T>(c, H.u[l], H.rhs[l], H.A[l], 4, nc, false);
+      gal_sweeps<T>(c, H.u[l], H.rhs[l], H.A[l], 4, nc, S.symmetric);
+    } else {
+      relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 0);
+      relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 1, S.symmetric);
+    }
     reduce<T>(c, H.u[l], n3, 0, sc + 2);
     k_zerosum_guard<T><<<1, 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 2, H.guard + l);
     LAUNCHCHK();
     return;
   }
-  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.pre(n), S.guard, H.rhs[l + 1], sc + 0);
+  if (H.A[l]) gal_relax<T>(c, H.u[l], H.rhs[l], H.A[l], H.r[l], n, S.pre(n), H.rhs[l + 1]);
+  else relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.pre(n), S.guard, H.rhs[l + 1], sc + 0);
   fill<T>(c, H.u[l + 1], T(0), cube(n / 2));
   vcycle(c, H, l + 1, ng_top, S);
   k_prolong<T, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(H.u[l], H.u[l + 1], n, 1);
   LAUNCHCHK();
-  relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.post(n), S.guard, (T*)nullptr, sc + 1, S.symmetric);
+  if (H.A[l]) gal_sweeps<T>(c, H.u[l], H.rhs[l], H.A[l], n, S.post(n), S.symmetric);
+  else relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.post(n), S.guard, (T*)nullptr, sc + 1, S.symmetric);
   reduce<T>(c, H.u[l], n3, 0, sc + 2);
   k_zerosum_guard<T><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
                                                       H.guard + l);
@@ -953,6 +1035,7 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
     C.dscal = Hd.scal + 3 * lb;
     C.dguard = Hd.guard + lb;
     C.ddx = Hd.dx[lb];
+    C.dA = Hd.A[lb];
     k_coarse_cycle<float><<<1, CW, 0, c->stream>>>(C);
     LAUNCHCHK();
     return;
@@ -969,14 +1052,16 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   }
   const i64 n3 = cube(n);
   double* sc = Hf.scal + 3 * l;
-  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1], sc + 0, false,
-            S.delta);
+  if (Hf.A[l]) gal_relax<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1]);
+  else relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1], sc + 0, false,
+                 S.delta);
   fill<float>(c, Hf.u[l + 1], 0.f, cube(n / 2));
   vcycle_mixed(c, Hf, Hd, l + 1, ng_top, S);
   k_prolong<float, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
   LAUNCHCHK();
-  relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1, S.symmetric,
-            S.delta);
+  if (Hf.A[l]) gal_sweeps<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], n, S.post(n), S.symmetric);
+  else relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1,
+                 S.symmetric, S.delta);
   reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
   k_zerosum_guard<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
                                                           Hf.guard + l);
@@ -999,6 +1084,13 @@ void build_fp32(mmh_ctx* c, Hier<float>& F, Hier<double>& H) {
     for (int f = 0; f < 6; ++f)  // one coefficient field per launch (n^3 <= 2^30 work-items)
       k_convert<double, float><<<nblk(cube(m)), 256, 0, c->stream>>>(F.G[l] + f * cube(m), H.G[l] + f * cube(m),
                                                                      cube(m));
+    F.A[l] = nullptr;
+    if (H.A[l]) {
+      F.A[l] = wsT<float>(c, t + "A", 27 * cube(m));
+      for (int f = 0; f < 27; ++f)
+        k_convert<double, float><<<nblk(cube(m)), 256, 0, c->stream>>>(F.A[l] + f * cube(m), H.A[l] + f * cube(m),
+                                                                       cube(m));
+    }
     LAUNCHCHK();
   }
 }
@@ -1211,6 +1303,18 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
   double* rs = wsT<double>(c, "sd_rs", n3);
   Hier<double> H;
   build_hier(c, H, "sd", us, rs, defpot, ubig, nu, 1.0, n, 0);
+  if (o->krylov && c->galerkin && !S.delta && galerkin_fits(c, H, "sd", o->mixed != 0)) {
+    try {
+      build_galerkin(c, H, "sd");
+    } catch (const MmhError& e) {
+      if (e.code != MMH_ENOMEM) throw;
+      // out of memory after all: release what was built and use the rebuilt operators
+      for (int l = 1; l < H.nlev; ++l) {
+        ws_free(c, "sd" + std::to_string(l) + "A:8");
+        H.A[l] = nullptr;
+      }
+    }
+  }
   double* sums = H.scal + 3 * MMH_MAXLEV;
   rgzerosum_dev<double>(c, rhs, defpot, 1.0, n, sums);
   reduce<double>(c, rhs, n3, 1, sums + 3);
@@ -1231,7 +1335,15 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
   }
   if (o->krylov) {
     Hier<float> F;
-    if (o->mixed) build_fp32(c, F, H);
+    if (o->mixed) {
+      build_fp32(c, F, H);
+      // the FP32 cycle reads FP32 copies; only the FP64 4^3 level keeps its FP64 operator
+      for (int l = 1; l + 1 < H.nlev; ++l)
+        if (H.A[l]) {
+          ws_free(c, "sd" + std::to_string(l) + "A:8");
+          H.A[l] = nullptr;
+        }
+    }
     try {
       if (o->mixed == 2) refine_fp32(c, H, F, us, rs, n, S, o, st, rden, sums);
       else pcg(c, H, o->mixed ? &F : nullptr, us, rs, n, S, o, st, rden, sums);
@@ -1552,6 +1664,8 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
     } else if (k == "march_tile") {
       require(value >= 0 && value <= 2, "set_option: march_tile is 0, 1 or 2");
       c->march_tile = value;
+    } else if (k == "galerkin") {
+      c->galerkin = value ? 1 : 0;
     } else if (k == "coarse_fused") {
       require(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
               "set_option: coarse_fused is 0 (off) or the largest fused level size, 4 to 32");
@@ -1696,6 +1810,23 @@ int mmh_residual_d(mmh_ctx* c, double* r, const double* phi, const double* rhs, 
     double* bm = (double*)ws_get(c, "res_bm", sizeof(double) * nb);
     k_resid<double, 0><<<nb, RB, 0, c->stream>>>(r, bm, phi, rhs, G, n, 1.0 / (dx * dx));
     LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_galerkin_d(mmh_ctx* c, double* A_out, const double* defpot, const double* ubig, int nu, int n, double dx,
+                   int level) {
+  return guarded(c, [&] {
+    require(is_pow2(n) && n >= 8 && n <= 1024 && nu >= 1, "galerkin: bad size");
+    require(level >= 1 && (n >> level) >= 4, "galerkin: level out of range");
+    const i64 n3 = cube(n);
+    double* us = wsT<double>(c, "gk_us", n3);
+    double* rs = wsT<double>(c, "gk_rs", n3);
+    Hier<double> H;
+    build_hier(c, H, "gk", us, rs, defpot, ubig, nu, dx, n, 0);
+    build_galerkin(c, H, "gk");
+    const int m = n >> level;
+    relayout(c, A_out, H.A[level], m, 27, 0);
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }

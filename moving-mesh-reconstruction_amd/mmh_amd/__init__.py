@@ -226,6 +226,13 @@ class Context:
         nu = nu or u.numel() // phi.numel()
         self.call("mmh_residual_d", _ptr(r), _ptr(phi), _ptr(rhs), _ptr(def_), _ptr(u), nu, n, C.c_double(dx))
 
+    def galerkin(self, A, def_, u, level, dx=1.0, nu=None):
+        """Galerkin coarse operator of the Krylov preconditioner at `level` (27 fields of
+        (n >> level)^3, Fortran order each) from the fine operator of def and u(1)."""
+        n = _cube_side(def_)
+        nu = nu or u.numel() // def_.numel()
+        self.call("mmh_galerkin_d", _ptr(A), _ptr(def_), _ptr(u), nu, n, C.c_double(dx), level)
+
     def particle_density(self, xp, ng, nf, den, wp=None):
         """haloden.f90's density assignment: xp device float32 (np, 3) positions in [0, nf);
         den device float32 ng^3 (Fortran order) receives the mean-1 density."""

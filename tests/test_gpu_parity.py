@@ -323,7 +323,9 @@ def test_mixed_preconditioner_1024(ctx):
     """At 1024^3 the FP32 copy of the metric is 6 * 2^30 values, more than one dispatch can cover;
     three PCG iterations with the FP32 preconditioner must track the FP64 ones: iterates to 1e-6
     relative, residual histories to 1e-3 (FP32 rounding of the V-cycle at |phi| ~ 255 moves the
-    first residuals by about 1e-4; a truncated metric copy diverges instead)."""
+    first residuals by about 1e-4; a truncated metric copy diverges instead).  Both runs use the
+    rebuilt coarse operators (option galerkin = 0): at 1024^3 the Galerkin ones fit beside the
+    FP64 run's workspace for one precision only, and the comparison needs the same preconditioner."""
     n = 1024
     print("case mixed preconditioner n=%d" % n, flush=True)
     f64 = dict(dtype=torch.float64, device="cuda:0")
@@ -332,12 +334,16 @@ def test_mixed_preconditioner_1024(ctx):
     d = torch.zeros(n ** 3, **f64)
     rhs0 = (1.0 - rho) / 30.0
     out = []
-    for mixed in (0, 1):
-        phi, rhs = torch.zeros(n ** 3, **f64), rhs0.clone()
-        st, ok = ctx.solve(phi, rhs, d, rho, tol=1e-30, max_cycles=3, schedule=1, nu_pre=2, nu_post=2, nu_coarse=8,
-                           mixed=mixed, nu=1, krylov=True)
-        out.append((phi, [st.relres[i] for i in range(st.cycles)]))
-        del rhs
+    ctx.set_option("galerkin", 0)
+    try:
+        for mixed in (0, 1):
+            phi, rhs = torch.zeros(n ** 3, **f64), rhs0.clone()
+            st, ok = ctx.solve(phi, rhs, d, rho, tol=1e-30, max_cycles=3, schedule=1, nu_pre=2, nu_post=2,
+                               nu_coarse=8, mixed=mixed, nu=1, krylov=True)
+            out.append((phi, [st.relres[i] for i in range(st.cycles)]))
+            del rhs
+    finally:
+        ctx.set_option("galerkin", 1)
     ctx.release_workspace()
     (a, ra), (b, rb) = out
     print("relres fp64 %s mixed %s" % (ra, rb), flush=True)
