@@ -7,9 +7,10 @@ P(k) ~ k^-1.5 exp(-(k/k_s)^2), k_s = k_Nyquist/2, seed 42, periodic N^3 grid.  T
 the mesh equation of the first outer iteration, L(def = 0, rho) phi = rhs, rhs from the grid
 limiter (calcdefp's right-hand side), solved from phi = 0 until
 max|rhs - L phi| / max|rhs| <= 1e-8.  Solver: flexible preconditioned CG whose preconditioner is one
-symmetric V(2,2) cycle of the reference's 8-colour curvilinear Gauss-Seidel multigrid (FP64, or FP32
-with an FP64 coarsest level for the mixed path); the plain reference V-cycle iteration is timed per
-cycle beside it (on this input it does not reach 1e-8 at 512^3 within 120 cycles).
+symmetric V-cycle with the reference's 8-colour curvilinear Gauss-Seidel smoother on the finest level
+(1+1 sweeps) and Galerkin coarse operators below (2+2 sweeps, 4+4 on 4^3); FP64, or FP32 with an FP64
+coarsest level for the mixed path.  The plain reference V-cycle iteration is timed per cycle beside
+it (on this input it does not reach 1e-8 at 512^3 within 120 cycles).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512]
 
@@ -157,16 +158,22 @@ def batched_leg(ctx, torch):
         return time.perf_counter() - t0, st
 
     def looped():
+        # the same algorithm as the batch: rebuilt coarse operators, V(2,2) on every level
         work = [(torch.zeros_like(r), r.clone(), d, u) for (u, r, d) in probs]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for phi, r, d, u in work:
-            st, ok = ctx.solve(phi, r, d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2, nu_coarse=8,
-                               nu=5, krylov=True)
-            if not ok:
-                raise RuntimeError("single solve did not converge")
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
+        ctx.set_option("galerkin", 0)
+        ctx.set_option("nu_top", 0)
+        try:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for phi, r, d, u in work:
+                st, ok = ctx.solve(phi, r, d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
+                                   nu_coarse=8, nu=5, krylov=True)
+                if not ok:
+                    raise RuntimeError("single solve did not converge")
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+        finally:
+            ctx.set_option("galerkin", 1)
 
     batched()  # warm-up: workspaces and launch-shape tuning
     tb, st = batched()
@@ -189,8 +196,10 @@ def main():
     ap.add_argument("--sigma", type=float, default=2.0)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--nu", type=int, default=2, help="pre/post sweeps of the preconditioning V-cycle")
-    ap.add_argument("--nu-coarse", type=int, default=8, help="sweeps each way on the 4^3 level")
+    ap.add_argument("--nu", type=int, default=2, help="pre/post sweeps of the preconditioning V-cycle below the top")
+    ap.add_argument("--nu-top", type=int, default=1, help="pre/post sweeps on the finest level (0: same as --nu)")
+    ap.add_argument("--nu-coarse", type=int, default=4, help="sweeps each way on the 4^3 level")
+    ap.add_argument("--no-galerkin", action="store_true", help="coarse levels rebuilt from restricted fields")
     ap.add_argument("--ref-cycles", type=int, default=3, help="plain reference V-cycles timed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweeps", type=int, default=20, help="sweeps timed for the roofline")
@@ -235,7 +244,11 @@ def main():
         krylov_probe(ctx, torch, defn, u, rhs0, n)
         return
 
+    # preconditioner: Galerkin coarse operators, nu_top sweeps each way on the finest level
+    ctx.set_option("galerkin", 0 if args.no_galerkin else 1)
+
     def step(mixed):
+        ctx.set_option("nu_top", args.nu_top)
         phi = torch.zeros(n3, **f64)
         rhs = rhs0.clone()
         st, ok = ctx.solve(phi, rhs, defn, u, tol=args.tol, max_cycles=200, schedule=1, nu_pre=args.nu,
@@ -321,7 +334,11 @@ def main():
         "dtype": "f32 multigrid preconditioner inside f64 PCG",
         "data": "synthetic lognormal density (sigma=%.1f, seed %d), generated on device" % (args.sigma, args.seed),
         "config": {"workload": "first-iteration mesh solve to %.0e relative residual, %d^3 periodic, "
-                               "PCG with a symmetric V(%d,%d) multigrid preconditioner" % (args.tol, n, args.nu, args.nu),
+                               "PCG with a symmetric V-cycle preconditioner (%s coarse operators; %d+%d sweeps on "
+                               "the finest level, %d+%d below, %d+%d on 4^3)" % (
+                                   args.tol, n, "rebuilt" if args.no_galerkin else "Galerkin",
+                                   args.nu_top or args.nu, args.nu_top or args.nu, args.nu, args.nu,
+                                   args.nu_coarse, args.nu_coarse),
                    "n": n, "sigma": args.sigma, "seed": args.seed,
                    "fp64_s": round(t64, 4), "fp64_cycles": c64, "mixed_cycles": cmx,
                    "speedup_mixed_vs_fp64": round(t64 / tmx, 3),

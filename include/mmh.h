@@ -112,6 +112,11 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *   "galerkin"    1 (default) / 0: Galerkin coarse operators (mmh_galerkin_d) on the levels of
  *                 the Krylov solve's preconditioner; 0 rebuilds them from restricted fields as the
  *                 reference's V-cycle does (changes the preconditioner, not the converged result)
+ *   "nu_top"      0 (default) or the sweeps each way on the finest level of the Krylov solve's
+ *                 preconditioner (nu_pre/nu_post then apply below it); with Galerkin coarse levels
+ *                 1 reaches the iteration count of 2 at 512^3
+ *   "coarse_fused" 4 (default), 8, 16, 32 or 0: the preconditioner's levels up to this size run in
+ *                 one workgroup (bit-identical)
  *   "pcg_graph"   1 (default) / 0: the Krylov solve (mmh_solve_d, krylov = 1) captures its
  *                 preconditioning V-cycle as a HIP graph on its second call and replays it after;
  *                 on the null stream the solve then runs on a stream of its own, ordered after

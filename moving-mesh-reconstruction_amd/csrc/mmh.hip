@@ -48,7 +48,8 @@ struct mmh_ctx {
   int march_op = 1;        // residual/operator of large levels by k-marching workgroups (option "march_op")
   int march_tile = 0;      // tile shape of the k-marching kernels (option "march_tile": 0, 1, 2)
   int galerkin = 1;        // Krylov preconditioner: Galerkin coarse operators (option "galerkin")
-  int coarse_fused = 16;   // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
+  int nu_top = 0;          // Krylov preconditioner: sweeps on the finest level, 0 = nu_pre (option "nu_top")
+  int coarse_fused = 4;    // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
   int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
@@ -233,8 +234,9 @@ struct Sched {
   bool guard = true;       // the reference's divergence guard on coarse levels
   bool symmetric = false;  // post-smoothing in reverse colour order (symmetric V-cycle)
   bool delta = false;      // difference form of the FP32 stencil (mixed preconditioner only)
-  int pre(int n) const { return kind == 0 ? 4 * ng / n : nu_pre; }
-  int post(int n) const { return kind == 0 ? 4 * ng / n : nu_post; }
+  int nu_top = 0;  // kind 1: sweeps each way on the finest level when > 0 (option "nu_top")
+  int pre(int n) const { return kind == 0 ? 4 * ng / n : (nu_top > 0 && n == ng ? nu_top : nu_pre); }
+  int post(int n) const { return kind == 0 ? 4 * ng / n : (nu_top > 0 && n == ng ? nu_top : nu_post); }
   int coarse() const { return kind == 0 ? 32 : nu_coarse; }
 };
 
@@ -502,11 +504,12 @@ template <class T> void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const T* A, 
 
 // The levels l.. of a hierarchy as one coarse cycle (k_coarse_cycle), when the schedule allows it:
 // the Krylov preconditioner's fixed symmetric schedule without divergence guard, levels up to the
-// size set by option "coarse_fused" (16 by default: one workgroup is faster than ~40 launches at
-// 16^3 and below, slower at 32^3 where a level's sweeps move megabytes through one CU).
+// size set by option "coarse_fused" (4 by default: with the rebuilt 19-point operators one
+// workgroup beats the launches up to 16^3, with the 27-point Galerkin operators only at 4^3; at
+// 32^3 a level's sweeps move megabytes through one CU and the launches win).
 template <class T> bool coarse_cycle_applies(const mmh_ctx* c, const Hier<T>& H, int l, const Sched& S) {
   return c->coarse_fused && !S.guard && !S.delta && S.kind == 1 && S.nu_pre == S.nu_post &&
-         H.n[l] <= std::min(c->coarse_fused, CTOP_MAX) && H.nlev - l <= CLEV_MAX;
+         !(S.nu_top > 0 && H.n[l] == S.ng) && H.n[l] <= std::min(c->coarse_fused, CTOP_MAX) && H.nlev - l <= CLEV_MAX;
 }
 template <class T> CoarseCycle<T> coarse_cycle_of(Hier<T>& H, int l, const Sched& S) {
   CoarseCycle<T> C{};
@@ -1296,6 +1299,7 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
     S.guard = false;
     S.symmetric = true;
     S.ng = n;
+    S.nu_top = c->nu_top;
   }
   S.delta = o->mixed != 0 && c->mixed_delta;
   const i64 n3 = cube(n);
@@ -1666,6 +1670,9 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->march_tile = value;
     } else if (k == "galerkin") {
       c->galerkin = value ? 1 : 0;
+    } else if (k == "nu_top") {
+      require(value >= 0 && value <= 16, "set_option: nu_top is 0 to 16");
+      c->nu_top = value;
     } else if (k == "coarse_fused") {
       require(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
               "set_option: coarse_fused is 0 (off) or the largest fused level size, 4 to 32");

@@ -135,7 +135,7 @@ def test_single_solve_coarse_cycle_equals_launches(ctx, probs, mixed):
                 st, ok = ctx.solve(phi, r.clone(), d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
                                    nu_coarse=8, mixed=mixed, nu=5, krylov=True)
             finally:
-                ctx.set_option("coarse_fused", 16)
+                ctx.set_option("coarse_fused", 4)
             assert ok, case
             out.append((phi.cpu().numpy(), [st.relres[i] for i in range(st.cycles)]))
         xl, rl = out[-1]
