@@ -19,6 +19,7 @@ struct BatchGroup {
   double* rhs[MMH_MAXLEV];
   double* r[MMH_MAXLEV];
   double* G[MMH_MAXLEV];
+  double* A[MMH_MAXLEV] = {};  // Galerkin operators of levels 1.. (27 m^3 per grid), if built
   // PCG vectors at the finest level (sub-lattice layout), B n^3 each
   double *x, *b, *cr, *ro, *p, *q, *z;
   // scratch of the per-grid reductions, and per-grid scalars
@@ -76,6 +77,18 @@ void batch_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, con
   LAUNCHCHK();
 }
 
+// Sweeps of a batched Galerkin level (gal_sweeps for the stack).
+void batch_gal_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, int l, int ns, bool reverse) {
+  const int m = g.m[l];
+  const dim3 grid(nblk(cube(m / 2)), g.B);
+  for (int sw = 0; sw < ns; ++sw)
+    for (int cc = 0; cc < 8; ++cc) {
+      const int col = reverse ? 7 - cc : cc;
+      k_gal_colour<double><<<grid, 256, 0, c->stream>>>(phi, rhs, g.A[l], m, col >> 2, (col >> 1) & 1, col & 1, cube(m));
+    }
+  LAUNCHCHK();
+}
+
 // Symmetric V(nu, nu) cycle of the batch from level l, the arithmetic of vcycle<double> with the
 // Krylov schedule (no guard, reverse colour order on the way up, mean removal on every level).
 void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, const Sched& S) {
@@ -85,16 +98,27 @@ void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, co
   double* rh = l == 0 ? rhs0 : g.rhs[l];
   const auto shape = batch_shape(c, m, g.B);
   if (m == 4) {
-    batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), false, shape);
-    batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), S.symmetric, shape);
+    if (g.A[l]) {
+      batch_gal_sweeps(c, g, u, rh, l, S.coarse(), false);
+      batch_gal_sweeps(c, g, u, rh, l, S.coarse(), S.symmetric);
+    } else {
+      batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), false, shape);
+      batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), S.symmetric, shape);
+    }
     batch_reduce(c, g, u, m3, 0, g.sums);
     k_zerosum_guard<double><<<dim3(1, g.B), 256, 0, c->stream>>>(u, g.sums, nullptr, m3, 0, nullptr, 1);
     LAUNCHCHK();
     return;
   }
   const int pre = S.pre(m);
-  batch_sweeps(c, g, u, rh, g.G[l], l, pre, false, shape);
-  op_resid<double>(c, g.r[l], g.bm, u, rh, g.G[l], m, 1.0 / (g.dx[l] * g.dx[l]), false, g.B, 1);
+  if (g.A[l]) {
+    batch_gal_sweeps(c, g, u, rh, l, pre, false);
+    k_gal_resid<double><<<dim3(nblk(m3), g.B), 256, 0, c->stream>>>(g.r[l], u, rh, g.A[l], m, m3);
+    LAUNCHCHK();
+  } else {
+    batch_sweeps(c, g, u, rh, g.G[l], l, pre, false, shape);
+    op_resid<double>(c, g.r[l], g.bm, u, rh, g.G[l], m, 1.0 / (g.dx[l] * g.dx[l]), false, g.B, 1);
+  }
   const int nrmax = std::min(4, std::max(1, m / 4));
   const int nrl = pre > nrmax ? (pre % nrmax ? pre % nrmax : nrmax) : pre;
   k_restrict_resid<double, 1><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(g.rhs[l + 1], g.r[l], m,
@@ -104,7 +128,8 @@ void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, co
   batch_vcycle(c, g, l + 1, u0, rhs0, S);
   k_prolong<double, 1><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(u, g.u[l + 1], m, 1, 1);
   LAUNCHCHK();
-  batch_sweeps(c, g, u, rh, g.G[l], l, S.post(m), S.symmetric, shape);
+  if (g.A[l]) batch_gal_sweeps(c, g, u, rh, l, S.post(m), S.symmetric);
+  else batch_sweeps(c, g, u, rh, g.G[l], l, S.post(m), S.symmetric, shape);
   batch_reduce(c, g, u, m3, 0, g.sums);
   k_zerosum_guard<double><<<dim3(nblk(m3), g.B), 256, 0, c->stream>>>(u, g.sums, nullptr, m3, 0, nullptr, 1);
   LAUNCHCHK();
@@ -196,6 +221,7 @@ void batch_alloc(mmh_ctx* c, BatchGroup& g) {
     g.rhs[l] = l == 0 ? nullptr : wsT<double>(c, tl + "rhs", B * cube(m));
     g.r[l] = wsT<double>(c, tl + "r", B * cube(m));
     g.G[l] = wsT<double>(c, tl + "G", B * 6 * cube(m));
+    g.A[l] = (l > 0 && c->galerkin) ? wsT<double>(c, tl + "A", B * 27 * cube(m)) : nullptr;
   }
   const i64 n3 = cube(g.n);
   g.x = wsT<double>(c, t + "x", B * n3);
@@ -248,6 +274,17 @@ double batch_setup_item(mmh_ctx* c, BatchGroup& g, int s, double* u_user, double
     k_metric<double, 1><<<nblk(cube(m)), 256, 0, c->stream>>>(g.G[l] + (i64)s * 6 * cube(m), dl, wl, nul, m, g.dx[l], 0);
     LAUNCHCHK();
   }
+  // Galerkin operators of this grid (build_galerkin's kernels on the grid's slices)
+  for (int l = 1; l < g.nlev; ++l) {
+    if (!g.A[l]) break;
+    const int m = g.m[l];
+    double* Al = g.A[l] + (i64)s * 27 * cube(m);
+    if (l == 1)
+      k_rap19<double><<<nblk(cube(m)), 256, 0, c->stream>>>(Al, g.G[0] + (i64)s * 6 * n3, n, 1.0 / (g.dx[0] * g.dx[0]));
+    else
+      k_rap27<<<nblk(cube(m)), 256, 0, c->stream>>>(Al, g.A[l - 1] + (i64)s * 27 * cube(g.m[l - 1]), g.m[l - 1]);
+    LAUNCHCHK();
+  }
   double* sums = wsT<double>(c, g.tag + "setup_sums", 8);
   rgzerosum_dev<double>(c, rhs_user, def, 1.0, n, sums);
   reduce<double>(c, rhs_user, n3, 1, sums + 3);
@@ -266,7 +303,7 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
   require(nu >= 1, "solve_batch: nu >= 1");
   Sched S;
   S.kind = 1; S.nu_pre = o->nu_pre; S.nu_post = o->nu_post; S.nu_coarse = o->nu_coarse;
-  S.guard = false; S.symmetric = true;
+  S.guard = false; S.symmetric = true; S.nu_top = c->nu_top;
   require(S.nu_pre >= 1 && S.nu_coarse >= 1, "solve_batch: sweep counts >= 1");
   // Graph capture needs a stream other than the null stream: the batch runs on its own stream.
   StreamSwap swap_guard(c);
@@ -282,6 +319,7 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     g.B = (int)kv.second.size();
     g.items = kv.second;
     batch_alloc(c, g);
+    S.ng = g.n;  // the group's finest level (nu_top applies there)
     const i64 n3 = cube(g.n);
     std::vector<double> rden(g.B);
     std::vector<int> act(g.B);

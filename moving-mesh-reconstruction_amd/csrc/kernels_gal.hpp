@@ -197,10 +197,15 @@ __device__ __forceinline__ T gal_resid_point(const T* __restrict__ phi, const T*
   return rhs[p] - (gal_offsum<T>(phi, A, n, i, j, k, p) + A[13 * n3 + p] * phi[p]);
 }
 
+// Batched launches: gridDim.y = B grids stored one after another (bstride = n^3 values; A holds
+// 27 n^3 per grid).
 template <class T>
 __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
-                                                    int n, int kpar, int io, int jo) {
+                                                    int n, int kpar, int io, int jo, i64 bstride = 0) {
   const int h = n >> 1;
+  phi += (i64)blockIdx.y * bstride;
+  rhs += (i64)blockIdx.y * bstride;
+  A += (i64)blockIdx.y * 27 * bstride;
   const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
   const i64 q = (i64)blk * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
@@ -210,7 +215,11 @@ __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T
 // r = rhs - A phi at every point (one thread per point, sub-lattice order); r stored.
 template <class T>
 __global__ __launch_bounds__(256) void k_gal_resid(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
-                                                   const T* __restrict__ A, int n) {
+                                                   const T* __restrict__ A, int n, i64 bstride = 0) {
+  r += (i64)blockIdx.y * bstride;
+  phi += (i64)blockIdx.y * bstride;
+  rhs += (i64)blockIdx.y * bstride;
+  A += (i64)blockIdx.y * 27 * bstride;
   const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (i64)n * n * n) return;
   int i, j, k;

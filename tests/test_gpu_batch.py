@@ -41,20 +41,26 @@ def probs(ctx):
     return [problem(ctx, *c) for c in CASES]
 
 
-def test_batch_bit_identical_to_single_solves(ctx, probs):
-    """The batch runs the reference's rebuilt coarse operators; single solves do so with option
-    galerkin = 0."""
-    print("case batch vs single: %s" % (CASES,), flush=True)
-    xs, st, ok = run_batch(ctx, probs)
-    assert ok
-    for (u, r, d), x, s, case in zip(probs, xs, st, CASES):
-        phi = torch.zeros_like(r)
-        ctx.set_option("galerkin", 0)
-        try:
+@pytest.mark.parametrize("galerkin,nu_top", [(1, 0), (1, 1), (0, 0)])
+def test_batch_bit_identical_to_single_solves(ctx, probs, galerkin, nu_top):
+    """With Galerkin or rebuilt coarse operators, and with one sweep on the finest level or the
+    common count, every realisation of the batch equals its single solve."""
+    print("case batch vs single: galerkin %d nu_top %d %s" % (galerkin, nu_top, CASES), flush=True)
+    ctx.set_option("galerkin", galerkin)
+    ctx.set_option("nu_top", nu_top)
+    try:
+        xs, st, ok = run_batch(ctx, probs)
+        singles = []
+        for (u, r, d) in probs:
+            phi = torch.zeros_like(r)
             st1, ok1 = ctx.solve(phi, r.clone(), d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
                                  nu_coarse=8, nu=5, krylov=True)
-        finally:
-            ctx.set_option("galerkin", 1)
+            singles.append((phi, st1, ok1))
+    finally:
+        ctx.set_option("galerkin", 1)
+        ctx.set_option("nu_top", 0)
+    assert ok
+    for (phi, st1, ok1), x, s, case in zip(singles, xs, st, CASES):
         assert ok1 and s.converged
         assert s.cycles == st1.cycles, (case, s.cycles, st1.cycles)
         assert s.relres == st1.relres[st1.cycles - 1]
