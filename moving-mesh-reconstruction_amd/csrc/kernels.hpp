@@ -387,17 +387,27 @@ __global__ __launch_bounds__(RB) void k_dot(double* __restrict__ bsum, const T* 
   if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
 }
 
-// Per-block sums of z*r and z*ro in FP64 (z widened exactly from its storage type), with the
-// blocking of k_dot: the two dot products of one flexible-CG iteration in one pass over z.
+// z with its mean removed as k_zerosum_guard (mode 0) would store it, when zsum (the FP64 sum of
+// z) is given; z itself otherwise.
+template <class TZ> __device__ __forceinline__ TZ z_centred(const TZ* __restrict__ z, i64 p, const double* zsum, i64 n) {
+  if (!zsum) return z[p];
+  const double m = zsum[0] / (double)n;
+  return (TZ)((double)z[p] - m);
+}
+
+// Per-block sums of z*r and z*ro in FP64 (z widened exactly from its storage type, its mean removed
+// on the fly when zsum is given), with the blocking of k_dot: the two dot products of one
+// flexible-CG iteration in one pass over z.
 template <class TZ>
 __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __restrict__ b2, const TZ* __restrict__ z,
-                                             const double* __restrict__ r, const double* __restrict__ ro, i64 n) {
+                                             const double* __restrict__ r, const double* __restrict__ ro, i64 n,
+                                             const double* __restrict__ zsum = nullptr) {
   __shared__ double s1[RB], s2[RB];
   const int t = threadIdx.x;
   const i64 p = (i64)blockIdx.x * RB + t;
   double a1 = 0.0, a2 = 0.0;
   if (p < n) {
-    const double zz = (double)z[p];
+    const double zz = (double)z_centred<TZ>(z, p, zsum, n);
     a1 = zz * r[p];
     a2 = zz * ro[p];
   }
@@ -444,10 +454,14 @@ __global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ x, double
 }
 
 // p = z + beta p.
-template <class TZ> __global__ void k_cg_dir(double* __restrict__ p, const TZ* __restrict__ z, double beta, i64 n) {
+// p = z + beta p (z centred on the fly when zsum is given); with accumulate = 0, p = z.
+template <class TZ>
+__global__ void k_cg_dir(double* __restrict__ p, const TZ* __restrict__ z, double beta, i64 n,
+                         const double* __restrict__ zsum = nullptr, int accumulate = 1) {
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  p[i] = (double)z[i] + beta * p[i];
+  const double zz = (double)z_centred<TZ>(z, i, zsum, n);
+  p[i] = accumulate ? zz + beta * p[i] : zz;
 }
 
 // CG updates in the working precision T with FP64 scalars (the FP32 Krylov solver).

@@ -234,6 +234,7 @@ struct Sched {
   bool guard = true;       // the reference's divergence guard on coarse levels
   bool symmetric = false;  // post-smoothing in reverse colour order (symmetric V-cycle)
   bool delta = false;      // difference form of the FP32 stencil (mixed preconditioner only)
+  bool defer_top_mean = false;  // finest level: sum only, its consumers remove the mean on the fly
   int nu_top = 0;  // kind 1: sweeps each way on the finest level when > 0 (option "nu_top")
   int pre(int n) const { return kind == 0 ? 4 * ng / n : (nu_top > 0 && n == ng ? nu_top : nu_pre); }
   int post(int n) const { return kind == 0 ? 4 * ng / n : (nu_top > 0 && n == ng ? nu_top : nu_post); }
@@ -566,6 +567,7 @@ template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const 
   if (H.A[l]) gal_sweeps<T>(c, H.u[l], H.rhs[l], H.A[l], n, S.post(n), S.symmetric);
   else relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.post(n), S.guard, (T*)nullptr, sc + 1, S.symmetric);
   reduce<T>(c, H.u[l], n3, 0, sc + 2);
+  if (S.defer_top_mean && l == 0) return;
   k_zerosum_guard<T><<<nblk(n3), 256, 0, c->stream>>>(H.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
                                                       H.guard + l);
   LAUNCHCHK();
@@ -1066,6 +1068,7 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   else relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1,
                  S.symmetric, S.delta);
   reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
+  if (S.defer_top_mean && l == 0) return;
   k_zerosum_guard<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
                                                           Hf.guard + l);
   LAUNCHCHK();
@@ -1138,13 +1141,19 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   Hier<double> P = H;
   P.u[0] = z;
   P.rhs[0] = r;
+  // The V-cycle's last step, removing the mean of z, is left to z's readers (the dot products and
+  // the direction update subtract it as they read, with the expression of k_zerosum_guard), which
+  // saves a pass over z; not where the finest level runs inside the one-workgroup coarse cycle.
+  Sched Sz = S;
+  Sz.defer_top_mean = n > 4 && !(F ? coarse_cycle_applies(c, *F, 0, S) : coarse_cycle_applies(c, H, 0, S));
+  const double* zsum = Sz.defer_top_mean ? (F ? F->scal + 2 : H.scal + 2) : nullptr;
   auto precond_work = [&] {
     if (!F) {
       fill<double>(c, z, 0.0, n3);
-      vcycle<double>(c, P, 0, n, S);
+      vcycle<double>(c, P, 0, n, Sz);
     } else {
       fill<float>(c, z32, 0.f, n3);
-      vcycle_mixed(c, *F, H, 0, n, S);
+      vcycle_mixed(c, *F, H, 0, n, Sz);
     }
   };
   auto precond = [&] {
@@ -1157,8 +1166,8 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   };
   // z.r and z.ro of the current z (deterministic block sums, one read-back for both)
   auto dots = [&](const double* rr, const double* rro, double& zr, double& zro) {
-    if (F) k_dot2<float><<<nb, RB, 0, c->stream>>>(bs, bs2, z32, rr, rro, n3);
-    else k_dot2<double><<<nb, RB, 0, c->stream>>>(bs, bs2, z, rr, rro, n3);
+    if (F) k_dot2<float><<<nb, RB, 0, c->stream>>>(bs, bs2, z32, rr, rro, n3, zsum);
+    else k_dot2<double><<<nb, RB, 0, c->stream>>>(bs, bs2, z, rr, rro, n3, zsum);
     LAUNCHCHK();
     reduce<double>(c, bs, nb, 0, sums + 8);
     reduce<double>(c, bs2, nb, 0, sums + 9);
@@ -1173,8 +1182,8 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   precond();
   double rz, unused;
   dots(r, r, rz, unused);
-  if (F) k_convert<float, double><<<nblk(n3), 256, 0, c->stream>>>(p, z32, n3);
-  else HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(p, z32, 0.0, n3, zsum, 0);
+  else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(p, z, 0.0, n3, zsum, 0);
   LAUNCHCHK();
   for (int it = 0; it < o->max_cycles; ++it) {
     const unsigned nq = op_apply<double>(c, q, bs, p, H.G[0], n, 1.0);
@@ -1192,8 +1201,8 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
     double rz_new, zro;
     dots(r, ro, rz_new, zro);
     const double beta = (rz_new - zro) / rz;
-    if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(p, z32, beta, n3);
-    else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(p, z, beta, n3);
+    if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(p, z32, beta, n3, zsum);
+    else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(p, z, beta, n3, zsum);
     LAUNCHCHK();
     rz = rz_new;
   }
