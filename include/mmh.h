@@ -99,7 +99,8 @@ int mmh_destroy(mmh_ctx* ctx);
 const char* mmh_error_string(mmh_ctx* ctx);
 int mmh_set_stream(mmh_ctx* ctx, void* hip_stream); /* NULL = the null stream (default) */
 int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffers */
-/* Options (results never depend on them; all are bit-identical):
+/* Options.  All are bit-identical except "mixed_delta", "galerkin" and "nu_top", which change the
+ * preconditioner of the Krylov solve (its iterates and iteration count, not the tolerance reached):
  *   "sweep_impl"  mmh_time_sweeps: 0 = the solver's smoother (sub-lattice layout; the wavefront
  *                 kernel where "wave" is on), 1 = colour kernels on column-major arrays,
  *                 2 = colour kernels on sub-lattice arrays
