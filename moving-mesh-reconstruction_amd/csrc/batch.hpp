@@ -158,7 +158,7 @@ void batch_iteration(mmh_ctx* c, BatchGroup& g, const Sched& S, double tol) {
   batch_reduce(c, g, g.bs, ns, 0, g.pq);
   k_cg_scalars<<<sb, 64, 0, c->stream>>>(0, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
                                          g.active, g.iters, g.relres);
-  k_cg_update_b<<<dim3(nb, g.B), RB, 0, c->stream>>>(g.x, g.cr, g.ro, g.p, g.q, g.alpha, g.bm, n3);
+  k_cg_update_b<<<dim3(nb, g.B), RB, 0, c->stream>>>(g.cr, g.ro, g.q, g.alpha, g.bm, n3);
   LAUNCHCHK();
   batch_reduce(c, g, g.bm, nb, 1, g.rmax);
   k_cg_scalars<<<sb, 64, 0, c->stream>>>(1, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
@@ -168,7 +168,7 @@ void batch_iteration(mmh_ctx* c, BatchGroup& g, const Sched& S, double tol) {
   batch_dot(c, g, g.z, g.ro, g.zro);
   k_cg_scalars<<<sb, 64, 0, c->stream>>>(2, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
                                          g.active, g.iters, g.relres);
-  k_cg_dir_b<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.p, g.z, g.beta, n3);
+  k_cg_dir_b<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.p, g.z, g.alpha, g.beta, n3);
   LAUNCHCHK();
 }
 

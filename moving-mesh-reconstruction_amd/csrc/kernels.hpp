@@ -427,16 +427,16 @@ __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __
   }
 }
 
-// x += alpha p; r -= alpha q (the previous r kept in rold); per-block max|r|.  With r32, the new r
-// is also stored rounded to FP32 (the right-hand side of an FP32 preconditioner).
-__global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ x, double* __restrict__ r, double* __restrict__ rold,
-                                                  const double* __restrict__ p, const double* __restrict__ q, double alpha,
-                                                  double* __restrict__ bmax, i64 n, float* __restrict__ r32 = nullptr) {
+// r -= alpha q (the previous r kept in rold); per-block max|r|.  With r32, the new r is also
+// stored rounded to FP32 (the right-hand side of an FP32 preconditioner).  x is advanced by the
+// direction update (k_cg_dir), one pass later.
+__global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ r, double* __restrict__ rold,
+                                                  const double* __restrict__ q, double alpha, double* __restrict__ bmax,
+                                                  i64 n, float* __restrict__ r32 = nullptr) {
   __shared__ double s[RB];
   const i64 i = (i64)blockIdx.x * RB + threadIdx.x;
   double a = 0.0;
   if (i < n) {
-    x[i] = x[i] + alpha * p[i];
     const double ro = r[i];
     rold[i] = ro;
     const double rn = ro - alpha * q[i];
@@ -453,15 +453,24 @@ __global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ x, double
   if (threadIdx.x == 0) bmax[blockIdx.x] = s[0];
 }
 
+// x += alpha p (the last step's update of the iterate).
+__global__ void k_axpy(double* __restrict__ x, const double* __restrict__ p, double alpha, i64 n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] + alpha * p[i];
+}
+
 // p = z + beta p.
-// p = z + beta p (z centred on the fly when zsum is given); with accumulate = 0, p = z.
+// x += alpha p (with x), then p = z + beta p (z centred on the fly when zsum is given); with
+// accumulate = 0, p = z.  One pass for the iterate's update and the new direction.
 template <class TZ>
-__global__ void k_cg_dir(double* __restrict__ p, const TZ* __restrict__ z, double beta, i64 n,
-                         const double* __restrict__ zsum = nullptr, int accumulate = 1) {
+__global__ void k_cg_dir(double* __restrict__ x, double* __restrict__ p, const TZ* __restrict__ z, double alpha,
+                         double beta, i64 n, const double* __restrict__ zsum = nullptr, int accumulate = 1) {
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double zz = (double)z_centred<TZ>(z, i, zsum, n);
-  p[i] = accumulate ? zz + beta * p[i] : zz;
+  const double pi = p[i];
+  if (x) x[i] = x[i] + alpha * pi;
+  p[i] = accumulate ? zz + beta * pi : zz;
 }
 
 // CG updates in the working precision T with FP64 scalars (the FP32 Krylov solver).
@@ -497,17 +506,16 @@ template <class T> __global__ void k_cg_dir_t(T* __restrict__ p, const T* __rest
 }
 
 // Batched CG updates: grid y uses alpha[y] / beta[y] (0 once it has converged, which freezes x and r).
-__global__ __launch_bounds__(RB) void k_cg_update_b(double* __restrict__ x, double* __restrict__ r, double* __restrict__ rold,
-                                                    const double* __restrict__ p, const double* __restrict__ q,
-                                                    const double* __restrict__ alpha, double* __restrict__ bmax, i64 n) {
+__global__ __launch_bounds__(RB) void k_cg_update_b(double* __restrict__ r, double* __restrict__ rold,
+                                                    const double* __restrict__ q, const double* __restrict__ alpha,
+                                                    double* __restrict__ bmax, i64 n) {
   __shared__ double s[RB];
   const i64 y = blockIdx.y;
-  x += y * n; r += y * n; rold += y * n; p += y * n; q += y * n;
+  r += y * n; rold += y * n; q += y * n;
   const double al = alpha[y];
   const i64 i = (i64)blockIdx.x * RB + threadIdx.x;
   double a = 0.0;
   if (i < n) {
-    x[i] = x[i] + al * p[i];
     const double ro = r[i];
     rold[i] = ro;
     const double rn = ro - al * q[i];
@@ -523,11 +531,15 @@ __global__ __launch_bounds__(RB) void k_cg_update_b(double* __restrict__ x, doub
   if (threadIdx.x == 0) bmax[y * gridDim.x + blockIdx.x] = s[0];
 }
 
-__global__ void k_cg_dir_b(double* __restrict__ p, const double* __restrict__ z, const double* __restrict__ beta, i64 n) {
+// x += alpha p, p = z + beta p per grid (alpha = beta = 0 freezes a converged grid).
+__global__ void k_cg_dir_b(double* __restrict__ x, double* __restrict__ p, const double* __restrict__ z,
+                           const double* __restrict__ alpha, const double* __restrict__ beta, i64 n) {
   const i64 y = blockIdx.y;
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  p[y * n + i] = z[y * n + i] + beta[y] * p[y * n + i];
+  const double pi = p[y * n + i];
+  x[y * n + i] = x[y * n + i] + alpha[y] * pi;
+  p[y * n + i] = z[y * n + i] + beta[y] * pi;
 }
 
 // Per-grid CG scalars, one thread per grid.  step 0: alpha = rz / pq for active grids, else 0.

@@ -1182,27 +1182,32 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   precond();
   double rz, unused;
   dots(r, r, rz, unused);
-  if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(p, z32, 0.0, n3, zsum, 0);
-  else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(p, z, 0.0, n3, zsum, 0);
+  if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(nullptr, p, z32, 0.0, 0.0, n3, zsum, 0);
+  else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(nullptr, p, z, 0.0, 0.0, n3, zsum, 0);
   LAUNCHCHK();
   for (int it = 0; it < o->max_cycles; ++it) {
     const unsigned nq = op_apply<double>(c, q, bs, p, H.G[0], n, 1.0);
     reduce<double>(c, bs, nq, 0, sums + 5);
     const double pq = read_scalar<double>(c, sums + 5);
     const double alpha = rz / pq;
-    k_cg_update<<<nb, RB, 0, c->stream>>>(x, r, ro, p, q, alpha, bm, n3, r32);
+    k_cg_update<<<nb, RB, 0, c->stream>>>(r, ro, q, alpha, bm, n3, r32);
     LAUNCHCHK();
     reduce<double>(c, bm, nb, 1, sums + 6);
     const double rr = read_scalar<double>(c, sums + 6) / rden;
     if (st) { st->relres[it] = rr; st->cycles = it + 1; }
-    if (rr <= o->tol) break;
-    if (it + 1 == o->max_cycles) throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
+    if (rr <= o->tol || it + 1 == o->max_cycles) {
+      // the iterate's last update (the others ride on the direction update)
+      k_axpy<<<nblk(n3), 256, 0, c->stream>>>(x, p, alpha, n3);
+      LAUNCHCHK();
+      if (rr <= o->tol) break;
+      throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
+    }
     precond();
     double rz_new, zro;
     dots(r, ro, rz_new, zro);
     const double beta = (rz_new - zro) / rz;
-    if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(p, z32, beta, n3, zsum);
-    else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(p, z, beta, n3, zsum);
+    if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(x, p, z32, alpha, beta, n3, zsum);
+    else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(x, p, z, alpha, beta, n3, zsum);
     LAUNCHCHK();
     rz = rz_new;
   }
