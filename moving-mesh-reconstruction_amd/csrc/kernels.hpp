@@ -16,6 +16,7 @@ typedef int64_t i64;
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 __device__ __forceinline__ i64 ix3(int i, int j, int k, int n) { return (i64)i + (i64)n * ((i64)j + (i64)n * (i64)k); }
 template <class T> __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
+
 template <class T> __device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
 
 // Workgroups are dealt to the 8 XCDs round-robin (id mod 8), each XCD with its own L2.  Remapping
@@ -28,6 +29,49 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
 
 constexpr int RB = 256;  // reduction block
 constexpr int RPT = 8;   // elements per thread per reduction block
+
+// The halving tree s[t] = op(s[t], s[t + h]), h = NT/2 .. 1, over the values a of the NT threads
+// of a workgroup (op 0: sum, op 1: maximum); the result is returned in thread 0.  Levels h >= 64
+// pair values of different wavefronts through s[NT] in shared memory, one barrier each; levels
+// h < 64 pair lanes t and t + h of wavefront 0 with __shfl_down, the same operands in the same
+// order as in shared memory, so the result is bit-identical to the tree run wholly in s.
+template <int NT, int OP> __device__ __forceinline__ double tree_reduce(double a, double* s) {
+  static_assert(NT >= 128 && (NT & (NT - 1)) == 0, "power of two of at least two wavefronts");
+  const int t = threadIdx.x;
+  s[t] = a;
+  __syncthreads();
+#pragma unroll
+  for (int h = NT / 2; h > 64; h >>= 1) {
+    if (t < h) s[t] = OP == 0 ? s[t] + s[t + h] : dmax(s[t], s[t + h]);
+    __syncthreads();
+  }
+  if (t < 64) {
+    a = OP == 0 ? s[t] + s[t + 64] : dmax(s[t], s[t + 64]);
+#pragma unroll
+    for (int h = 32; h > 0; h >>= 1) {
+      const double o = __shfl_down(a, h);
+      a = OP == 0 ? a + o : dmax(a, o);
+    }
+  }
+  return a;
+}
+
+// Maximum of a over the RB threads of a workgroup, stored by thread 0 to *out: a butterfly over
+// the 64 lanes of each wavefront, then one barrier and the RB/64 wavefront values.  A maximum does
+// not depend on the order it is taken in, so this equals the halving tree over shared memory.
+__device__ __forceinline__ void block_max_store(double a, double* out) {
+  __shared__ double w[RB / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) a = dmax(a, __shfl_xor(a, off));
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = w[0];
+#pragma unroll
+    for (int v = 1; v < RB / 64; ++v) m = dmax(m, w[v]);
+    *out = m;
+  }
+}
 
 // ------------------------------------------------------------------------------------------------
 // Deterministic FP64 reductions.  Block b covers elements [b*RB*RPT, (b+1)*RB*RPT); thread t adds
@@ -53,13 +97,8 @@ __global__ __launch_bounds__(RB) void k_reduce_stage(double* __restrict__ out, c
       else a = dmax(a, fabs(v));
     }
   }
-  s[t] = a;
-  __syncthreads();
-  for (int h = RB / 2; h > 0; h >>= 1) {
-    if (t < h) s[t] = OP == 0 ? s[t] + s[t + h] : dmax(s[t], s[t + h]);
-    __syncthreads();
-  }
-  if (t == 0) out[blockIdx.x] = s[0];
+  a = tree_reduce<RB, OP>(a, s);
+  if (t == 0) out[blockIdx.x] = a;
 }
 
 template <class T> __global__ void k_fill(T* __restrict__ a, T v, i64 n, i64 bstride = 0) {
@@ -275,7 +314,6 @@ template <class T, int L, bool DELTA = false>
 __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restrict__ bmax, const T* __restrict__ phi,
                                               const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2,
                                               int batched = 0) {
-  __shared__ double s[RB];
   const i64 n3 = (i64)n * n * n;
   if (batched) {
     const i64 y = blockIdx.y;
@@ -309,13 +347,7 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
     if (r) r[p] = rv;
     a = (double)fabs(rv);
   }
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int h = RB / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + h]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bmax[tile] = s[0];
+  block_max_store(a, bmax + tile);
 }
 
 // q = L[p] (the operator of k_resid: (t1 + t2 + tdiag*p)/dx^2) and per-block sums of p*q in FP64,
@@ -360,13 +392,8 @@ __global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __r
     q[c] = v;
     a = (double)p[c] * (double)v;
   }
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int h = RB / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bsum[tile] = s[0];
+  a = tree_reduce<RB, 0>(a, s);
+  if (threadIdx.x == 0) bsum[tile] = a;
 }
 
 // Per-block sums of a*b (FP64), same blocking as k_apply_dot.
@@ -378,13 +405,8 @@ __global__ __launch_bounds__(RB) void k_dot(double* __restrict__ bsum, const T* 
   b += (i64)blockIdx.y * bstride;
   bsum += (i64)blockIdx.y * gridDim.x;
   const i64 p = (i64)blockIdx.x * RB + threadIdx.x;
-  s[threadIdx.x] = p < n ? (double)a[p] * (double)b[p] : 0.0;
-  __syncthreads();
-  for (int h = RB / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bsum[blockIdx.x] = s[0];
+  const double v = tree_reduce<RB, 0>(p < n ? (double)a[p] * (double)b[p] : 0.0, s);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = v;
 }
 
 // z with its mean removed as k_zerosum_guard (mode 0) would store it, when zsum (the FP64 sum of
@@ -411,19 +433,11 @@ __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __
     a1 = zz * r[p];
     a2 = zz * ro[p];
   }
-  s1[t] = a1;
-  s2[t] = a2;
-  __syncthreads();
-  for (int h = RB / 2; h > 0; h >>= 1) {
-    if (t < h) {
-      s1[t] = s1[t] + s1[t + h];
-      s2[t] = s2[t] + s2[t + h];
-    }
-    __syncthreads();
-  }
+  a1 = tree_reduce<RB, 0>(a1, s1);
+  a2 = tree_reduce<RB, 0>(a2, s2);
   if (t == 0) {
-    b1[blockIdx.x] = s1[0];
-    b2[blockIdx.x] = s2[0];
+    b1[blockIdx.x] = a1;
+    b2[blockIdx.x] = a2;
   }
 }
 
@@ -433,7 +447,6 @@ __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __
 __global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ r, double* __restrict__ rold,
                                                   const double* __restrict__ q, double alpha, double* __restrict__ bmax,
                                                   i64 n, float* __restrict__ r32 = nullptr) {
-  __shared__ double s[RB];
   const i64 i = (i64)blockIdx.x * RB + threadIdx.x;
   double a = 0.0;
   if (i < n) {
@@ -444,13 +457,7 @@ __global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ r, double
     if (r32) r32[i] = (float)rn;
     a = fabs(rn);
   }
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int h = RB / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + h]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bmax[blockIdx.x] = s[0];
+  block_max_store(a, bmax + blockIdx.x);
 }
 
 // x += alpha p (the last step's update of the iterate).
@@ -478,7 +485,6 @@ template <class T>
 __global__ __launch_bounds__(RB) void k_cg_update_t(T* __restrict__ x, T* __restrict__ r, T* __restrict__ rold,
                                                     const T* __restrict__ p, const T* __restrict__ q, double alpha,
                                                     double* __restrict__ bmax, i64 n) {
-  __shared__ double s[RB];
   const i64 i = (i64)blockIdx.x * RB + threadIdx.x;
   const T al = (T)alpha;
   double a = 0.0;
@@ -490,13 +496,7 @@ __global__ __launch_bounds__(RB) void k_cg_update_t(T* __restrict__ x, T* __rest
     r[i] = rn;
     a = fabs((double)rn);
   }
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int h = RB / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + h]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bmax[blockIdx.x] = s[0];
+  block_max_store(a, bmax + blockIdx.x);
 }
 
 template <class T> __global__ void k_cg_dir_t(T* __restrict__ p, const T* __restrict__ z, double beta, i64 n) {
@@ -509,7 +509,6 @@ template <class T> __global__ void k_cg_dir_t(T* __restrict__ p, const T* __rest
 __global__ __launch_bounds__(RB) void k_cg_update_b(double* __restrict__ r, double* __restrict__ rold,
                                                     const double* __restrict__ q, const double* __restrict__ alpha,
                                                     double* __restrict__ bmax, i64 n) {
-  __shared__ double s[RB];
   const i64 y = blockIdx.y;
   r += y * n; rold += y * n; q += y * n;
   const double al = alpha[y];
@@ -522,13 +521,7 @@ __global__ __launch_bounds__(RB) void k_cg_update_b(double* __restrict__ r, doub
     r[i] = rn;
     a = fabs(rn);
   }
-  s[threadIdx.x] = a;
-  __syncthreads();
-  for (int h = RB / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) s[threadIdx.x] = dmax(s[threadIdx.x], s[threadIdx.x + h]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bmax[y * gridDim.x + blockIdx.x] = s[0];
+  block_max_store(a, bmax + y * gridDim.x + blockIdx.x);
 }
 
 // x += alpha p, p = z + beta p per grid (alpha = beta = 0 freezes a converged grid).

@@ -217,14 +217,8 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
     g5m = g5c; g5c = g5p; g5p = g5n;
     g6m = g6c; g6c = g6p; g6p = g6n;
   }
-  __syncthreads();
-  red[tid] = acc;
-  __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (tid < s) red[tid] = MODE == 0 ? dmax(red[tid], red[tid + s]) : red[tid] + red[tid + s];
-    __syncthreads();
-  }
-  if (tid == 0) bres[b] = red[0];
+  acc = tree_reduce<NT, MODE == 0 ? 1 : 0>(acc, red);
+  if (tid == 0) bres[b] = acc;
 }
 
 }  // namespace mmh
