@@ -62,29 +62,41 @@ std::pair<int, int> batch_shape(mmh_ctx* c, int m, int B) {
 }
 
 void batch_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, const double* G, int l, int ns, bool reverse,
-                  std::pair<int, int> shape) {
+                  std::pair<int, int> shape, bool from_zero = false) {
   const int m = g.m[l];
   const i64 h3 = cube(m / 2);
   const double dx2 = g.dx[l] * g.dx[l];
   const int bsz = shape.first;
   const dim3 grid(nblk(h3, bsz), g.B);
   for (int sw = 0; sw < ns; ++sw)
-    for (int cc = 0; cc < 8; ++cc) {
+    for (int cc = 0, done = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
-      k_colour<double, 1><<<grid, bsz, 0, c->stream>>>(phi, rhs, G, m, dx2, col >> 2, (col >> 1) & 1, col & 1, cube(m),
-                                                       shape.second);
+      if (sw == 0 && from_zero)
+        k_colour<double, 1, false, true><<<grid, bsz, 0, c->stream>>>(phi, rhs, G, m, dx2, col >> 2, (col >> 1) & 1,
+                                                                      col & 1, cube(m), shape.second, (unsigned)done);
+      else
+        k_colour<double, 1><<<grid, bsz, 0, c->stream>>>(phi, rhs, G, m, dx2, col >> 2, (col >> 1) & 1, col & 1, cube(m),
+                                                         shape.second);
+      done |= 1 << col;
     }
   LAUNCHCHK();
 }
 
 // Sweeps of a batched Galerkin level (gal_sweeps for the stack).
-void batch_gal_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, int l, int ns, bool reverse) {
+void batch_gal_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, int l, int ns, bool reverse,
+                      bool from_zero = false) {
   const int m = g.m[l];
   const dim3 grid(nblk(cube(m / 2)), g.B);
   for (int sw = 0; sw < ns; ++sw)
-    for (int cc = 0; cc < 8; ++cc) {
+    for (int cc = 0, done = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
-      k_gal_colour<double><<<grid, 256, 0, c->stream>>>(phi, rhs, g.A[l], m, col >> 2, (col >> 1) & 1, col & 1, cube(m));
+      if (sw == 0 && from_zero)
+        k_gal_colour<double, true><<<grid, 256, 0, c->stream>>>(phi, rhs, g.A[l], m, col >> 2, (col >> 1) & 1, col & 1,
+                                                                cube(m), (unsigned)done);
+      else
+        k_gal_colour<double><<<grid, 256, 0, c->stream>>>(phi, rhs, g.A[l], m, col >> 2, (col >> 1) & 1, col & 1,
+                                                          cube(m));
+      done |= 1 << col;
     }
   LAUNCHCHK();
 }
@@ -99,10 +111,10 @@ void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, co
   const auto shape = batch_shape(c, m, g.B);
   if (m == 4) {
     if (g.A[l]) {
-      batch_gal_sweeps(c, g, u, rh, l, S.coarse(), false);
+      batch_gal_sweeps(c, g, u, rh, l, S.coarse(), false, c->zero_start);
       batch_gal_sweeps(c, g, u, rh, l, S.coarse(), S.symmetric);
     } else {
-      batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), false, shape);
+      batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), false, shape, c->zero_start);
       batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), S.symmetric, shape);
     }
     batch_reduce(c, g, u, m3, 0, g.sums);
@@ -112,11 +124,11 @@ void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, co
   }
   const int pre = S.pre(m);
   if (g.A[l]) {
-    batch_gal_sweeps(c, g, u, rh, l, pre, false);
+    batch_gal_sweeps(c, g, u, rh, l, pre, false, c->zero_start);
     k_gal_resid<double><<<dim3(nblk(m3), g.B), 256, 0, c->stream>>>(g.r[l], u, rh, g.A[l], m, m3);
     LAUNCHCHK();
   } else {
-    batch_sweeps(c, g, u, rh, g.G[l], l, pre, false, shape);
+    batch_sweeps(c, g, u, rh, g.G[l], l, pre, false, shape, c->zero_start);
     op_resid<double>(c, g.r[l], g.bm, u, rh, g.G[l], m, 1.0 / (g.dx[l] * g.dx[l]), false, g.B, 1);
   }
   const int nrmax = std::min(4, std::max(1, m / 4));

@@ -219,9 +219,13 @@ __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __re
 // layout L.  DELTA = true (the FP32 preconditioner of the mixed solve only) returns in t1 the face
 // sum over (phi_nb - phi_c) instead of over phi_nb, so that t1 + tdiag phi_c is formed from
 // differences: in FP32 the update then loses precision to |grad phi| rather than to |phi|.
-template <class T, int L, bool DELTA = false>
+// MASK = true (with DELTA = false): bit b of live says whether colour class b (4 (k&1) + 2 (i&1) +
+// (j&1)) of phi may be non-zero; values of the other classes are taken as 0 without being loaded,
+// and so are the G2/G3/G5 factors of cross terms whose phi difference is then 0.  The result has
+// the value the unmasked stencil gives when those classes hold zeros (a zero may differ in sign).
+template <class T, int L, bool DELTA = false, bool MASK = false>
 __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __restrict__ G, i64 n3, int n, int i, int j,
-                                          int k, T& t1, T& t2, T& tdiag) {
+                                          int k, T& t1, T& t2, T& tdiag, unsigned live = 0xFFu) {
   const T* G1 = G;
   const T* G2 = G + n3;
   const T* G3 = G + 2 * n3;
@@ -239,6 +243,29 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __
   T b33p = (G6[X_(i, j, kp)] + G6[c]) / T(2);
   T b33m = (G6[X_(i, j, km)] + G6[c]) / T(2);
   tdiag = -(b11p + b11m + b22p + b22m + b33p + b33m);
+  if constexpr (MASK) {
+    static_assert(!DELTA, "the masked stencil is the plain form");
+    const unsigned cc = 4u * (k & 1) + 2u * (i & 1) + (j & 1);
+    const bool fx = (live >> (cc ^ 2u)) & 1u, fy = (live >> (cc ^ 1u)) & 1u, fz = (live >> (cc ^ 4u)) & 1u;
+    const bool fxy = (live >> (cc ^ 3u)) & 1u, fyz = (live >> (cc ^ 5u)) & 1u, fxz = (live >> (cc ^ 6u)) & 1u;
+    auto P = [&](bool on, i64 a) { return on ? phi[a] : T(0); };
+    auto F = [&](bool on, const T* g, i64 a) { return on ? g[a] : T(0); };
+    t1 = (b11p * P(fx, X_(ip, j, k)) + b11m * P(fx, X_(im, j, k)) + b22p * P(fy, X_(i, jp, k)) +
+          b22m * P(fy, X_(i, jm, k)) + b33p * P(fz, X_(i, j, kp)) + b33m * P(fz, X_(i, j, km)));
+    t2 = ((P(fxy, X_(ip, jp, k)) - P(fxy, X_(im, jp, k))) * F(fxy, G2, X_(i, jp, k))
+        - (P(fxy, X_(ip, jm, k)) - P(fxy, X_(im, jm, k))) * F(fxy, G2, X_(i, jm, k))
+        + (P(fxy, X_(ip, jp, k)) - P(fxy, X_(ip, jm, k))) * F(fxy, G2, X_(ip, j, k))
+        - (P(fxy, X_(im, jp, k)) - P(fxy, X_(im, jm, k))) * F(fxy, G2, X_(im, j, k))
+        + (P(fyz, X_(i, jp, kp)) - P(fyz, X_(i, jm, kp))) * F(fyz, G5, X_(i, j, kp))
+        - (P(fyz, X_(i, jp, km)) - P(fyz, X_(i, jm, km))) * F(fyz, G5, X_(i, j, km))
+        + (P(fyz, X_(i, jp, kp)) - P(fyz, X_(i, jp, km))) * F(fyz, G5, X_(i, jp, k))
+        - (P(fyz, X_(i, jm, kp)) - P(fyz, X_(i, jm, km))) * F(fyz, G5, X_(i, jm, k))
+        + (P(fxz, X_(ip, j, kp)) - P(fxz, X_(im, j, kp))) * F(fxz, G3, X_(i, j, kp))
+        - (P(fxz, X_(ip, j, km)) - P(fxz, X_(im, j, km))) * F(fxz, G3, X_(i, j, km))
+        + (P(fxz, X_(ip, j, kp)) - P(fxz, X_(ip, j, km))) * F(fxz, G3, X_(ip, j, k))
+        - (P(fxz, X_(im, j, kp)) - P(fxz, X_(im, j, km))) * F(fxz, G3, X_(im, j, k))) / T(4);
+    return;
+  }
   if (DELTA) {
     const T pc = phi[c];
     t1 = (b11p * (phi[X_(ip, j, k)] - pc) + b11m * (phi[X_(im, j, k)] - pc) + b22p * (phi[X_(i, jp, k)] - pc) +
@@ -264,14 +291,14 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __
 
 // Gauss-Seidel update of point q (0 <= q < (n/2)^3) of the colour class (kpar, io, jo), in place
 // (relaxplane, mmh/gauss1.f90:954-980).
-template <class T, int L, bool DELTA = false>
+template <class T, int L, bool DELTA = false, bool MASK = false>
 __device__ __forceinline__ void colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G,
-                                             int n, T dx2, int kpar, int io, int jo, i64 q) {
+                                             int n, T dx2, int kpar, int io, int jo, i64 q, unsigned live = 0xFFu) {
   const int h = n >> 1;
   const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
   const i64 n3 = (i64)n * n * n;
   T t1, t2, tdiag;
-  stencil19<T, L, DELTA>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+  stencil19<T, L, DELTA, MASK>(phi, G, n3, n, i, j, k, t1, t2, tdiag, live);
   const i64 c = lx<L>(i, j, k, n);
   if (DELTA) phi[c] = phi[c] + (rhs[c] * dx2 - t1 - t2) / tdiag;
   else phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
@@ -283,9 +310,11 @@ __device__ __forceinline__ void colour_point(T* __restrict__ phi, const T* __res
 // unit-stride.
 // Batched launches: gridDim.y = B grids of the same size stored one after another (bstride = n^3
 // values; G holds 6 n^3 per grid).  remap = 0 keeps the hardware's block order (a tuning choice).
-template <class T, int L, bool DELTA = false>
+// MASK: the first sweep from phi = 0, where live holds the classes updated earlier in the sweep.
+template <class T, int L, bool DELTA = false, bool MASK = false>
 __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G, int n,
-                                                T dx2, int kpar, int io, int jo, i64 bstride = 0, int remap = 1) {
+                                                T dx2, int kpar, int io, int jo, i64 bstride = 0, int remap = 1,
+                                                unsigned live = 0xFFu) {
   const int h = n >> 1;
   phi += (i64)blockIdx.y * bstride;
   rhs += (i64)blockIdx.y * bstride;
@@ -293,7 +322,7 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
   const unsigned blk = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const i64 q = (i64)blk * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
-  colour_point<T, L, DELTA>(phi, rhs, G, n, dx2, kpar, io, jo, q);
+  colour_point<T, L, DELTA, MASK>(phi, rhs, G, n, dx2, kpar, io, jo, q, live);
 }
 
 // Residual rhs - L[phi] at point (i, j, k) (calcresid, mmh/gauss1.f90:683-709).

@@ -161,17 +161,22 @@ __global__ __launch_bounds__(256) void k_rap27(double* __restrict__ Ac, const do
   for (int e = 0; e < 27; ++e) Ac[(i64)e * h3 + p] = acc[e];
 }
 
-// sum over the 26 off-centre offsets of A(p, e) phi(p + e) at point (i, j, k) of a 27-point level
-template <class T>
+// sum over the 26 off-centre offsets of A(p, e) phi(p + e) at point (i, j, k) of a 27-point level.
+// With MASK, neighbours in colour classes b (4 (k&1) + 2 (i&1) + (j&1)) whose bit of live is clear
+// hold zeros: their terms are left out, coefficient and value unread (the sum keeps its value; a
+// zero may differ in sign).
+template <class T, bool MASK = false>
 __device__ __forceinline__ T gal_offsum(const T* __restrict__ phi, const T* __restrict__ A, int n, int i, int j, int k,
-                                       i64 p) {
+                                       i64 p, unsigned live = 0xFFu) {
   const int m = n - 1;
   const i64 n3 = (i64)n * n * n;
+  const unsigned cc = 4u * (k & 1) + 2u * (i & 1) + (j & 1);
   T s = T(0);
 #pragma unroll
   for (int e = 0; e < 27; ++e) {
     if (e == 13) continue;
     const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
+    if (MASK && !((live >> (cc ^ (4u * (ez != 0) + 2u * (ex != 0) + (ey != 0)))) & 1u)) continue;
     s = s + A[(i64)e * n3 + p] * phi[lx<1>((i + ex) & m, (j + ey) & m, (k + ez) & m, n)];
   }
   return s;
@@ -179,14 +184,14 @@ __device__ __forceinline__ T gal_offsum(const T* __restrict__ phi, const T* __re
 
 // Gauss-Seidel update of point q of colour class (kpar, io, jo) on a 27-point level (the 8
 // classes decouple the 27-point stencil as they do the 19-point one).
-template <class T>
+template <class T, bool MASK = false>
 __device__ __forceinline__ void gal_colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
-                                                 int n, int kpar, int io, int jo, i64 q) {
+                                                 int n, int kpar, int io, int jo, i64 q, unsigned live = 0xFFu) {
   const int h = n >> 1;
   const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
   const i64 p = lx<1>(i, j, k, n);
   const i64 n3 = (i64)n * n * n;
-  phi[p] = (rhs[p] - gal_offsum<T>(phi, A, n, i, j, k, p)) / A[13 * n3 + p];
+  phi[p] = (rhs[p] - gal_offsum<T, MASK>(phi, A, n, i, j, k, p, live)) / A[13 * n3 + p];
 }
 
 template <class T>
@@ -198,10 +203,11 @@ __device__ __forceinline__ T gal_resid_point(const T* __restrict__ phi, const T*
 }
 
 // Batched launches: gridDim.y = B grids stored one after another (bstride = n^3 values; A holds
-// 27 n^3 per grid).
-template <class T>
+// 27 n^3 per grid).  MASK: the first sweep from phi = 0 (live: classes updated earlier in it).
+template <class T, bool MASK = false>
 __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
-                                                    int n, int kpar, int io, int jo, i64 bstride = 0) {
+                                                    int n, int kpar, int io, int jo, i64 bstride = 0,
+                                                    unsigned live = 0xFFu) {
   const int h = n >> 1;
   phi += (i64)blockIdx.y * bstride;
   rhs += (i64)blockIdx.y * bstride;
@@ -209,7 +215,7 @@ __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T
   const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
   const i64 q = (i64)blk * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
-  gal_colour_point<T>(phi, rhs, A, n, kpar, io, jo, q);
+  gal_colour_point<T, MASK>(phi, rhs, A, n, kpar, io, jo, q, live);
 }
 
 // r = rhs - A phi at every point (one thread per point, sub-lattice order); r stored.

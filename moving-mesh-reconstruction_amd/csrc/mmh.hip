@@ -50,6 +50,7 @@ struct mmh_ctx {
   int galerkin = 1;        // Krylov preconditioner: Galerkin coarse operators (option "galerkin")
   int nu_top = 0;          // Krylov preconditioner: sweeps on the finest level, 0 = nu_pre (option "nu_top")
   int coarse_fused = 4;    // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
+  int zero_start = 1;      // first sweep from u = 0 reads only the colour classes already updated (option "zero_start")
   int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
@@ -236,6 +237,7 @@ struct Sched {
   bool delta = false;      // difference form of the FP32 stencil (mixed preconditioner only)
   bool defer_top_mean = false;  // finest level: sum only, its consumers remove the mean on the fly
   int nu_top = 0;  // kind 1: sweeps each way on the finest level when > 0 (option "nu_top")
+  bool top_zero = false;  // the finest level's u is 0 on entry (a preconditioner's z); coarser ones always are
   int pre(int n) const { return kind == 0 ? 4 * ng / n : (nu_top > 0 && n == ng ? nu_top : nu_pre); }
   int post(int n) const { return kind == 0 ? 4 * ng / n : (nu_top > 0 && n == ng ? nu_top : nu_post); }
   int coarse() const { return kind == 0 ? 32 : nu_coarse; }
@@ -438,13 +440,16 @@ template <class T> void wave_sweep(mmh_ctx* c, T* out, const T* in, const T* rhs
   LAUNCHCHK();
 }
 
+// from_zero: phi is 0 on entry, so the first sweep's colours read only the classes updated before
+// them in that sweep (k_colour's MASK form; same values).
 template <class T>
 void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, int layout = 1, bool reverse = false,
-            bool delta = false) {
+            bool delta = false, bool from_zero = false) {
   const T dx2 = dx * dx;
   const int h = n / 2;
   const unsigned nb = nblk(cube(h));
-  if (layout == 1 && ns >= 2 && !delta && wave_fits<T>(c, n)) {
+  from_zero = from_zero && layout == 1 && !delta;
+  if (layout == 1 && ns >= 2 && !delta && !from_zero && wave_fits<T>(c, n)) {
     // pairs of sweeps through a second buffer; an odd last sweep goes colour by colour below
     T* tmp = wsT<T>(c, "wave_tmp" + std::to_string(n), cube(n));
     const int pairs = ns / 2;
@@ -455,12 +460,15 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, i
     ns -= 2 * pairs;
   }
   for (int s = 0; s < ns; ++s)
-    for (int cc = 0; cc < 8; ++cc) {
+    for (int cc = 0, done = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
       const int kp = col >> 2, io = (col >> 1) & 1, jo = col & 1;
-      if (layout == 1 && delta) k_colour<T, 1, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+      if (s == 0 && from_zero)
+        k_colour<T, 1, false, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo, 0, 1, (unsigned)done);
+      else if (layout == 1 && delta) k_colour<T, 1, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
       else if (layout == 1) k_colour<T, 1><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
       else k_colour<T, 0><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+      done |= 1 << col;
     }
   LAUNCHCHK();
 }
@@ -469,8 +477,8 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, i
 // rc != null), max-norm into *rinf_dev, and the calcresid restriction into rc.
 template <class T>
 void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n, T dx, int nrelax, int want_norm, T* rc,
-               double* rinf_dev, bool reverse = false, bool delta = false) {
-  sweeps(c, phi, rhs, G, n, dx, nrelax, 1, reverse, delta);
+               double* rinf_dev, bool reverse = false, bool delta = false, bool from_zero = false) {
+  sweeps(c, phi, rhs, G, n, dx, nrelax, 1, reverse, delta, from_zero);
   if (!want_norm && !rc) return;
   double* bmax = (double*)ws_get(c, "bmax", sizeof(double) * stencil_grid(n, 1));
   const unsigned nb = op_resid<T>(c, rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx), delta);
@@ -485,17 +493,23 @@ void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n,
 
 // Sweeps and relax of a Galerkin (27-point) level: the colour order of the 19-point sweeps, the
 // residual stored in r and restricted into rc with calcresid's plane order rule.
-template <class T> void gal_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* A, int n, int ns, bool reverse) {
+template <class T>
+void gal_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* A, int n, int ns, bool reverse, bool from_zero = false) {
   const unsigned nb = nblk(cube(n / 2));
   for (int s = 0; s < ns; ++s)
-    for (int cc = 0; cc < 8; ++cc) {
+    for (int cc = 0, done = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
-      k_gal_colour<T><<<nb, 256, 0, c->stream>>>(phi, rhs, A, n, col >> 2, (col >> 1) & 1, col & 1);
+      if (s == 0 && from_zero)
+        k_gal_colour<T, true><<<nb, 256, 0, c->stream>>>(phi, rhs, A, n, col >> 2, (col >> 1) & 1, col & 1, 0,
+                                                         (unsigned)done);
+      else k_gal_colour<T><<<nb, 256, 0, c->stream>>>(phi, rhs, A, n, col >> 2, (col >> 1) & 1, col & 1);
+      done |= 1 << col;
     }
   LAUNCHCHK();
 }
-template <class T> void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const T* A, T* r, int n, int nrelax, T* rc) {
-  gal_sweeps<T>(c, phi, rhs, A, n, nrelax, false);
+template <class T>
+void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const T* A, T* r, int n, int nrelax, T* rc, bool from_zero = false) {
+  gal_sweeps<T>(c, phi, rhs, A, n, nrelax, false, from_zero);
   k_gal_resid<T><<<nblk(cube(n)), 256, 0, c->stream>>>(r, phi, rhs, A, n);
   const int nrmax = std::min(4, std::max(1, n / 4));
   const int nrl = nrelax > nrmax ? (nrelax % nrmax ? nrelax % nrmax : nrmax) : nrelax;
@@ -544,13 +558,14 @@ template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const 
     LAUNCHCHK();
     return;
   }
+  const bool zero = c->zero_start && (l > 0 || S.top_zero);
   if (n == 4) {
     const int nc = S.coarse();
     if (H.A[l]) {
-      gal_sweeps<T>(c, H.u[l], H.rhs[l], H.A[l], 4, nc, false);
+      gal_sweeps<T>(c, H.u[l], H.rhs[l], H.A[l], 4, nc, false, zero);
       gal_sweeps<T>(c, H.u[l], H.rhs[l], H.A[l], 4, nc, S.symmetric);
     } else {
-      relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 0);
+      relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 0, false, false, zero);
       relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], 4, H.dx[l], nc, S.guard, (T*)nullptr, sc + 1, S.symmetric);
     }
     reduce<T>(c, H.u[l], n3, 0, sc + 2);
@@ -558,8 +573,9 @@ template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const 
     LAUNCHCHK();
     return;
   }
-  if (H.A[l]) gal_relax<T>(c, H.u[l], H.rhs[l], H.A[l], H.r[l], n, S.pre(n), H.rhs[l + 1]);
-  else relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.pre(n), S.guard, H.rhs[l + 1], sc + 0);
+  if (H.A[l]) gal_relax<T>(c, H.u[l], H.rhs[l], H.A[l], H.r[l], n, S.pre(n), H.rhs[l + 1], zero);
+  else relax_dev(c, H.u[l], H.rhs[l], H.G[l], H.r[l], n, H.dx[l], S.pre(n), S.guard, H.rhs[l + 1], sc + 0, false, false,
+                 zero);
   fill<T>(c, H.u[l + 1], T(0), cube(n / 2));
   vcycle(c, H, l + 1, ng_top, S);
   k_prolong<T, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(H.u[l], H.u[l + 1], n, 1);
@@ -1057,9 +1073,10 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   }
   const i64 n3 = cube(n);
   double* sc = Hf.scal + 3 * l;
-  if (Hf.A[l]) gal_relax<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1]);
+  const bool zero = c->zero_start && (l > 0 || S.top_zero);
+  if (Hf.A[l]) gal_relax<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1], zero);
   else relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1], sc + 0, false,
-                 S.delta);
+                 S.delta, zero);
   fill<float>(c, Hf.u[l + 1], 0.f, cube(n / 2));
   vcycle_mixed(c, Hf, Hd, l + 1, ng_top, S);
   k_prolong<float, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
@@ -1145,6 +1162,7 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   // the direction update subtract it as they read, with the expression of k_zerosum_guard), which
   // saves a pass over z; not where the finest level runs inside the one-workgroup coarse cycle.
   Sched Sz = S;
+  Sz.top_zero = true;  // z is cleared before each V-cycle
   Sz.defer_top_mean = n > 4 && !(F ? coarse_cycle_applies(c, *F, 0, S) : coarse_cycle_applies(c, H, 0, S));
   const double* zsum = Sz.defer_top_mean ? (F ? F->scal + 2 : H.scal + 2) : nullptr;
   auto precond_work = [&] {
@@ -1691,6 +1709,8 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       require(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
               "set_option: coarse_fused is 0 (off) or the largest fused level size, 4 to 32");
       c->coarse_fused = value;
+    } else if (k == "zero_start") {
+      c->zero_start = value ? 1 : 0;
     } else if (k == "pcg_graph") {
       c->pcg_graph = value ? 1 : 0;
     } else if (k == "mixed_delta") {

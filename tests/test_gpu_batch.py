@@ -148,3 +148,42 @@ def test_single_solve_coarse_cycle_equals_launches(ctx, probs, mixed):
         for xf, rf in out[:-1]:
             assert rf == rl, case
             assert np.array_equal(xf, xl), case
+
+
+@pytest.mark.parametrize("mixed,galerkin", [(0, 1), (1, 1), (0, 0), (1, 0)])
+def test_zero_start_first_sweep_equals_full_sweep(ctx, probs, mixed, galerkin):
+    """A level's first sweep from u = 0 loads only the colour classes updated before it in that
+    sweep (option zero_start): the iterates and solutions equal those of the full first sweep, for
+    the 19-point and Galerkin levels, FP64 and FP32 preconditioners, and the batched solve."""
+    print("case zero_start on/off, mixed=%d galerkin=%d" % (mixed, galerkin), flush=True)
+    ctx.set_option("galerkin", galerkin)
+    try:
+        for (u, r, d), case in zip(probs, CASES):
+            out = []
+            for zs in (1, 0):
+                ctx.set_option("zero_start", zs)
+                try:
+                    phi = torch.zeros_like(r)
+                    st, ok = ctx.solve(phi, r.clone(), d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2,
+                                       nu_post=2, nu_coarse=8, mixed=mixed, nu=5, krylov=True)
+                finally:
+                    ctx.set_option("zero_start", 1)
+                assert ok, case
+                out.append((phi.cpu().numpy(), [st.relres[i] for i in range(st.cycles)]))
+            (x1, r1), (x0, r0) = out
+            assert len(r1) > 3, case
+            assert r1 == r0, case
+            assert np.array_equal(x1, x0), case
+        if mixed == 0:
+            xs1, st1, ok1 = run_batch(ctx, probs)
+            ctx.set_option("zero_start", 0)
+            try:
+                xs0, st0, ok0 = run_batch(ctx, probs)
+            finally:
+                ctx.set_option("zero_start", 1)
+            assert ok1 and ok0
+            for a, b, s1, s0 in zip(xs1, xs0, st1, st0):
+                assert s1.cycles == s0.cycles
+                assert np.array_equal(a, b)
+    finally:
+        ctx.set_option("galerkin", 1)
