@@ -23,7 +23,7 @@ struct BatchGroup {
   // PCG vectors at the finest level (sub-lattice layout), B n^3 each
   double *x, *b, *cr, *ro, *p, *q, *z;
   // scratch of the per-grid reductions, and per-grid scalars
-  double *part, *part2, *bm, *bs;
+  double *part, *part2, *bm, *bs, *bs2;
   double *sums, *rz, *pq, *rmax, *rden, *rznew, *zro, *alpha, *beta, *relres;
   int *active, *iters;
   std::string tag;
@@ -147,12 +147,15 @@ void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, co
   LAUNCHCHK();
 }
 
-void batch_dot(mmh_ctx* c, BatchGroup& g, const double* a, const double* b, double* dst) {
+// z.a and z.b of every grid in one pass over z (k_dot2, as the single solve's dot products).
+void batch_dot2(mmh_ctx* c, BatchGroup& g, const double* z, const double* a, const double* b, double* dst_a,
+                double* dst_b) {
   const i64 n3 = cube(g.n);
-  const unsigned nb = nblk(n3, RB);
-  k_dot<double><<<dim3(nb, g.B), RB, 0, c->stream>>>(g.bs, a, b, n3, n3);
+  const unsigned nd = nblk(n3, RB * RPT);
+  k_dot2<double><<<dim3(nd, g.B), RB, 0, c->stream>>>(g.bs, g.bs2, z, a, b, n3, nullptr, n3);
   LAUNCHCHK();
-  batch_reduce(c, g, g.bs, nb, 0, dst);
+  batch_reduce(c, g, g.bs, nd, 0, dst_a);
+  batch_reduce(c, g, g.bs2, nd, 0, dst_b);
 }
 
 void batch_precond(mmh_ctx* c, BatchGroup& g, const Sched& S) {
@@ -176,8 +179,7 @@ void batch_iteration(mmh_ctx* c, BatchGroup& g, const Sched& S, double tol) {
   k_cg_scalars<<<sb, 64, 0, c->stream>>>(1, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
                                          g.active, g.iters, g.relres);
   batch_precond(c, g, S);
-  batch_dot(c, g, g.z, g.cr, g.rznew);
-  batch_dot(c, g, g.z, g.ro, g.zro);
+  batch_dot2(c, g, g.z, g.cr, g.ro, g.rznew, g.zro);
   k_cg_scalars<<<sb, 64, 0, c->stream>>>(2, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
                                          g.active, g.iters, g.relres);
   k_cg_dir_b<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.p, g.z, g.alpha, g.beta, n3);
@@ -248,6 +250,7 @@ void batch_alloc(mmh_ctx* c, BatchGroup& g) {
   g.part2 = wsT<double>(c, t + "part2", B * nbmax);
   g.bm = wsT<double>(c, t + "bm", B * nbmax);
   g.bs = wsT<double>(c, t + "bs", B * nbmax);
+  g.bs2 = wsT<double>(c, t + "bs2", B * nblk(n3, RB * RPT));
   double* sc = wsT<double>(c, t + "scal", 10 * B);
   g.sums = sc; g.rz = sc + B; g.pq = sc + 2 * B; g.rmax = sc + 3 * B; g.rden = sc + 4 * B;
   g.rznew = sc + 5 * B; g.zro = sc + 6 * B; g.alpha = sc + 7 * B; g.beta = sc + 8 * B; g.relres = sc + 9 * B;
@@ -353,7 +356,7 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     // r = b - L x, z = M r, rz = <r, z>, p = z
     op_resid<double>(c, g.cr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, false, g.B, 1);
     batch_precond(c, g, S);
-    batch_dot(c, g, g.cr, g.z, g.rz);
+    batch_dot2(c, g, g.z, g.cr, g.cr, g.rz, g.zro);  // zro: set again before its first use
     HIPCHK(hipMemcpyAsync(g.p, g.z, sizeof(double) * g.B * n3, hipMemcpyDeviceToDevice, c->stream));
     std::vector<int> hact(g.B);
     auto any_active = [&] {

@@ -447,20 +447,29 @@ template <class TZ> __device__ __forceinline__ TZ z_centred(const TZ* __restrict
 }
 
 // Per-block sums of z*r and z*ro in FP64 (z widened exactly from its storage type, its mean removed
-// on the fly when zsum is given), with the blocking of k_dot: the two dot products of one
-// flexible-CG iteration in one pass over z.
+// on the fly when zsum is given): the two dot products of one flexible-CG iteration in one pass
+// over z.  Blocking of reduce(): block b covers [b*RB*RPT, (b+1)*RB*RPT), thread t adds elements t,
+// t+RB, ... of it in order, then the halving tree.  Batched (gridDim.y = B): grid y reads at
+// y*bstride and writes its block sums at y*gridDim.x (zsum applies to unbatched launches only).
 template <class TZ>
 __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __restrict__ b2, const TZ* __restrict__ z,
                                              const double* __restrict__ r, const double* __restrict__ ro, i64 n,
-                                             const double* __restrict__ zsum = nullptr) {
+                                             const double* __restrict__ zsum = nullptr, i64 bstride = 0) {
   __shared__ double s1[RB], s2[RB];
   const int t = threadIdx.x;
-  const i64 p = (i64)blockIdx.x * RB + t;
+  const i64 y = blockIdx.y;
+  z += y * bstride; r += y * bstride; ro += y * bstride;
+  b1 += y * gridDim.x; b2 += y * gridDim.x;
+  const i64 base = (i64)blockIdx.x * (RB * RPT);
   double a1 = 0.0, a2 = 0.0;
-  if (p < n) {
-    const double zz = (double)z_centred<TZ>(z, p, zsum, n);
-    a1 = zz * r[p];
-    a2 = zz * ro[p];
+#pragma unroll
+  for (int m = 0; m < RPT; ++m) {
+    const i64 p = base + (i64)m * RB + t;
+    if (p < n) {
+      const double zz = (double)z_centred<TZ>(z, p, zsum, n);
+      a1 = a1 + zz * r[p];
+      a2 = a2 + zz * ro[p];
+    }
   }
   a1 = tree_reduce<RB, 0>(a1, s1);
   a2 = tree_reduce<RB, 0>(a2, s2);

@@ -1184,11 +1184,12 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   };
   // z.r and z.ro of the current z (deterministic block sums, one read-back for both)
   auto dots = [&](const double* rr, const double* rro, double& zr, double& zro) {
-    if (F) k_dot2<float><<<nb, RB, 0, c->stream>>>(bs, bs2, z32, rr, rro, n3, zsum);
-    else k_dot2<double><<<nb, RB, 0, c->stream>>>(bs, bs2, z, rr, rro, n3, zsum);
+    const unsigned nd = nblk(n3, RB * RPT);
+    if (F) k_dot2<float><<<nd, RB, 0, c->stream>>>(bs, bs2, z32, rr, rro, n3, zsum);
+    else k_dot2<double><<<nd, RB, 0, c->stream>>>(bs, bs2, z, rr, rro, n3, zsum);
     LAUNCHCHK();
-    reduce<double>(c, bs, nb, 0, sums + 8);
-    reduce<double>(c, bs2, nb, 0, sums + 9);
+    reduce<double>(c, bs, nd, 0, sums + 8);
+    reduce<double>(c, bs2, nd, 0, sums + 9);
     double h[2];
     HIPCHK(hipMemcpyAsync(h, sums + 8, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
