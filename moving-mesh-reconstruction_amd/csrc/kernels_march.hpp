@@ -32,9 +32,9 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
                                                       const T* __restrict__ G, int n, T di2, int KS, int batched) {
   constexpr int NT = TI * TJ, RI = TI + 2, RJ = TJ + 2, PL = RI * RJ, NH = PL - NT;
   static_assert(NH <= NT, "one halo point per thread at most");
-  __shared__ T ring[4][PL];
+  static_assert(4 * PL * sizeof(T) >= NT * sizeof(double), "the block reduction reuses the phi ring");
+  __shared__ __align__(16) T ring[4][PL];
   __shared__ T gs[2][5 * PL];  // G1..G5 of the current plane, double-buffered
-  __shared__ double red[NT];
   const i64 n3 = (i64)n * n * n;
   if (batched) {
     const i64 y = blockIdx.y;
@@ -126,6 +126,7 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
     if (tid < NH) nphih = LD(pz, oh);
   };
   load_next(k0, k0 + 1);
+  T rhs_c = MODE == 0 ? LD(rhs + K(k0), oc) : T(0);  // rhs of the current plane, loaded a step ahead
   const i64 km0 = K(k0 - 1), kc0 = K(k0), kp0 = K(k0 + 1);
   T g3m = LD(G3 + km0, oc), g5m = LD(G5 + km0, oc), g6m = LD(G6 + km0, oc);
   T g3c = LD(G3 + kc0, oc), g5c = LD(G5 + kc0, oc), g6c = LD(G6 + kc0, oc);
@@ -160,13 +161,14 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
     }
     const T g1c = n1, g4c = n4;
     // loads for the next step
-    T g3n = T(0), g5n = T(0), g6n = T(0);
+    T g3n = T(0), g5n = T(0), g6n = T(0), rhs_n = T(0);
     if (t + 1 < KS) {
       load_next(k + 1, k + 2);
       const i64 k2 = K(k + 2);
       g3n = LD(G3 + k2, oc);
       g5n = LD(G5 + k2, oc);
       g6n = LD(G6 + k2, oc);
+      if (MODE == 0) rhs_n = LD(rhs + K(k + 1), oc);
     }
     __syncthreads();
     const T* sm = ring[(k - 1) & 3];
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
     const T pc = PH_(sc, 0, 0);
 #undef PH_
     if (MODE == 0) {
-      const T rv = LD(rhs + kc, oc) - (t1 + t2 + tdiag * pc) * di2;
+      const T rv = rhs_c - (t1 + t2 + tdiag * pc) * di2;
       if (out) *(T*)((char*)(out + kc) + oc) = rv;
       acc = dmax(acc, (double)fabs(rv));
     } else {
@@ -216,8 +218,10 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
     g3m = g3c; g3c = g3p; g3p = g3n;
     g5m = g5c; g5c = g5p; g5p = g5n;
     g6m = g6c; g6c = g6p; g6p = g6n;
+    rhs_c = rhs_n;
   }
-  acc = tree_reduce<NT, MODE == 0 ? 1 : 0>(acc, red);
+  __syncthreads();  // the last step's reads of the ring are done: it becomes the reduction's scratch
+  acc = tree_reduce<NT, MODE == 0 ? 1 : 0>(acc, reinterpret_cast<double*>(&ring[0][0]));
   if (tid == 0) bres[b] = acc;
 }
 
