@@ -125,7 +125,7 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 preconditioning V-cycle as a HIP graph on its second call and replays it after;
  *                 on the null stream the solve then runs on a stream of its own, ordered after
  *                 the work already queued
- *   "batch_graph", "batch_autotune": see mmh_solve_batch_d */
+ *   "batch_graph", "batch_autotune", "batch_streams": see mmh_solve_batch_d */
 int mmh_set_option(mmh_ctx* ctx, const char* key, int value);
 
 /* ---- solver (device pointers) --------------------------------------------------------------- */
@@ -174,8 +174,9 @@ typedef struct {
  * grouped.  Each group advances with one launch per stage for all its grids, one PCG iteration
  * replayed as a HIP graph, and colour-kernel launch shapes tuned on first use.  Only the FP64
  * Krylov form (opts: krylov = 1, schedule = 1, nu_pre = nu_post, mixed = 0); every item's result
- * is bit-identical to its mmh_solve_d.  Options "batch_graph" and "batch_autotune" (mmh_set_option)
- * switch graph replay and tuning off. */
+ * is bit-identical to its mmh_solve_d.  The groups iterate side by side, each on a stream of its
+ * own.  Options "batch_graph", "batch_autotune" and "batch_streams" (mmh_set_option) switch graph
+ * replay, tuning and the group streams off (results unchanged). */
 int mmh_solve_batch_d(mmh_ctx*, int count, const int* n, double* const* u, double* const* rhs,
                       double* const* defpot, double* const* ubig, int nu, const mmh_solve_opts* opts,
                       mmh_batch_stats* st);

@@ -24,6 +24,7 @@ struct BatchGroup {
   double *x, *b, *cr, *ro, *p, *q, *z;
   // scratch of the per-grid reductions, and per-grid scalars
   double *part, *part2, *bm, *bs, *bs2;
+  int* hflags = nullptr;  // pinned host copy of active
   double *sums, *rz, *pq, *rmax, *rden, *rznew, *zro, *alpha, *beta, *relres;
   int *active, *iters;
   std::string tag;
@@ -327,60 +328,136 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     require(is_pow2(ns[i]) && ns[i] >= 8 && ns[i] <= 1024, "solve_batch: sizes must be powers of two in [8, 1024]");
     bysize[ns[i]].push_back(i);
   }
-  bool all_ok = true;
-  for (auto& kv : bysize) {
+  // Phase 1, on the solve's stream: per size group, set-up, the first residual and preconditioner
+  // application.  Phase 2: the PCG iterations, each group on a stream of its own (option
+  // batch_streams; the small groups' launches then overlap the large ones' work), one iteration of
+  // every unfinished group per round, their active flags copied to pinned host memory and read
+  // once per round.  Phase 3: final residuals and results.  A group issues the same launches in
+  // the same order either way, so results do not depend on the option.
+  struct GroupRun {
     BatchGroup g;
-    g.n = kv.first;
-    g.B = (int)kv.second.size();
-    g.items = kv.second;
-    batch_alloc(c, g);
-    S.ng = g.n;  // the group's finest level (nu_top applies there)
-    const i64 n3 = cube(g.n);
-    std::vector<double> rden(g.B);
-    std::vector<int> act(g.B);
-    for (int s = 0; s < g.B; ++s) {
-      const int i = g.items[s];
-      const double rn = batch_setup_item(c, g, s, u[i], rhs[i], defpot[i], ubig[i], nu);
-      rden[s] = rn > 0 ? rn : 1.0;
-      act[s] = rn > 0 ? 1 : 0;
-      if (rn == 0) fill<double>(c, g.x + (i64)s * n3, 0.0, n3);  // L u = 0: the mean-free solution is 0
-    }
-    HIPCHK(hipMemcpyAsync(g.rden, rden.data(), sizeof(double) * g.B, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(g.active, act.data(), sizeof(int) * g.B, hipMemcpyHostToDevice, c->stream));
-    fill<double>(c, g.relres, 0.0, g.B);
-    {
-      std::vector<int> zeros(g.B, 0);
-      HIPCHK(hipMemcpyAsync(g.iters, zeros.data(), sizeof(int) * g.B, hipMemcpyHostToDevice, c->stream));
-    }
-    if (c->batch_autotune) batch_autotune(c, g);
-    // r = b - L x, z = M r, rz = <r, z>, p = z
-    op_resid<double>(c, g.cr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, false, g.B, 1);
-    batch_precond(c, g, S);
-    batch_dot2(c, g, g.z, g.cr, g.cr, g.rz, g.zro);  // zro: set again before its first use
-    HIPCHK(hipMemcpyAsync(g.p, g.z, sizeof(double) * g.B * n3, hipMemcpyDeviceToDevice, c->stream));
-    std::vector<int> hact(g.B);
-    auto any_active = [&] {
-      HIPCHK(hipMemcpyAsync(hact.data(), g.active, sizeof(int) * g.B, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
-      for (int v : hact)
-        if (v) return true;
-      return false;
-    };
+    std::vector<double> rden;
+    hipStream_t s = nullptr;
     int it = 0;
-    if (any_active()) {
-      // the first iteration runs eagerly (it also sizes every workspace), the rest replay its graph
-      Replay graph(c);
-      batch_iteration(c, g, S, o->tol);
-      ++it;
-      if (c->batch_graph && it < o->max_cycles && any_active())
-        graph.capture([&] { batch_iteration(c, g, S, o->tol); });
-      while (it < o->max_cycles && any_active()) {
-        if (graph.exec) graph.launch();
-        else batch_iteration(c, g, S, o->tol);
-        ++it;
+    bool done = false;
+    std::unique_ptr<Replay> graph;
+  };
+  struct Streams {  // group streams and pinned flags, released on every exit path
+    mmh_ctx* c;
+    hipStream_t main;
+    std::vector<hipStream_t> own;
+    int* hflags = nullptr;
+    ~Streams() {
+      for (hipStream_t q : own) {
+        (void)hipStreamSynchronize(q);
+        (void)hipStreamDestroy(q);
       }
+      if (hflags) (void)hipHostFree(hflags);
+      c->stream = main;
     }
-    const bool group_ok = !any_active();
+  } streams{c, c->stream, {}, nullptr};
+  std::vector<GroupRun> runs(bysize.size());
+  int total_items = 0;
+  {
+    size_t gi = 0;
+    for (auto& kv : bysize) {
+      GroupRun& R = runs[gi++];
+      BatchGroup& g = R.g;
+      g.n = kv.first;
+      g.B = (int)kv.second.size();
+      g.items = kv.second;
+      total_items += g.B;
+      batch_alloc(c, g);
+      S.ng = g.n;
+      const i64 n3 = cube(g.n);
+      R.rden.assign(g.B, 1.0);
+      std::vector<int> act(g.B);
+      for (int s = 0; s < g.B; ++s) {
+        const int i = g.items[s];
+        const double rn = batch_setup_item(c, g, s, u[i], rhs[i], defpot[i], ubig[i], nu);
+        R.rden[s] = rn > 0 ? rn : 1.0;
+        act[s] = rn > 0 ? 1 : 0;
+        if (rn == 0) fill<double>(c, g.x + (i64)s * n3, 0.0, n3);  // L u = 0: the mean-free solution is 0
+      }
+      HIPCHK(hipMemcpyAsync(g.rden, R.rden.data(), sizeof(double) * g.B, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(g.active, act.data(), sizeof(int) * g.B, hipMemcpyHostToDevice, c->stream));
+      fill<double>(c, g.relres, 0.0, g.B);
+      {
+        std::vector<int> zeros(g.B, 0);
+        HIPCHK(hipMemcpyAsync(g.iters, zeros.data(), sizeof(int) * g.B, hipMemcpyHostToDevice, c->stream));
+      }
+      if (c->batch_autotune) batch_autotune(c, g);
+      // r = b - L x, z = M r, rz = <r, z>, p = z
+      op_resid<double>(c, g.cr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, false, g.B, 1);
+      batch_precond(c, g, S);
+      batch_dot2(c, g, g.z, g.cr, g.cr, g.rz, g.zro);  // zro: set again before its first use
+      HIPCHK(hipMemcpyAsync(g.p, g.z, sizeof(double) * g.B * n3, hipMemcpyDeviceToDevice, c->stream));
+      // the host staging vectors above must outlive their copies
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+  }
+  HIPCHK(hipHostMalloc((void**)&streams.hflags, sizeof(int) * total_items, hipHostMallocDefault));
+  {
+    int off = 0;
+    for (GroupRun& R : runs) {
+      if (c->batch_streams) {
+        hipStream_t q = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+        streams.own.push_back(q);
+        R.s = q;  // phase 1 ended with a synchronisation of the solve's stream
+      } else {
+        R.s = streams.main;
+      }
+      R.g.hflags = streams.hflags + off;
+      off += R.g.B;
+    }
+  }
+  auto copy_flags = [&](GroupRun& R) {
+    HIPCHK(hipMemcpyAsync(R.g.hflags, R.g.active, sizeof(int) * R.g.B, hipMemcpyDeviceToHost, R.s));
+  };
+  auto any_active = [&](const GroupRun& R) {
+    for (int s = 0; s < R.g.B; ++s)
+      if (R.g.hflags[s]) return true;
+    return false;
+  };
+  for (GroupRun& R : runs) copy_flags(R);
+  for (GroupRun& R : runs) {
+    HIPCHK(hipStreamSynchronize(R.s));
+    R.done = !any_active(R) || o->max_cycles < 1;
+  }
+  for (bool pending = true; pending;) {
+    pending = false;
+    for (GroupRun& R : runs) {
+      if (R.done) continue;
+      c->stream = R.s;
+      S.ng = R.g.n;
+      // the first iteration runs eagerly (it also sizes every workspace), the rest replay its graph
+      if (R.it == 0) {
+        batch_iteration(c, R.g, S, o->tol);
+      } else {
+        if (!R.graph && c->batch_graph) {
+          R.graph.reset(new Replay(c));
+          R.graph->capture([&] { batch_iteration(c, R.g, S, o->tol); });
+        }
+        if (R.graph) R.graph->launch();
+        else batch_iteration(c, R.g, S, o->tol);
+      }
+      ++R.it;
+      copy_flags(R);
+      pending = true;
+    }
+    for (GroupRun& R : runs) {
+      if (R.done) continue;
+      HIPCHK(hipStreamSynchronize(R.s));
+      if (!any_active(R) || R.it >= o->max_cycles) R.done = true;
+    }
+  }
+  bool all_ok = true;
+  for (GroupRun& R : runs) {
+    BatchGroup& g = R.g;
+    c->stream = R.s;
+    const i64 n3 = cube(g.n);
+    const bool group_ok = !any_active(R);
     all_ok = all_ok && group_ok;
     // true residual of the final iterates, mean removal, back to the caller's layout
     const unsigned nsr = op_resid<double>(c, (double*)nullptr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, false, g.B, 1);
@@ -397,9 +474,11 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     if (st)
       for (int s = 0; s < g.B; ++s) {
         st[g.items[s]].cycles = hi[s];
-        st[g.items[s]].relres = hr[s] / rden[s];
-        st[g.items[s]].converged = hact[s] ? 0 : 1;
+        st[g.items[s]].relres = hr[s] / R.rden[s];
+        st[g.items[s]].converged = g.hflags[s] ? 0 : 1;
       }
   }
+  runs.clear();  // graphs go before their streams
+  c->stream = streams.main;
   if (!all_ok) throw MmhError(MMH_ENOCONV, "solve_batch: tolerance not reached for every realisation");
 }

@@ -55,6 +55,7 @@ struct mmh_ctx {
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
   int batch_autotune = 1;  // batched solve: pick the colour kernel's launch shape by timing
+  int batch_streams = 1;   // batched solve: size groups iterate side by side on streams of their own
   std::map<std::string, std::pair<int, int>> tune;  // (grid size, batch) -> (block size, XCD remap)
 };
 
@@ -152,14 +153,17 @@ struct StreamSwap {
   }
 };
 
-// A sequence of launches captured once on the context's stream and then replayed.  While it
-// exists, workspace buffers may be added but not moved (ws_get refuses to grow one).
+// A sequence of launches captured once on the context's current stream and then replayed on that
+// stream.  While it exists, workspace buffers may be added but not moved (ws_get refuses to grow
+// one).
 struct Replay {
   mmh_ctx* c;
+  hipStream_t st = nullptr;  // the stream it was captured on
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   explicit Replay(mmh_ctx* cc) : c(cc) {}
   template <class F> void capture(F&& work) {
+    st = c->stream;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     try {
       work();
@@ -173,10 +177,10 @@ struct Replay {
     HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     ++c->graphs_live;
   }
-  void launch() { HIPCHK(hipGraphLaunch(exec, c->stream)); }
+  void launch() { HIPCHK(hipGraphLaunch(exec, st)); }
   ~Replay() {
     if (exec) {
-      (void)hipStreamSynchronize(c->stream);
+      (void)hipStreamSynchronize(st);
       (void)hipGraphExecDestroy(exec);
       --c->graphs_live;
     }
@@ -1712,6 +1716,8 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->coarse_fused = value;
     } else if (k == "zero_start") {
       c->zero_start = value ? 1 : 0;
+    } else if (k == "batch_streams") {
+      c->batch_streams = value ? 1 : 0;
     } else if (k == "pcg_graph") {
       c->pcg_graph = value ? 1 : 0;
     } else if (k == "mixed_delta") {
