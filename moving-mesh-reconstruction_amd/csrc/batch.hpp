@@ -25,6 +25,7 @@ struct BatchGroup {
   // scratch of the per-grid reductions, and per-grid scalars
   double *part, *part2, *bm, *bs, *bs2;
   int* hflags = nullptr;  // pinned host copy of active
+  double* ssum = nullptr;  // per grid: set-up sums (8 each)
   double *sums, *rz, *pq, *rmax, *rden, *rznew, *zro, *alpha, *beta, *relres;
   int *active, *iters;
   std::string tag;
@@ -253,6 +254,7 @@ void batch_alloc(mmh_ctx* c, BatchGroup& g) {
   g.bs = wsT<double>(c, t + "bs", B * nbmax);
   g.bs2 = wsT<double>(c, t + "bs2", B * nblk(n3, RB * RPT));
   double* sc = wsT<double>(c, t + "scal", 10 * B);
+  g.ssum = wsT<double>(c, t + "setup_sums", 8 * B);
   g.sums = sc; g.rz = sc + B; g.pq = sc + 2 * B; g.rmax = sc + 3 * B; g.rden = sc + 4 * B;
   g.rznew = sc + 5 * B; g.zro = sc + 6 * B; g.alpha = sc + 7 * B; g.beta = sc + 8 * B; g.relres = sc + 9 * B;
   int* ic = (int*)ws_get(c, t + "flags", sizeof(int) * 2 * B);
@@ -261,8 +263,11 @@ void batch_alloc(mmh_ctx* c, BatchGroup& g) {
 
 // Level data of item s: the metric of every level from def and u(1) (build_hier's arithmetic),
 // the projected right-hand side and the initial guess in sub-lattice order.  Returns max|rhs|.
-double batch_setup_item(mmh_ctx* c, BatchGroup& g, int s, double* u_user, double* rhs_user, const double* def,
-                        const double* ubig, int nu) {
+// Set-up of grid s of the group, all on the device (no host round trip): metrics, Galerkin
+// operators, rgzerosum, max|rhs| into g.ssum[8 s + 3], the iterate and right-hand side in
+// sub-lattice order.
+void batch_setup_item(mmh_ctx* c, BatchGroup& g, int s, double* u_user, double* rhs_user, const double* def,
+                      const double* ubig, int nu) {
   const int n = g.n;
   const i64 n3 = cube(n);
   const double* dl = def;
@@ -301,12 +306,11 @@ double batch_setup_item(mmh_ctx* c, BatchGroup& g, int s, double* u_user, double
       k_rap27<<<nblk(cube(m)), 256, 0, c->stream>>>(Al, g.A[l - 1] + (i64)s * 27 * cube(g.m[l - 1]), g.m[l - 1]);
     LAUNCHCHK();
   }
-  double* sums = wsT<double>(c, g.tag + "setup_sums", 8);
+  double* sums = g.ssum + 8 * s;
   rgzerosum_dev<double>(c, rhs_user, def, 1.0, n, sums);
   reduce<double>(c, rhs_user, n3, 1, sums + 3);
   relayout(c, g.x + (i64)s * n3, u_user, n, 1, 1);
   relayout(c, g.b + (i64)s * n3, rhs_user, n, 1, 1);
-  return read_scalar<double>(c, sums + 3);
 }
 
 void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, double* const* rhs,
@@ -371,31 +375,23 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
       S.ng = g.n;
       const i64 n3 = cube(g.n);
       R.rden.assign(g.B, 1.0);
-      std::vector<int> act(g.B);
       for (int s = 0; s < g.B; ++s) {
         const int i = g.items[s];
-        const double rn = batch_setup_item(c, g, s, u[i], rhs[i], defpot[i], ubig[i], nu);
-        R.rden[s] = rn > 0 ? rn : 1.0;
-        act[s] = rn > 0 ? 1 : 0;
-        if (rn == 0) fill<double>(c, g.x + (i64)s * n3, 0.0, n3);  // L u = 0: the mean-free solution is 0
+        batch_setup_item(c, g, s, u[i], rhs[i], defpot[i], ubig[i], nu);
       }
-      HIPCHK(hipMemcpyAsync(g.rden, R.rden.data(), sizeof(double) * g.B, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(g.active, act.data(), sizeof(int) * g.B, hipMemcpyHostToDevice, c->stream));
+      k_batch_flags<<<nblk(g.B, 64), 64, 0, c->stream>>>(g.B, g.ssum, g.rden, g.active, g.iters);
+      k_zero_inactive<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.active, n3);
+      LAUNCHCHK();
       fill<double>(c, g.relres, 0.0, g.B);
-      {
-        std::vector<int> zeros(g.B, 0);
-        HIPCHK(hipMemcpyAsync(g.iters, zeros.data(), sizeof(int) * g.B, hipMemcpyHostToDevice, c->stream));
-      }
       if (c->batch_autotune) batch_autotune(c, g);
       // r = b - L x, z = M r, rz = <r, z>, p = z
       op_resid<double>(c, g.cr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, false, g.B, 1);
       batch_precond(c, g, S);
       batch_dot2(c, g, g.z, g.cr, g.cr, g.rz, g.zro);  // zro: set again before its first use
       HIPCHK(hipMemcpyAsync(g.p, g.z, sizeof(double) * g.B * n3, hipMemcpyDeviceToDevice, c->stream));
-      // the host staging vectors above must outlive their copies
-      HIPCHK(hipStreamSynchronize(c->stream));
     }
   }
+  HIPCHK(hipStreamSynchronize(c->stream));  // the group streams start after the set-up
   HIPCHK(hipHostMalloc((void**)&streams.hflags, sizeof(int) * total_items, hipHostMallocDefault));
   {
     int off = 0;
@@ -469,6 +465,7 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     std::vector<int> hi(g.B);
     HIPCHK(hipMemcpyAsync(hr.data(), g.rmax, sizeof(double) * g.B, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(hi.data(), g.iters, sizeof(int) * g.B, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(R.rden.data(), g.rden, sizeof(double) * g.B, hipMemcpyDeviceToHost, c->stream));
     for (int s = 0; s < g.B; ++s) relayout(c, u[g.items[s]], g.x + (i64)s * n3, g.n, 1, 0);
     HIPCHK(hipStreamSynchronize(c->stream));
     if (st)

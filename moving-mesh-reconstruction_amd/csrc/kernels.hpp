@@ -573,6 +573,26 @@ __global__ void k_cg_dir_b(double* __restrict__ x, double* __restrict__ p, const
   p[y * n + i] = z[y * n + i] + beta[y] * pi;
 }
 
+// Batched set-up, one thread per grid: rn = max|rhs| of grid y (ssum[8y + 3], written by its
+// set-up) gives the residual scale rden = rn (1 where rn = 0) and the active flag rn > 0; the
+// iteration count starts at 0.
+__global__ void k_batch_flags(int B, const double* __restrict__ ssum, double* __restrict__ rden, int* __restrict__ active,
+                              int* __restrict__ iters) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= B) return;
+  const double rn = ssum[8 * y + 3];
+  rden[y] = rn > 0 ? rn : 1.0;
+  active[y] = rn > 0 ? 1 : 0;
+  iters[y] = 0;
+}
+
+// x = 0 for the grids that start inactive (L u = 0: the mean-free solution is 0); gridDim.y = B.
+__global__ void k_zero_inactive(double* __restrict__ x, const int* __restrict__ active, i64 n) {
+  const i64 y = blockIdx.y;
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !active[y]) x[y * n + i] = 0.0;
+}
+
 // Per-grid CG scalars, one thread per grid.  step 0: alpha = rz / pq for active grids, else 0.
 // step 1: relres = rmax / rden and the iteration count advance for active grids; a grid whose
 // relres <= tol becomes inactive.  step 2: beta = (rz_new - zro) / rz, rz = rz_new (0 for inactive grids).
