@@ -187,3 +187,22 @@ def test_zero_start_first_sweep_equals_full_sweep(ctx, probs, mixed, galerkin):
                 assert np.array_equal(a, b)
     finally:
         ctx.set_option("galerkin", 1)
+
+
+def test_batch_group_streams_equal_one_stream(ctx, probs):
+    """The size groups iterate side by side on streams of their own (option batch_streams); with
+    the option off they take turns on one stream.  Every result and statistic is the same."""
+    print("case batch_streams on/off %s" % CASES, flush=True)
+    out = []
+    for bs in (1, 0):
+        ctx.set_option("batch_streams", bs)
+        try:
+            xs, st, ok = run_batch(ctx, probs)
+        finally:
+            ctx.set_option("batch_streams", 1)
+        assert ok
+        out.append((xs, [(s.cycles, s.relres, s.converged) for s in st]))
+    (x1, s1), (x0, s0) = out
+    assert s1 == s0
+    for a, b in zip(x1, x0):
+        assert np.array_equal(a, b)
