@@ -504,7 +504,6 @@ __global__ void k_axpy(double* __restrict__ x, const double* __restrict__ p, dou
   if (i < n) x[i] = x[i] + alpha * p[i];
 }
 
-// p = z + beta p.
 // x += alpha p (with x), then p = z + beta p (z centred on the fly when zsum is given); with
 // accumulate = 0, p = z.  One pass for the iterate's update and the new direction.
 template <class TZ>
