@@ -25,7 +25,7 @@ struct BatchGroup {
   // scratch of the per-grid reductions, and per-grid scalars
   double *part, *part2, *bm, *bs, *bs2;
   int* hflags = nullptr;  // pinned host copy of active
-  double* ssum = nullptr;  // per grid: set-up sums (8 each)
+  double* ssum = nullptr;  // set-up sums: det sums, rhs sums, max|rhs| (B each)
   double *sums, *rz, *pq, *rmax, *rden, *rznew, *zro, *alpha, *beta, *relres;
   int *active, *iters;
   std::string tag;
@@ -263,54 +263,82 @@ void batch_alloc(mmh_ctx* c, BatchGroup& g) {
 
 // Level data of item s: the metric of every level from def and u(1) (build_hier's arithmetic),
 // the projected right-hand side and the initial guess in sub-lattice order.  Returns max|rhs|.
-// Set-up of grid s of the group, all on the device (no host round trip): metrics, Galerkin
-// operators, rgzerosum, max|rhs| into g.ssum[8 s + 3], the iterate and right-hand side in
-// sub-lattice order.
-void batch_setup_item(mmh_ctx* c, BatchGroup& g, int s, double* u_user, double* rhs_user, const double* def,
-                      const double* ubig, int nu) {
-  const int n = g.n;
+// Set-up of every grid of the group, one launch per stage for all of them (grids stacked,
+// gridDim.y = B) and all on the device: the caller's arrays are gathered through a table of
+// pointers (tab keeps the host copy alive), then per level the restricted deformation and density
+// and the metric, the Galerkin operators, rgzerosum (the projected rhs also goes back to the
+// caller's array, as in the single solve), max|rhs| into g.ssum[2B + s], and the iterate and
+// right-hand side in sub-lattice order.  Per grid this is the single solve's arithmetic (reductions
+// by batch_reduce keep reduce()'s staging), so results stay bit-identical to it.
+void batch_setup_group(mmh_ctx* c, BatchGroup& g, std::vector<const void*>& tab, double* const* u, double* const* rhs,
+                       double* const* defpot, double* const* ubig, int nu) {
+  const int n = g.n, B = g.B;
   const i64 n3 = cube(n);
-  const double* dl = def;
-  const double* wl = ubig;
-  int nul = nu;
-  if (nu > 1) {
-    double* w = wsT<double>(c, g.tag + "u1c", n3);
-    k_copy_strided<double><<<nblk(n3), 256, 0, c->stream>>>(w, ubig, n3, nu);
-    LAUNCHCHK();
-    wl = w;
-    nul = 1;
+  tab.resize(4 * (size_t)B);
+  for (int s = 0; s < B; ++s) {
+    const int i = g.items[s];
+    tab[s] = ubig[i];
+    tab[B + s] = defpot[i];
+    tab[2 * B + s] = rhs[i];
+    tab[3 * B + s] = u[i];
   }
+  const void** dtab = (const void**)ws_get(c, g.tag + "ptrs", sizeof(void*) * 4 * B);
+  HIPCHK(hipMemcpyAsync(dtab, tab.data(), sizeof(void*) * 4 * B, hipMemcpyHostToDevice, c->stream));
+  const double* const* t_ubig = (const double* const*)dtab;
+  const double* const* t_def = (const double* const*)(dtab + B);
+  double* const* t_rhs = (double* const*)(dtab + 2 * B);
+  const double* const* t_u = (const double* const*)(dtab + 3 * B);
+  double* wst = wsT<double>(c, g.tag + "ws0", B * n3);
+  double* dst = wsT<double>(c, g.tag + "ds0", B * n3);
+  double* rst = wsT<double>(c, g.tag + "rs", B * n3);
+  const dim3 gr(nblk(n3), B);
+  k_gather<double><<<gr, 256, 0, c->stream>>>(wst, t_ubig, n3, nu);  // density: component 1 of u
+  k_gather<double><<<gr, 256, 0, c->stream>>>(dst, t_def, n3, 1);
+  k_gather<double><<<gr, 256, 0, c->stream>>>(rst, (const double* const*)t_rhs, n3, 1);
+  LAUNCHCHK();
+  const double* dl = dst;
+  const double* wl = wst;
   for (int l = 0; l < g.nlev; ++l) {
     const int m = g.m[l];
+    const dim3 gl(nblk(cube(m)), B);
     if (l > 0) {
-      double* d2 = wsT<double>(c, g.tag + "d" + std::to_string(l), cube(m));
-      double* w2 = wsT<double>(c, g.tag + "w" + std::to_string(l), cube(m));
-      k_restrict27<double><<<nblk(cube(m)), 256, 0, c->stream>>>(d2, dl, 2 * m, 1);
-      k_restrict27<double><<<nblk(cube(m)), 256, 0, c->stream>>>(w2, wl, 2 * m, nul);
+      double* d2 = wsT<double>(c, g.tag + "ds" + std::to_string(l), B * cube(m));
+      double* w2 = wsT<double>(c, g.tag + "ws" + std::to_string(l), B * cube(m));
+      k_restrict27<double><<<gl, 256, 0, c->stream>>>(d2, dl, 2 * m, 1, 1);
+      k_restrict27<double><<<gl, 256, 0, c->stream>>>(w2, wl, 2 * m, 1, 1);
       LAUNCHCHK();
       dl = d2;
       wl = w2;
-      nul = 1;
     }
-    k_metric<double, 1><<<nblk(cube(m)), 256, 0, c->stream>>>(g.G[l] + (i64)s * 6 * cube(m), dl, wl, nul, m, g.dx[l], 0);
+    k_metric<double, 1><<<gl, 256, 0, c->stream>>>(g.G[l], dl, wl, 1, m, g.dx[l], 0, 1);
     LAUNCHCHK();
   }
-  // Galerkin operators of this grid (build_galerkin's kernels on the grid's slices)
+  // Galerkin operators (build_galerkin's kernels on the stacked levels)
   for (int l = 1; l < g.nlev; ++l) {
     if (!g.A[l]) break;
-    const int m = g.m[l];
-    double* Al = g.A[l] + (i64)s * 27 * cube(m);
-    if (l == 1)
-      k_rap19<double><<<nblk(cube(m)), 256, 0, c->stream>>>(Al, g.G[0] + (i64)s * 6 * n3, n, 1.0 / (g.dx[0] * g.dx[0]));
-    else
-      k_rap27<<<nblk(cube(m)), 256, 0, c->stream>>>(Al, g.A[l - 1] + (i64)s * 27 * cube(g.m[l - 1]), g.m[l - 1]);
+    const dim3 gl(nblk(cube(g.m[l])), B);
+    if (l == 1) k_rap19<double><<<gl, 256, 0, c->stream>>>(g.A[1], g.G[0], n, 1.0 / (g.dx[0] * g.dx[0]), 1);
+    else k_rap27<<<gl, 256, 0, c->stream>>>(g.A[l], g.A[l - 1], g.m[l - 1], 1);
     LAUNCHCHK();
   }
-  double* sums = g.ssum + 8 * s;
-  rgzerosum_dev<double>(c, rhs_user, def, 1.0, n, sums);
-  reduce<double>(c, rhs_user, n3, 1, sums + 3);
-  relayout(c, g.x + (i64)s * n3, u_user, n, 1, 1);
-  relayout(c, g.b + (i64)s * n3, rhs_user, n, 1, 1);
+  // rgzerosum per grid (mmh/gauss1.f90:1219-1417: det, its sum and the rhs sum, the update), then
+  // max|rhs|, the projected rhs back to the caller, the initial iterates and rhs in sub-lattice order
+  double* det = wsT<double>(c, g.tag + "det", B * n3);
+  double* dsum = g.ssum;
+  double* asum = g.ssum + B;
+  double* rnmax = g.ssum + 2 * B;
+  k_det<double><<<gr, 256, 0, c->stream>>>(det, dst, n, 1.0, 1);
+  LAUNCHCHK();
+  batch_reduce(c, g, det, n3, 0, dsum);
+  batch_reduce(c, g, rst, n3, 0, asum);
+  k_rgz_update_b<double><<<gr, 256, 0, c->stream>>>(rst, det, dsum, asum, n3);
+  LAUNCHCHK();
+  batch_reduce(c, g, rst, n3, 1, rnmax);
+  k_scatter<double><<<gr, 256, 0, c->stream>>>(t_rhs, rst, n3);
+  k_gather<double><<<gr, 256, 0, c->stream>>>(det, t_u, n3, 1);  // det is free again
+  k_relayout<double, 1><<<gr, 256, 0, c->stream>>>(g.x, det, n, 1, 1);
+  k_relayout<double, 1><<<gr, 256, 0, c->stream>>>(g.b, rst, n, 1, 1);
+  LAUNCHCHK();
 }
 
 void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, double* const* rhs,
@@ -332,15 +360,19 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     require(is_pow2(ns[i]) && ns[i] >= 8 && ns[i] <= 1024, "solve_batch: sizes must be powers of two in [8, 1024]");
     bysize[ns[i]].push_back(i);
   }
-  // Phase 1, on the solve's stream: per size group, set-up, the first residual and preconditioner
-  // application.  Phase 2: the PCG iterations, each group on a stream of its own (option
-  // batch_streams; the small groups' launches then overlap the large ones' work), one iteration of
-  // every unfinished group per round, their active flags copied to pinned host memory and read
-  // once per round.  Phase 3: final residuals and results.  A group issues the same launches in
-  // the same order either way, so results do not depend on the option.
+  // Phase 1, queued on the solve's stream without host round trips, largest grids first: per size
+  // group, set-up, the first residual and preconditioner application, then an event.  Phase 2: the
+  // PCG iterations, each group on a stream of its own (option batch_streams) that waits for its
+  // group's event only, so the large group iterates while the small ones are still being set up
+  // and the small groups' launches overlap its work; one iteration of every unfinished group per
+  // round, their active flags copied to pinned host memory and read once per round (a group whose
+  // grids all start converged runs one iteration with alpha = beta = 0, which changes nothing).
+  // Phase 3: final residuals and results.  A group issues the same launches in the same order
+  // either way, so results do not depend on the option.
   struct GroupRun {
     BatchGroup g;
     std::vector<double> rden;
+    std::vector<const void*> tab;  // the group's pointer table (host copy)
     hipStream_t s = nullptr;
     int it = 0;
     bool done = false;
@@ -350,21 +382,25 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     mmh_ctx* c;
     hipStream_t main;
     std::vector<hipStream_t> own;
+    std::vector<hipEvent_t> ev;
     int* hflags = nullptr;
     ~Streams() {
       for (hipStream_t q : own) {
         (void)hipStreamSynchronize(q);
         (void)hipStreamDestroy(q);
       }
+      (void)hipStreamSynchronize(main);
+      for (hipEvent_t e : ev) (void)hipEventDestroy(e);
       if (hflags) (void)hipHostFree(hflags);
       c->stream = main;
     }
-  } streams{c, c->stream, {}, nullptr};
+  } streams{c, c->stream, {}, {}, nullptr};
   std::vector<GroupRun> runs(bysize.size());
   int total_items = 0;
   {
     size_t gi = 0;
-    for (auto& kv : bysize) {
+    for (auto it = bysize.rbegin(); it != bysize.rend(); ++it) {
+      auto& kv = *it;
       GroupRun& R = runs[gi++];
       BatchGroup& g = R.g;
       g.n = kv.first;
@@ -375,11 +411,8 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
       S.ng = g.n;
       const i64 n3 = cube(g.n);
       R.rden.assign(g.B, 1.0);
-      for (int s = 0; s < g.B; ++s) {
-        const int i = g.items[s];
-        batch_setup_item(c, g, s, u[i], rhs[i], defpot[i], ubig[i], nu);
-      }
-      k_batch_flags<<<nblk(g.B, 64), 64, 0, c->stream>>>(g.B, g.ssum, g.rden, g.active, g.iters);
+      batch_setup_group(c, g, R.tab, u, rhs, defpot, ubig, nu);
+      k_batch_flags<<<nblk(g.B, 64), 64, 0, c->stream>>>(g.B, g.ssum + 2 * g.B, g.rden, g.active, g.iters);
       k_zero_inactive<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.active, n3);
       LAUNCHCHK();
       fill<double>(c, g.relres, 0.0, g.B);
@@ -389,9 +422,12 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
       batch_precond(c, g, S);
       batch_dot2(c, g, g.z, g.cr, g.cr, g.rz, g.zro);  // zro: set again before its first use
       HIPCHK(hipMemcpyAsync(g.p, g.z, sizeof(double) * g.B * n3, hipMemcpyDeviceToDevice, c->stream));
+      hipEvent_t e = nullptr;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      streams.ev.push_back(e);
+      HIPCHK(hipEventRecord(e, c->stream));
     }
   }
-  HIPCHK(hipStreamSynchronize(c->stream));  // the group streams start after the set-up
   HIPCHK(hipHostMalloc((void**)&streams.hflags, sizeof(int) * total_items, hipHostMallocDefault));
   {
     int off = 0;
@@ -400,7 +436,8 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
         hipStream_t q = nullptr;
         HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
         streams.own.push_back(q);
-        R.s = q;  // phase 1 ended with a synchronisation of the solve's stream
+        R.s = q;
+        HIPCHK(hipStreamWaitEvent(q, streams.ev[&R - runs.data()], 0));  // the group's set-up
       } else {
         R.s = streams.main;
       }
@@ -416,27 +453,21 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
       if (R.g.hflags[s]) return true;
     return false;
   };
-  for (GroupRun& R : runs) copy_flags(R);
-  for (GroupRun& R : runs) {
-    HIPCHK(hipStreamSynchronize(R.s));
-    R.done = !any_active(R) || o->max_cycles < 1;
-  }
   for (bool pending = true; pending;) {
     pending = false;
     for (GroupRun& R : runs) {
       if (R.done) continue;
       c->stream = R.s;
       S.ng = R.g.n;
-      // the first iteration runs eagerly (it also sizes every workspace), the rest replay its graph
-      if (R.it == 0) {
-        batch_iteration(c, R.g, S, o->tol);
-      } else {
-        if (!R.graph && c->batch_graph) {
+      // one iteration captured as a graph (batch_alloc sized every buffer it uses) and replayed
+      if (c->batch_graph) {
+        if (!R.graph) {
           R.graph.reset(new Replay(c));
           R.graph->capture([&] { batch_iteration(c, R.g, S, o->tol); });
         }
-        if (R.graph) R.graph->launch();
-        else batch_iteration(c, R.g, S, o->tol);
+        R.graph->launch();
+      } else {
+        batch_iteration(c, R.g, S, o->tol);
       }
       ++R.it;
       copy_flags(R);

@@ -140,9 +140,14 @@ template <int L> __device__ __forceinline__ void point_of(i64 p, int n, int& i, 
 }
 
 template <class T, int TO>
-__global__ void k_relayout(T* __restrict__ dst, const T* __restrict__ src, int n, int comps) {
+__global__ void k_relayout(T* __restrict__ dst, const T* __restrict__ src, int n, int comps, int batched = 0) {
   // TO = 1: column-major -> sub-lattice; TO = 0: sub-lattice -> column-major; comps fields of n^3
+  // (batched: grid y of gridDim.y at y comps n^3)
   const i64 n3 = (i64)n * n * n;
+  if (batched) {
+    dst += (i64)blockIdx.y * comps * n3;
+    src += (i64)blockIdx.y * comps * n3;
+  }
   const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n3 * comps) return;
   const int f = (int)(p / n3);
@@ -157,10 +162,17 @@ __global__ void k_relayout(T* __restrict__ dst, const T* __restrict__ src, int n
 // Metric G = rho_box * adj(A)/det (mmh/gauss1.f90:1128-1205, cbajraw, triad branch) or
 // adj(A) adj(A)^T / det (lsquare, 1173-1181); A = I + Hess(def)/dx^2; det <= 1e-5 -> identity.
 // def and u are column-major; G is 6 fields (G11, G12, G13, G22, G23, G33) in layout L.
+// Batched (gridDim.y = B, batched = 1): grid y at def + y n^3, u + y nu n^3, G + y 6 n^3.
 template <class T, int L>
 __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __restrict__ def, const T* __restrict__ u,
-                                                int nu, int n, T dx, int lsquare) {
+                                                int nu, int n, T dx, int lsquare, int batched = 0) {
   const i64 n3 = (i64)n * n * n;
+  if (batched) {
+    const i64 y = blockIdx.y;
+    G += y * 6 * n3;
+    def += y * n3;
+    u += y * nu * n3;
+  }
   const i64 pp = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (pp >= n3) return;
   // threads walk the column-major inputs in order; the store goes to layout L
@@ -572,14 +584,13 @@ __global__ void k_cg_dir_b(double* __restrict__ x, double* __restrict__ p, const
   p[y * n + i] = z[y * n + i] + beta[y] * pi;
 }
 
-// Batched set-up, one thread per grid: rn = max|rhs| of grid y (ssum[8y + 3], written by its
-// set-up) gives the residual scale rden = rn (1 where rn = 0) and the active flag rn > 0; the
-// iteration count starts at 0.
-__global__ void k_batch_flags(int B, const double* __restrict__ ssum, double* __restrict__ rden, int* __restrict__ active,
+// Batched set-up, one thread per grid: rn = max|rhs| of grid y (rnmax[y]) gives the residual
+// scale rden = rn (1 where rn = 0) and the active flag rn > 0; the iteration count starts at 0.
+__global__ void k_batch_flags(int B, const double* __restrict__ rnmax, double* __restrict__ rden, int* __restrict__ active,
                               int* __restrict__ iters) {
   const int y = blockIdx.x * blockDim.x + threadIdx.x;
   if (y >= B) return;
-  const double rn = ssum[8 * y + 3];
+  const double rn = rnmax[y];
   rden[y] = rn > 0 ? rn : 1.0;
   active[y] = rn > 0 ? 1 : 0;
   iters[y] = 0;
@@ -688,8 +699,13 @@ __device__ __forceinline__ T box27(const T* __restrict__ u, int st, int n, int i
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_restrict27(T* __restrict__ half, const T* __restrict__ full, int n, int stride) {
+__global__ __launch_bounds__(256) void k_restrict27(T* __restrict__ half, const T* __restrict__ full, int n, int stride,
+                                                    int batched = 0) {
   const int h = n >> 1;
+  if (batched) {  // grid y of gridDim.y: half at y (n/2)^3, full at y stride n^3
+    half += (i64)blockIdx.y * h * h * h;
+    full += (i64)blockIdx.y * stride * n * n * n;
+  }
   const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
   const int i = (int)(q % h), j = (int)((q / h) % h), k = (int)(q / ((i64)h * h));
@@ -758,7 +774,11 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __r
 
 // Unclamped sqrt(g) of the mesh (mmh/gauss1.f90:1283-1295, rgzerosum).
 template <class T>
-__global__ __launch_bounds__(256) void k_det(T* __restrict__ det, const T* __restrict__ def, int n, T dx) {
+__global__ __launch_bounds__(256) void k_det(T* __restrict__ det, const T* __restrict__ def, int n, T dx, int batched = 0) {
+  if (batched) {  // grid y of gridDim.y at y n^3
+    det += (i64)blockIdx.y * n * n * n;
+    def += (i64)blockIdx.y * n * n * n;
+  }
   const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (i64)n * n * n) return;
   const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((i64)n * n));
@@ -782,6 +802,33 @@ __global__ void k_rgz_update(T* __restrict__ arr, const T* __restrict__ det, con
   if (p >= n3) return;
   const T dfact = (T)(sums[1] / sums[0]);
   arr[p] = arr[p] - dfact * det[p];
+}
+
+// k_rgz_update for gridDim.y = B grids of n3 values, grid y with its sums dsum[y] and asum[y].
+template <class T>
+__global__ void k_rgz_update_b(T* __restrict__ arr, const T* __restrict__ det, const double* __restrict__ dsum,
+                               const double* __restrict__ asum, i64 n3) {
+  const i64 y = blockIdx.y;
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n3) return;
+  const T dfact = (T)(asum[y] / dsum[y]);
+  arr[y * n3 + p] = arr[y * n3 + p] - dfact * det[y * n3 + p];
+}
+
+// dst (gridDim.y = B grids of n values side by side) from B separate arrays: dst[y n + p] =
+// src[y][p stride] (stride > 1 picks one component of an interleaved array).
+template <class T>
+__global__ void k_gather(T* __restrict__ dst, const T* const* __restrict__ src, i64 n, int stride) {
+  const i64 y = blockIdx.y;
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) dst[y * n + p] = src[y][p * stride];
+}
+
+// The inverse of k_gather with stride 1: dst[y][p] = src[y n + p].
+template <class T> __global__ void k_scatter(T* const* __restrict__ dst, const T* __restrict__ src, i64 n) {
+  const i64 y = blockIdx.y;
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) dst[y][p] = src[y * n + p];
 }
 
 // zerosum (mmh/gauss1.f90:1450-1460) or, when the level's divergence guard fired, szero.

@@ -109,10 +109,15 @@ __device__ __forceinline__ void rap19_points(double (&acc)[27], const T* __restr
 // Level 1 from the fine 19-point operator: one thread per coarse point q (column-major coarse
 // order), A1 (27 fields of (n/2)^3, sub-lattice layout of the coarse level).
 template <class T>
-__global__ __launch_bounds__(256) void k_rap19(double* __restrict__ A1, const T* __restrict__ G, int n, double di2) {
+__global__ __launch_bounds__(256) void k_rap19(double* __restrict__ A1, const T* __restrict__ G, int n, double di2,
+                                               int batched = 0) {
   const int h = n >> 1;
   const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const i64 h3 = (i64)h * h * h;
+  if (batched) {  // grid y of gridDim.y: A1 at y 27 (n/2)^3, G at y 6 n^3
+    A1 += (i64)blockIdx.y * 27 * h3;
+    G += (i64)blockIdx.y * 6 * ((i64)n * n * n);
+  }
   if (q >= h3) return;
   int ci, cj, ck;
   point_of<0>(q, h, ci, cj, ck);
@@ -145,10 +150,15 @@ __device__ __forceinline__ void rap27_points(double (&acc)[27], const double* __
 }
 
 // Level l+1 from the 27-point level l (size n): one thread per coarse point.
-__global__ __launch_bounds__(256) void k_rap27(double* __restrict__ Ac, const double* __restrict__ A, int n) {
+__global__ __launch_bounds__(256) void k_rap27(double* __restrict__ Ac, const double* __restrict__ A, int n,
+                                               int batched = 0) {
   const int h = n >> 1;
   const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const i64 h3 = (i64)h * h * h;
+  if (batched) {  // grid y of gridDim.y: Ac at y 27 (n/2)^3, A at y 27 n^3
+    Ac += (i64)blockIdx.y * 27 * h3;
+    A += (i64)blockIdx.y * 27 * ((i64)n * n * n);
+  }
   if (q >= h3) return;
   int ci, cj, ck;
   point_of<0>(q, h, ci, cj, ck);
