@@ -25,33 +25,34 @@ struct BatchGroup {
   // scratch of the per-grid reductions, and per-grid scalars
   double *part, *part2, *bm, *bs, *bs2;
   int* hflags = nullptr;  // pinned host copy of active
+  int* act0 = nullptr;    // active at the start of the current iteration: the grids its kernels work on
   double* ssum = nullptr;  // set-up sums: det sums, rhs sums, max|rhs| (B each)
   double *sums, *rz, *pq, *rmax, *rden, *rznew, *zro, *alpha, *beta, *relres;
   int *active, *iters;
   std::string tag;
 };
 
-void batch_reduce(mmh_ctx* c, BatchGroup& g, const double* x, i64 n, int op, double* dst) {
+void batch_reduce(mmh_ctx* c, BatchGroup& g, const double* x, i64 n, int op, double* dst, const int* act = nullptr) {
   // per grid: the staging of reduce() (blocks of RB*RPT, halving tree), grids side by side
   const i64 ch = (i64)RB * RPT;
   i64 nb = (n + ch - 1) / ch;
   const unsigned B = g.B;
   if (nb <= 1) {
-    if (op == 0) k_reduce_stage<double, 0><<<dim3(1, B), RB, 0, c->stream>>>(dst, x, n, n);
-    else k_reduce_stage<double, 1><<<dim3(1, B), RB, 0, c->stream>>>(dst, x, n, n);
+    if (op == 0) k_reduce_stage<double, 0><<<dim3(1, B), RB, 0, c->stream>>>(dst, x, n, n, act);
+    else k_reduce_stage<double, 1><<<dim3(1, B), RB, 0, c->stream>>>(dst, x, n, n, act);
     LAUNCHCHK();
     return;
   }
   double* a = g.part;
   double* bb = g.part2;
-  if (op == 0) k_reduce_stage<double, 0><<<dim3((unsigned)nb, B), RB, 0, c->stream>>>(a, x, n, n);
-  else k_reduce_stage<double, 1><<<dim3((unsigned)nb, B), RB, 0, c->stream>>>(a, x, n, n);
+  if (op == 0) k_reduce_stage<double, 0><<<dim3((unsigned)nb, B), RB, 0, c->stream>>>(a, x, n, n, act);
+  else k_reduce_stage<double, 1><<<dim3((unsigned)nb, B), RB, 0, c->stream>>>(a, x, n, n, act);
   LAUNCHCHK();
   while (nb > 1) {
     const i64 nb2 = (nb + ch - 1) / ch;
     double* out = nb2 == 1 ? dst : bb;
-    if (op == 0) k_reduce_stage<double, 0><<<dim3((unsigned)nb2, B), RB, 0, c->stream>>>(out, a, nb, nb);
-    else k_reduce_stage<double, 1><<<dim3((unsigned)nb2, B), RB, 0, c->stream>>>(out, a, nb, nb);
+    if (op == 0) k_reduce_stage<double, 0><<<dim3((unsigned)nb2, B), RB, 0, c->stream>>>(out, a, nb, nb, act);
+    else k_reduce_stage<double, 1><<<dim3((unsigned)nb2, B), RB, 0, c->stream>>>(out, a, nb, nb, act);
     LAUNCHCHK();
     std::swap(a, bb);
     nb = nb2;
@@ -64,7 +65,7 @@ std::pair<int, int> batch_shape(mmh_ctx* c, int m, int B) {
 }
 
 void batch_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, const double* G, int l, int ns, bool reverse,
-                  std::pair<int, int> shape, bool from_zero = false) {
+                  std::pair<int, int> shape, bool from_zero = false, const int* act = nullptr) {
   const int m = g.m[l];
   const i64 h3 = cube(m / 2);
   const double dx2 = g.dx[l] * g.dx[l];
@@ -75,10 +76,11 @@ void batch_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, con
       const int col = reverse ? 7 - cc : cc;
       if (sw == 0 && from_zero)
         k_colour<double, 1, false, true><<<grid, bsz, 0, c->stream>>>(phi, rhs, G, m, dx2, col >> 2, (col >> 1) & 1,
-                                                                      col & 1, cube(m), shape.second, (unsigned)done);
+                                                                      col & 1, cube(m), shape.second, (unsigned)done,
+                                                                      act);
       else
         k_colour<double, 1><<<grid, bsz, 0, c->stream>>>(phi, rhs, G, m, dx2, col >> 2, (col >> 1) & 1, col & 1, cube(m),
-                                                         shape.second);
+                                                         shape.second, 0xFFu, act);
       done |= 1 << col;
     }
   LAUNCHCHK();
@@ -86,7 +88,7 @@ void batch_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, con
 
 // Sweeps of a batched Galerkin level (gal_sweeps for the stack).
 void batch_gal_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs, int l, int ns, bool reverse,
-                      bool from_zero = false) {
+                      bool from_zero = false, const int* act = nullptr) {
   const int m = g.m[l];
   const dim3 grid(nblk(cube(m / 2)), g.B);
   for (int sw = 0; sw < ns; ++sw)
@@ -94,10 +96,10 @@ void batch_gal_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs,
       const int col = reverse ? 7 - cc : cc;
       if (sw == 0 && from_zero)
         k_gal_colour<double, true><<<grid, 256, 0, c->stream>>>(phi, rhs, g.A[l], m, col >> 2, (col >> 1) & 1, col & 1,
-                                                                cube(m), (unsigned)done);
+                                                                cube(m), (unsigned)done, act);
       else
         k_gal_colour<double><<<grid, 256, 0, c->stream>>>(phi, rhs, g.A[l], m, col >> 2, (col >> 1) & 1, col & 1,
-                                                          cube(m));
+                                                          cube(m), 0xFFu, act);
       done |= 1 << col;
     }
   LAUNCHCHK();
@@ -105,7 +107,8 @@ void batch_gal_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs,
 
 // Symmetric V(nu, nu) cycle of the batch from level l, the arithmetic of vcycle<double> with the
 // Krylov schedule (no guard, reverse colour order on the way up, mean removal on every level).
-void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, const Sched& S) {
+// act: grids y with act[y] = 0 are left alone (null: every grid).
+void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, const Sched& S, const int* act = nullptr) {
   const int m = g.m[l];
   const i64 m3 = cube(m);
   double* u = l == 0 ? u0 : g.u[l];
@@ -113,78 +116,83 @@ void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, co
   const auto shape = batch_shape(c, m, g.B);
   if (m == 4) {
     if (g.A[l]) {
-      batch_gal_sweeps(c, g, u, rh, l, S.coarse(), false, c->zero_start);
-      batch_gal_sweeps(c, g, u, rh, l, S.coarse(), S.symmetric);
+      batch_gal_sweeps(c, g, u, rh, l, S.coarse(), false, c->zero_start, act);
+      batch_gal_sweeps(c, g, u, rh, l, S.coarse(), S.symmetric, false, act);
     } else {
-      batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), false, shape, c->zero_start);
-      batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), S.symmetric, shape);
+      batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), false, shape, c->zero_start, act);
+      batch_sweeps(c, g, u, rh, g.G[l], l, S.coarse(), S.symmetric, shape, false, act);
     }
-    batch_reduce(c, g, u, m3, 0, g.sums);
-    k_zerosum_guard<double><<<dim3(1, g.B), 256, 0, c->stream>>>(u, g.sums, nullptr, m3, 0, nullptr, 1);
+    batch_reduce(c, g, u, m3, 0, g.sums, act);
+    k_zerosum_guard<double><<<dim3(1, g.B), 256, 0, c->stream>>>(u, g.sums, nullptr, m3, 0, nullptr, 1, act);
     LAUNCHCHK();
     return;
   }
   const int pre = S.pre(m);
   if (g.A[l]) {
-    batch_gal_sweeps(c, g, u, rh, l, pre, false, c->zero_start);
-    k_gal_resid<double><<<dim3(nblk(m3), g.B), 256, 0, c->stream>>>(g.r[l], u, rh, g.A[l], m, m3);
+    batch_gal_sweeps(c, g, u, rh, l, pre, false, c->zero_start, act);
+    k_gal_resid<double><<<dim3(nblk(m3), g.B), 256, 0, c->stream>>>(g.r[l], u, rh, g.A[l], m, m3, act);
     LAUNCHCHK();
   } else {
-    batch_sweeps(c, g, u, rh, g.G[l], l, pre, false, shape, c->zero_start);
-    op_resid<double>(c, g.r[l], g.bm, u, rh, g.G[l], m, 1.0 / (g.dx[l] * g.dx[l]), false, g.B, 1);
+    batch_sweeps(c, g, u, rh, g.G[l], l, pre, false, shape, c->zero_start, act);
+    op_resid<double>(c, g.r[l], g.bm, u, rh, g.G[l], m, 1.0 / (g.dx[l] * g.dx[l]), false, g.B, 1, act);
   }
   const int nrmax = std::min(4, std::max(1, m / 4));
   const int nrl = pre > nrmax ? (pre % nrmax ? pre % nrmax : nrmax) : pre;
   k_restrict_resid<double, 1><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(g.rhs[l + 1], g.r[l], m,
-                                                                                   nrl % (m / 2), 1);
-  k_fill<double><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(g.u[l + 1], 0.0, cube(m / 2), cube(m / 2));
+                                                                                   nrl % (m / 2), 1, act);
+  k_fill<double><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(g.u[l + 1], 0.0, cube(m / 2), cube(m / 2), act);
   LAUNCHCHK();
-  batch_vcycle(c, g, l + 1, u0, rhs0, S);
-  k_prolong<double, 1><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(u, g.u[l + 1], m, 1, 1);
+  batch_vcycle(c, g, l + 1, u0, rhs0, S, act);
+  k_prolong<double, 1><<<dim3(nblk(cube(m / 2)), g.B), 256, 0, c->stream>>>(u, g.u[l + 1], m, 1, 1, act);
   LAUNCHCHK();
-  if (g.A[l]) batch_gal_sweeps(c, g, u, rh, l, S.post(m), S.symmetric);
-  else batch_sweeps(c, g, u, rh, g.G[l], l, S.post(m), S.symmetric, shape);
-  batch_reduce(c, g, u, m3, 0, g.sums);
-  k_zerosum_guard<double><<<dim3(nblk(m3), g.B), 256, 0, c->stream>>>(u, g.sums, nullptr, m3, 0, nullptr, 1);
+  if (g.A[l]) batch_gal_sweeps(c, g, u, rh, l, S.post(m), S.symmetric, false, act);
+  else batch_sweeps(c, g, u, rh, g.G[l], l, S.post(m), S.symmetric, shape, false, act);
+  batch_reduce(c, g, u, m3, 0, g.sums, act);
+  k_zerosum_guard<double><<<dim3(nblk(m3), g.B), 256, 0, c->stream>>>(u, g.sums, nullptr, m3, 0, nullptr, 1, act);
   LAUNCHCHK();
 }
 
 // z.a and z.b of every grid in one pass over z (k_dot2, as the single solve's dot products).
 void batch_dot2(mmh_ctx* c, BatchGroup& g, const double* z, const double* a, const double* b, double* dst_a,
-                double* dst_b) {
+                double* dst_b, const int* act = nullptr) {
   const i64 n3 = cube(g.n);
   const unsigned nd = nblk(n3, RB * RPT);
-  k_dot2<double><<<dim3(nd, g.B), RB, 0, c->stream>>>(g.bs, g.bs2, z, a, b, n3, nullptr, n3);
+  k_dot2<double><<<dim3(nd, g.B), RB, 0, c->stream>>>(g.bs, g.bs2, z, a, b, n3, nullptr, n3, act);
   LAUNCHCHK();
-  batch_reduce(c, g, g.bs, nd, 0, dst_a);
-  batch_reduce(c, g, g.bs2, nd, 0, dst_b);
+  batch_reduce(c, g, g.bs, nd, 0, dst_a, act);
+  batch_reduce(c, g, g.bs2, nd, 0, dst_b, act);
 }
 
-void batch_precond(mmh_ctx* c, BatchGroup& g, const Sched& S) {
+void batch_precond(mmh_ctx* c, BatchGroup& g, const Sched& S, const int* act = nullptr) {
   const i64 n3 = cube(g.n);
-  k_fill<double><<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.z, 0.0, n3, n3);
+  k_fill<double><<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.z, 0.0, n3, n3, act);
   LAUNCHCHK();
-  batch_vcycle(c, g, 0, g.z, g.cr, S);
+  batch_vcycle(c, g, 0, g.z, g.cr, S, act);
 }
 
-// One flexible-PCG iteration for every grid of the batch (the loop body of pcg()).
+// One flexible-PCG iteration for every grid of the batch (the loop body of pcg()).  The grids
+// that had converged before it (act0[y] = 0, set at the alpha step) are skipped by every kernel:
+// their alpha = beta = 0 would leave x and r unchanged, and nothing reads their other vectors
+// again.  The operator runs before this iteration's alpha step, on the previous iteration's
+// flags (a grid that converged then is computed once more, unused).
 void batch_iteration(mmh_ctx* c, BatchGroup& g, const Sched& S, double tol) {
   const i64 n3 = cube(g.n);
   const unsigned nb = nblk(n3, RB), sb = nblk(g.B, 64);
-  const unsigned ns = op_apply<double>(c, g.q, g.bs, g.p, g.G[0], g.n, 1.0, g.B, 1);
-  batch_reduce(c, g, g.bs, ns, 0, g.pq);
+  const int* act = g.act0;
+  const unsigned ns = op_apply<double>(c, g.q, g.bs, g.p, g.G[0], g.n, 1.0, g.B, 1, act);
+  batch_reduce(c, g, g.bs, ns, 0, g.pq, act);
   k_cg_scalars<<<sb, 64, 0, c->stream>>>(0, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
-                                         g.active, g.iters, g.relres);
-  k_cg_update_b<<<dim3(nb, g.B), RB, 0, c->stream>>>(g.cr, g.ro, g.q, g.alpha, g.bm, n3);
+                                         g.active, g.iters, g.relres, g.act0);
+  k_cg_update_b<<<dim3(nb, g.B), RB, 0, c->stream>>>(g.cr, g.ro, g.q, g.alpha, g.bm, n3, act);
   LAUNCHCHK();
-  batch_reduce(c, g, g.bm, nb, 1, g.rmax);
+  batch_reduce(c, g, g.bm, nb, 1, g.rmax, act);
   k_cg_scalars<<<sb, 64, 0, c->stream>>>(1, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
-                                         g.active, g.iters, g.relres);
-  batch_precond(c, g, S);
-  batch_dot2(c, g, g.z, g.cr, g.ro, g.rznew, g.zro);
+                                         g.active, g.iters, g.relres, g.act0);
+  batch_precond(c, g, S, act);
+  batch_dot2(c, g, g.z, g.cr, g.ro, g.rznew, g.zro, act);
   k_cg_scalars<<<sb, 64, 0, c->stream>>>(2, g.B, 0, tol, g.alpha, g.beta, g.rz, g.pq, g.rmax, g.rden, g.rznew, g.zro,
-                                         g.active, g.iters, g.relres);
-  k_cg_dir_b<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.p, g.z, g.alpha, g.beta, n3);
+                                         g.active, g.iters, g.relres, g.act0);
+  k_cg_dir_b<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.p, g.z, g.alpha, g.beta, n3, act);
   LAUNCHCHK();
 }
 
@@ -257,8 +265,8 @@ void batch_alloc(mmh_ctx* c, BatchGroup& g) {
   g.ssum = wsT<double>(c, t + "setup_sums", 8 * B);
   g.sums = sc; g.rz = sc + B; g.pq = sc + 2 * B; g.rmax = sc + 3 * B; g.rden = sc + 4 * B;
   g.rznew = sc + 5 * B; g.zro = sc + 6 * B; g.alpha = sc + 7 * B; g.beta = sc + 8 * B; g.relres = sc + 9 * B;
-  int* ic = (int*)ws_get(c, t + "flags", sizeof(int) * 2 * B);
-  g.active = ic; g.iters = ic + B;
+  int* ic = (int*)ws_get(c, t + "flags", sizeof(int) * 3 * B);
+  g.active = ic; g.iters = ic + B; g.act0 = ic + 2 * B;
 }
 
 // Level data of item s: the metric of every level from def and u(1) (build_hier's arithmetic),
@@ -412,7 +420,7 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
       const i64 n3 = cube(g.n);
       R.rden.assign(g.B, 1.0);
       batch_setup_group(c, g, R.tab, u, rhs, defpot, ubig, nu);
-      k_batch_flags<<<nblk(g.B, 64), 64, 0, c->stream>>>(g.B, g.ssum + 2 * g.B, g.rden, g.active, g.iters);
+      k_batch_flags<<<nblk(g.B, 64), 64, 0, c->stream>>>(g.B, g.ssum + 2 * g.B, g.rden, g.active, g.iters, g.act0);
       k_zero_inactive<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.active, n3);
       LAUNCHCHK();
       fill<double>(c, g.relres, 0.0, g.B);

@@ -81,7 +81,8 @@ __device__ __forceinline__ void block_max_store(double a, double* out) {
 // Batched launches (gridDim.y = B): item y reads x + y*xstride and writes out + y*gridDim.x.
 template <class T, int OP>
 __global__ __launch_bounds__(RB) void k_reduce_stage(double* __restrict__ out, const T* __restrict__ x, i64 n,
-                                                    i64 xstride = 0) {
+                                                    i64 xstride = 0, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   __shared__ double s[RB];
   const int t = threadIdx.x;
   x += (i64)blockIdx.y * xstride;
@@ -101,7 +102,8 @@ __global__ __launch_bounds__(RB) void k_reduce_stage(double* __restrict__ out, c
   if (t == 0) out[blockIdx.x] = a;
 }
 
-template <class T> __global__ void k_fill(T* __restrict__ a, T v, i64 n, i64 bstride = 0) {
+template <class T> __global__ void k_fill(T* __restrict__ a, T v, i64 n, i64 bstride = 0, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   a += (i64)blockIdx.y * bstride;
   i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) a[q] = v;
@@ -326,7 +328,8 @@ __device__ __forceinline__ void colour_point(T* __restrict__ phi, const T* __res
 template <class T, int L, bool DELTA = false, bool MASK = false>
 __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G, int n,
                                                 T dx2, int kpar, int io, int jo, i64 bstride = 0, int remap = 1,
-                                                unsigned live = 0xFFu) {
+                                                unsigned live = 0xFFu, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const int h = n >> 1;
   phi += (i64)blockIdx.y * bstride;
   rhs += (i64)blockIdx.y * bstride;
@@ -354,7 +357,8 @@ __device__ __forceinline__ T resid_point(const T* __restrict__ phi, const T* __r
 template <class T, int L, bool DELTA = false>
 __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restrict__ bmax, const T* __restrict__ phi,
                                               const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2,
-                                              int batched = 0) {
+                                              int batched = 0, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const i64 n3 = (i64)n * n * n;
   if (batched) {
     const i64 y = blockIdx.y;
@@ -395,7 +399,8 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
 // block b covering elements [b*RB, (b+1)*RB) combined by the halving tree (deterministic).
 template <class T, int L, bool DELTA = false>
 __global__ __launch_bounds__(RB) void k_apply_dot(T* __restrict__ q, double* __restrict__ bsum, const T* __restrict__ p,
-                                                  const T* __restrict__ G, int n, T di2, int batched = 0) {
+                                                  const T* __restrict__ G, int n, T di2, int batched = 0, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   __shared__ double s[RB];
   const i64 n3 = (i64)n * n * n;
   if (batched) {
@@ -466,7 +471,8 @@ template <class TZ> __device__ __forceinline__ TZ z_centred(const TZ* __restrict
 template <class TZ>
 __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __restrict__ b2, const TZ* __restrict__ z,
                                              const double* __restrict__ r, const double* __restrict__ ro, i64 n,
-                                             const double* __restrict__ zsum = nullptr, i64 bstride = 0) {
+                                             const double* __restrict__ zsum = nullptr, i64 bstride = 0, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   __shared__ double s1[RB], s2[RB];
   const int t = threadIdx.x;
   const i64 y = blockIdx.y;
@@ -557,7 +563,8 @@ template <class T> __global__ void k_cg_dir_t(T* __restrict__ p, const T* __rest
 // Batched CG updates: grid y uses alpha[y] / beta[y] (0 once it has converged, which freezes x and r).
 __global__ __launch_bounds__(RB) void k_cg_update_b(double* __restrict__ r, double* __restrict__ rold,
                                                     const double* __restrict__ q, const double* __restrict__ alpha,
-                                                    double* __restrict__ bmax, i64 n) {
+                                                    double* __restrict__ bmax, i64 n, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const i64 y = blockIdx.y;
   r += y * n; rold += y * n; q += y * n;
   const double al = alpha[y];
@@ -575,7 +582,8 @@ __global__ __launch_bounds__(RB) void k_cg_update_b(double* __restrict__ r, doub
 
 // x += alpha p, p = z + beta p per grid (alpha = beta = 0 freezes a converged grid).
 __global__ void k_cg_dir_b(double* __restrict__ x, double* __restrict__ p, const double* __restrict__ z,
-                           const double* __restrict__ alpha, const double* __restrict__ beta, i64 n) {
+                           const double* __restrict__ alpha, const double* __restrict__ beta, i64 n, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const i64 y = blockIdx.y;
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -585,14 +593,15 @@ __global__ void k_cg_dir_b(double* __restrict__ x, double* __restrict__ p, const
 }
 
 // Batched set-up, one thread per grid: rn = max|rhs| of grid y (rnmax[y]) gives the residual
-// scale rden = rn (1 where rn = 0) and the active flag rn > 0; the iteration count starts at 0.
+// scale rden = rn (1 where rn = 0) and the active flags rn > 0; the iteration count starts at 0.
 __global__ void k_batch_flags(int B, const double* __restrict__ rnmax, double* __restrict__ rden, int* __restrict__ active,
-                              int* __restrict__ iters) {
+                              int* __restrict__ iters, int* __restrict__ act0) {
   const int y = blockIdx.x * blockDim.x + threadIdx.x;
   if (y >= B) return;
   const double rn = rnmax[y];
   rden[y] = rn > 0 ? rn : 1.0;
   active[y] = rn > 0 ? 1 : 0;
+  act0[y] = active[y];
   iters[y] = 0;
 }
 
@@ -610,11 +619,12 @@ __global__ void k_cg_scalars(int step, int B, int it, double tol, double* __rest
                              double* __restrict__ rz, const double* __restrict__ pq, const double* __restrict__ rmax,
                              const double* __restrict__ rden, const double* __restrict__ rznew,
                              const double* __restrict__ zro, int* __restrict__ active, int* __restrict__ iters,
-                             double* __restrict__ relres) {
+                             double* __restrict__ relres, int* __restrict__ act0) {
   const int y = blockIdx.x * blockDim.x + threadIdx.x;
   if (y >= B) return;
   if (step == 0) {
     alpha[y] = active[y] ? rz[y] / pq[y] : 0.0;
+    act0[y] = active[y];  // the grids this iteration works on (a grid converging in it still updates x)
   } else if (step == 1) {
     if (active[y]) {
       const double rr = rmax[y] / rden[y];
@@ -672,7 +682,8 @@ __device__ __forceinline__ void restrict_point(T* __restrict__ rc, const T* __re
 
 template <class T, int L>
 __global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0,
-                                                        int batched = 0) {
+                                                        int batched = 0, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const int h = n >> 1;
   if (batched) {
     rc += (i64)blockIdx.y * h * h * h;
@@ -748,7 +759,8 @@ __device__ __forceinline__ void prolong_cell(T* __restrict__ u, const T* __restr
 
 template <class T, int L>
 __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __restrict__ uh, int n, int add,
-                                                 int batched = 0) {
+                                                 int batched = 0, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const i64 pp = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   const int h = n >> 1;
   if (batched) {
@@ -836,7 +848,8 @@ template <class T> __global__ void k_scatter(T* const* __restrict__ dst, const T
 // relaxerr > 1.001*r1 and r1 > 1e-5 (mmh/multigrid.f90:113).  guard_out records the decision.
 template <class T>
 __global__ void k_zerosum_guard(T* __restrict__ a, const double* __restrict__ sum, const double* __restrict__ r1re, i64 n3,
-                                int mode, int* __restrict__ guard_out, int batched = 0) {
+                                int mode, int* __restrict__ guard_out, int batched = 0, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (batched) {  // mode 0 only: item y removes the mean sum[y] / n3 of its grid
     a += (i64)blockIdx.y * n3;

@@ -217,7 +217,8 @@ __device__ __forceinline__ T gal_resid_point(const T* __restrict__ phi, const T*
 template <class T, bool MASK = false>
 __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
                                                     int n, int kpar, int io, int jo, i64 bstride = 0,
-                                                    unsigned live = 0xFFu) {
+                                                    unsigned live = 0xFFu, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const int h = n >> 1;
   phi += (i64)blockIdx.y * bstride;
   rhs += (i64)blockIdx.y * bstride;
@@ -231,7 +232,8 @@ __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T
 // r = rhs - A phi at every point (one thread per point, sub-lattice order); r stored.
 template <class T>
 __global__ __launch_bounds__(256) void k_gal_resid(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
-                                                   const T* __restrict__ A, int n, i64 bstride = 0) {
+                                                   const T* __restrict__ A, int n, i64 bstride = 0, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   r += (i64)blockIdx.y * bstride;
   phi += (i64)blockIdx.y * bstride;
   rhs += (i64)blockIdx.y * bstride;

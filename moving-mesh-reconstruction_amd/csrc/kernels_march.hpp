@@ -29,7 +29,8 @@ namespace mmh {
 template <class T, int MODE, int TI, int TJ>
 __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, double* __restrict__ bres,
                                                       const T* __restrict__ phi, const T* __restrict__ rhs,
-                                                      const T* __restrict__ G, int n, T di2, int KS, int batched) {
+                                                      const T* __restrict__ G, int n, T di2, int KS, int batched, const int* __restrict__ act = nullptr) {
+  if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   constexpr int NT = TI * TJ, RI = TI + 2, RJ = TJ + 2, PL = RI * RJ, NH = PL - NT;
   static_assert(NH <= NT, "one halo point per thread at most");
   static_assert(4 * PL * sizeof(T) >= NT * sizeof(double), "the block reduction reuses the phi ring");

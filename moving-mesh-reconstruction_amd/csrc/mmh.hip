@@ -358,45 +358,47 @@ template <class T, int V> int march_ks(const mmh_ctx* c, int n) {
 // of block results, 0 when it does not apply.
 template <class T, int MODE, int V>
 unsigned march_launch(mmh_ctx* c, T* out, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2, int B,
-                      int batched) {
+                      int batched, const int* act) {
   constexpr int TI = MarchTile<T, V>::TI, TJ = MarchTile<T, V>::TJ;
   const int ks = march_ks<T, V>(c, n);
   if (!ks) return 0;
   const unsigned nb = (unsigned)(cube(n) / ((i64)TI * TJ * ks));
-  k_op_march<T, MODE, TI, TJ><<<dim3(nb, B), TI * TJ, 0, c->stream>>>(out, bres, phi, rhs, G, n, di2, ks, batched);
+  k_op_march<T, MODE, TI, TJ><<<dim3(nb, B), TI * TJ, 0, c->stream>>>(out, bres, phi, rhs, G, n, di2, ks, batched, act);
   LAUNCHCHK();
   return nb;
 }
 template <class T, int MODE>
 unsigned march_op(mmh_ctx* c, T* out, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2, int B,
-                  int batched) {
+                  int batched, const int* act = nullptr) {
   switch (c->march_tile) {
-    case 1: return march_launch<T, MODE, 1>(c, out, bres, phi, rhs, G, n, di2, B, batched);
-    case 2: return march_launch<T, MODE, 2>(c, out, bres, phi, rhs, G, n, di2, B, batched);
-    default: return march_launch<T, MODE, 0>(c, out, bres, phi, rhs, G, n, di2, B, batched);
+    case 1: return march_launch<T, MODE, 1>(c, out, bres, phi, rhs, G, n, di2, B, batched, act);
+    case 2: return march_launch<T, MODE, 2>(c, out, bres, phi, rhs, G, n, di2, B, batched, act);
+    default: return march_launch<T, MODE, 0>(c, out, bres, phi, rhs, G, n, di2, B, batched, act);
   }
 }
 
 // r = rhs - L phi on a sub-lattice level (r may be null) with per-block max|r| into bres, for B
-// grids stored one after another when batched; returns the number of block results per grid.
+// grids stored one after another when batched (act: grids y with act[y] = 0 are skipped); returns
+// the number of block results per grid.
 template <class T>
 unsigned op_resid(mmh_ctx* c, T* r, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2,
-                  bool delta = false, int B = 1, int batched = 0) {
+                  bool delta = false, int B = 1, int batched = 0, const int* act = nullptr) {
   if (!delta)
-    if (const unsigned nb = march_op<T, 0>(c, r, bres, phi, rhs, G, n, di2, B, batched)) return nb;
+    if (const unsigned nb = march_op<T, 0>(c, r, bres, phi, rhs, G, n, di2, B, batched, act)) return nb;
   const unsigned nb = stencil_grid(n, 1);
-  if (delta) k_resid<T, 1, true><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched);
-  else k_resid<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched);
+  if (delta) k_resid<T, 1, true><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched, act);
+  else k_resid<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched, act);
   LAUNCHCHK();
   return nb;
 }
 
 // q = L p with per-block sums of p*q into bres (same conventions as op_resid).
 template <class T>
-unsigned op_apply(mmh_ctx* c, T* q, double* bres, const T* p, const T* G, int n, T di2, int B = 1, int batched = 0) {
-  if (const unsigned nb = march_op<T, 1>(c, q, bres, p, (const T*)nullptr, G, n, di2, B, batched)) return nb;
+unsigned op_apply(mmh_ctx* c, T* q, double* bres, const T* p, const T* G, int n, T di2, int B = 1, int batched = 0,
+                  const int* act = nullptr) {
+  if (const unsigned nb = march_op<T, 1>(c, q, bres, p, (const T*)nullptr, G, n, di2, B, batched, act)) return nb;
   const unsigned nb = stencil_grid(n, 1);
-  k_apply_dot<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(q, bres, p, G, n, di2, batched);
+  k_apply_dot<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(q, bres, p, G, n, di2, batched, act);
   LAUNCHCHK();
   return nb;
 }
