@@ -102,6 +102,29 @@ def test_batch_zero_rhs_item(ctx, probs):
     assert float(items[0][0].abs().max()) == 0.0
 
 
+def test_batch_group_converged_from_the_start(ctx, probs):
+    """A size group whose grids all have a zero right-hand side (converged before any iteration)
+    beside a group that iterates: the first gives zeros with no iteration, the second its single
+    solve's result."""
+    print("case batch all-converged group", flush=True)
+    u0, r0, d0 = probs[2]  # 64^3
+    u1, r1, d1 = probs[0]  # 32^3
+    items = [(torch.ones_like(r0), torch.zeros_like(r0), d0, u0),
+             (torch.ones_like(r0), torch.zeros_like(r0), d0, u0),
+             (torch.zeros_like(r1), r1.clone(), d1, u1)]
+    st, ok = ctx.solve_batch(items, **SOLVE)
+    torch.cuda.synchronize()
+    assert ok
+    for k in (0, 1):
+        assert st[k].cycles == 0 and st[k].converged
+        assert float(items[k][0].abs().max()) == 0.0
+    phi = torch.zeros_like(r1)
+    st1, ok1 = ctx.solve(phi, r1.clone(), d1, u1, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
+                         nu_coarse=8, nu=5, krylov=True)
+    assert ok1 and st[2].cycles == st1.cycles
+    assert np.array_equal(items[2][0].cpu().numpy(), phi.cpu().numpy())
+
+
 @pytest.mark.parametrize("mixed", [0, 1])
 def test_single_solve_graph_replay_equals_eager(ctx, probs, mixed):
     """The single solve replays its PCG preconditioner (one V-cycle, FP64 or FP32) as a captured
