@@ -376,3 +376,41 @@ def test_marching_operator_pcg(ctx, mixed):
     (a, ca), (b, cb) = out
     assert ca == cb
     assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("mixed,galerkin", [(0, 1), (1, 1), (0, 0)])
+@pytest.mark.parametrize("n", [32, 64])
+def test_krylov_solution_solves_the_reference_equation(ctx, n, mixed, galerkin):
+    """The PCG solve has no reference counterpart: what pins it is the equation it solves.  Its
+    result must satisfy the reference operator (the oracle's metric and calcresid residual) to the
+    requested tolerance (the recursive residual reached 1e-10; the true one may differ by rounding),
+    after the same rgzerosum projection of the right-hand side, and it must agree with the oracle's
+    own solution of the same system (reference V-cycles to 1e-8) as far as that residual fixes it."""
+    print("case Krylov solution vs oracle operator n=%d mixed=%d galerkin=%d" % (n, mixed, galerkin), flush=True)
+    rho = field(n, 2.0, 42, np.float64)
+    u = aos(rho)
+    d = smooth_def(n, 1.0, 4, np.float64)
+    rd = dev(np.zeros(n ** 3))
+    ctx.limiter_rhs(rd, dev(d), dev(u))
+    rhs = host(rd)
+    ctx.set_option("galerkin", galerkin)
+    try:
+        ad, bd = dev(np.zeros(n ** 3)), dev(rhs)
+        st, ok = ctx.solve(ad, bd, dev(d), dev(u), tol=1e-10, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
+                           nu_coarse=8, mixed=mixed, nu=5, krylov=True)
+    finally:
+        ctx.set_option("galerkin", 1)
+    assert ok
+    phi, bproj = host(ad), host(bd)
+    G = R.or_metric(d, u, 5, n, 1.0)
+    _, rinf = R.or_residual(phi, bproj, G, n, 1.0)
+    rel = rinf / np.abs(bproj).max()
+    a, b = np.zeros(n ** 3), rhs.copy()
+    rc, rr = R.or_solve(a, b, d, u, 5, n, 1e-8, 100, 0, 2, 2, 16, 0, 2)
+    err = np.abs((phi - phi.mean()) - (a - a.mean())).max() / np.abs(a - a.mean()).max()
+    print("true relres %.3e (recursive %.3e, %d iterations); oracle solve %d cycles, difference %.3e"
+          % (rel, st.relres[st.cycles - 1], st.cycles, len(rr), err), flush=True)
+    assert rel <= 1e-9
+    assert rc == 0
+    assert np.array_equal(b, bproj)
+    assert err <= 1e-5
