@@ -118,6 +118,8 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 1 reaches the iteration count of 2 at 512^3
  *   "coarse_fused" 4 (default), 8, 16, 32 or 0: the preconditioner's levels up to this size run in
  *                 one workgroup (bit-identical)
+ *   "coarse_fused_ref" 16 (default), 4, 8, 32 or 0: the same for the reference V-cycle (multigrid,
+ *                 schedule 0), its residual norms and divergence guard included (bit-identical)
  *   "zero_start"  1 (default) / 0: a V-cycle level's first Gauss-Seidel sweep from u = 0 (every
  *                 level below the finest; the finest too in a preconditioner) loads only the colour
  *                 classes updated earlier in that sweep (same values; less memory traffic)

@@ -8,8 +8,10 @@
 // (they fit in the L2).  Every phase applies the per-point functions of the multi-launch kernels
 // (colour_point, resid_point, restrict_point, prolong_cell) and the sums follow reduce()'s
 // blocked tree order exactly, so the result is bit-identical to the launch-by-launch cycle.
-// Used for the preconditioner of the Krylov solve only (no divergence guard: the mean removal is
-// always mode 0).
+// Two schedules: the Krylov preconditioner's (no guard, mean removal on every level) and the
+// reference V-cycle's (mgtemplate.f90 / mg4, multigrid.f90:94-121), whose levels also record the
+// residual norms r1 (after the pre-sweeps) and relaxerr (after the post-sweeps) and apply the
+// divergence guard, szero instead of zerosum, as k_zerosum_guard does.
 #pragma once
 #include "kernels.hpp"
 #include "kernels_gal.hpp"
@@ -35,8 +37,11 @@ template <class T> struct CoarseCycle {
   int* guard[CLEV_MAX];
   T dx[CLEV_MAX];
   int n[CLEV_MAX];
+  int sw[CLEV_MAX];       // sweeps each way on the level (pre == post)
   int nl;
-  int pre, post, coarse;  // sweeps per level (pre == post) and each way on 4^3
+  int coarse;             // sweeps each way on 4^3
+  int ref;                // reference schedule: residual norms and the divergence guard
+  int top;                // level 0 is the hierarchy's finest level (no guard there)
   int symmetric;          // post-smoothing in reverse colour order
   int mixed;
   double* du;             // FP64 4^3 level (mixed)
@@ -116,77 +121,135 @@ __device__ void cw_sweeps(T* u, const T* rhs, const T* G, const T* A, int n, T d
     }
 }
 
-// u -= sum(u)/n^3 (k_zerosum_guard mode 0) with the sum in reduce()'s order; the sum is also
-// stored where the multi-launch cycle keeps it.
+// Maximum of a over the workgroup (every thread gets it); a maximum does not depend on order.
+__device__ double cw_max(double a, double* wm) {
+  for (int off = 32; off > 0; off >>= 1) a = dmax(a, __shfl_xor(a, off));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = a;
+  __syncthreads();
+  double m = wm[0];
+  for (int v = 1; v < CW / 64; ++v) m = dmax(m, wm[v]);
+  __syncthreads();
+  return m;
+}
+
+// max |rhs - L u| of a level (the residual norm of relax, stored nowhere).
 template <class T>
-__device__ void cw_zerosum(T* u, int n, double* scal, int* guard, double* blk, double* blk2, double (*s)[RB]) {
+__device__ double cw_resid_norm(const T* u, const T* rhs, const T* G, const T* A, int n, T dx, double* wm) {
+  const T di2 = T(1) / (dx * dx);
+  const i64 n3 = cube_d(n);
+  double a = 0.0;
+  for (i64 p = threadIdx.x; p < n3; p += CW) {
+    int i, j, k;
+    point_of<1>(p, n, i, j, k);
+    const T rv = A ? gal_resid_point<T>(u, rhs, A, n, i, j, k) : resid_point<T, 1>(u, rhs, G, n, di2, i, j, k);
+    a = dmax(a, (double)fabs(rv));
+  }
+  return cw_max(a, wm);
+}
+
+// zerosum (k_zerosum_guard): u -= sum(u)/n^3 with the sum in reduce()'s order (stored where the
+// multi-launch cycle keeps it), or u = 0 when the guard of mode 1 (relaxerr > r1) or mode 2
+// (relaxerr > 1.001 r1 and r1 > 1e-5) fires; mode 0 has no guard.
+template <class T>
+__device__ void cw_zerosum(T* u, int n, double* scal, int* guard, int mode, double* blk, double* blk2, double (*s)[RB]) {
   const i64 n3 = cube_d(n);
   const double sum = cw_sum<T>(u, n3, blk, blk2, s);
+  bool g = false;
+  if (mode == 1) g = (T)scal[1] > (T)scal[0];
+  else if (mode == 2) g = ((T)scal[1] > (T)scal[0] * T(1.001) && (T)scal[0] > T(1.e-5));
   if (threadIdx.x == 0) {
     scal[2] = sum;
-    if (guard) *guard = 0;
+    if (guard) *guard = g ? 1 : 0;
   }
   const double m = sum / (double)n3;
-  for (i64 p = threadIdx.x; p < n3; p += CW) u[p] = (T)((double)u[p] - m);
+  for (i64 p = threadIdx.x; p < n3; p += CW) u[p] = g ? T(0) : (T)((double)u[p] - m);
   __syncthreads();
 }
 
-// The 4^3 level: coarse sweeps forward, then forward or reversed, then the mean removal.
+// The 4^3 level: coarse sweeps forward, then forward or reversed (with the reference schedule each
+// followed by the residual norm, r1 then relaxerr), then the mean removal (guard mode 2 when the
+// level is not the finest).
 template <class T>
 __device__ void cw_bottom(T* u, const T* rhs, const T* G, const T* A, T dx, int nc, bool rev, double* scal, int* guard,
-                          double* blk, double* blk2, double (*s)[RB]) {
+                          int ref, int mode, double* blk, double* blk2, double (*s)[RB], double* wm) {
   cw_sweeps<T>(u, rhs, G, A, 4, dx, nc, false);
+  if (ref) {
+    const double r1 = cw_resid_norm<T>(u, rhs, G, A, 4, dx, wm);
+    if (threadIdx.x == 0) scal[0] = r1;
+  }
   cw_sweeps<T>(u, rhs, G, A, 4, dx, nc, rev);
-  cw_zerosum<T>(u, 4, scal, guard, blk, blk2, s);
+  if (ref) {
+    const double re = cw_resid_norm<T>(u, rhs, G, A, 4, dx, wm);
+    if (threadIdx.x == 0) scal[1] = re;
+  }
+  __syncthreads();
+  cw_zerosum<T>(u, 4, scal, guard, mode, blk, blk2, s);
 }
 
 template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCycle<T> C) {
   __shared__ double s[CW / RB][RB];
   __shared__ double blk[64], blk2[64];  // block values of a level's sums (32^3: 16 blocks)
-  // down: pre-smoothing, residual, restriction into the next level's rhs, zero next level's u
+  __shared__ double wm[CW / 64];
+  // down: pre-smoothing, residual (and r1), restriction into the next level's rhs, zero next u
   for (int l = 0; l + 1 < C.nl; ++l) {
     const int n = C.n[l];
-    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], C.A[l], n, C.dx[l], C.pre, false);
+    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], C.A[l], n, C.dx[l], C.sw[l], false);
     const T di2 = T(1) / (C.dx[l] * C.dx[l]);
     const i64 n3 = cube_d(n), h3 = cube_d(n / 2);
+    double rmax = 0.0;
     for (i64 p = threadIdx.x; p < n3; p += CW) {
       int i, j, k;
       point_of<1>(p, n, i, j, k);
-      C.r[l][p] = C.A[l] ? gal_resid_point<T>(C.u[l], C.rhs[l], C.A[l], n, i, j, k)
-                         : resid_point<T, 1>(C.u[l], C.rhs[l], C.G[l], n, di2, i, j, k);
+      const T rv = C.A[l] ? gal_resid_point<T>(C.u[l], C.rhs[l], C.A[l], n, i, j, k)
+                          : resid_point<T, 1>(C.u[l], C.rhs[l], C.G[l], n, di2, i, j, k);
+      C.r[l][p] = rv;
+      rmax = dmax(rmax, (double)fabs(rv));
     }
     __syncthreads();
+    if (C.ref) {
+      rmax = cw_max(rmax, wm);
+      if (threadIdx.x == 0) C.scal[l][0] = rmax;
+    }
+    const int pre = C.sw[l];
     const int nrmax = min(4, max(1, n / 4));
-    const int nrl = C.pre > nrmax ? (C.pre % nrmax ? C.pre % nrmax : nrmax) : C.pre;
+    const int nrl = pre > nrmax ? (pre % nrmax ? pre % nrmax : nrmax) : pre;
     for (i64 q = threadIdx.x; q < h3; q += CW) {
       restrict_point<T, 1>(C.rhs[l + 1], C.r[l], n, nrl % (n / 2), q);
       C.u[l + 1][q] = T(0);
     }
     __syncthreads();
   }
-  // the 4^3 level
+  // the 4^3 level (guard mode 2 in the reference schedule unless it is the finest level)
   const int lb = C.nl - 1;
+  const int mode_b = (C.ref && !(C.top && lb == 0)) ? 2 : 0;
   if (C.mixed) {
     for (int p = threadIdx.x; p < 64; p += CW) {
       C.drhs[p] = (double)C.rhs[lb][p];
       C.du[p] = 0.0;
     }
     __syncthreads();
-    cw_bottom<double>(C.du, C.drhs, C.dG, C.dA, C.ddx, C.coarse, C.symmetric, C.dscal, C.dguard, blk, blk2, s);
+    cw_bottom<double>(C.du, C.drhs, C.dG, C.dA, C.ddx, C.coarse, C.symmetric, C.dscal, C.dguard, C.ref, mode_b, blk,
+                      blk2, s, wm);
     for (int p = threadIdx.x; p < 64; p += CW) C.u[lb][p] = (T)C.du[p];
     __syncthreads();
   } else {
-    cw_bottom<T>(C.u[lb], C.rhs[lb], C.G[lb], C.A[lb], C.dx[lb], C.coarse, C.symmetric, C.scal[lb], C.guard[lb], blk,
-                 blk2, s);
+    cw_bottom<T>(C.u[lb], C.rhs[lb], C.G[lb], C.A[lb], C.dx[lb], C.coarse, C.symmetric, C.scal[lb], C.guard[lb],
+                 C.ref, mode_b, blk, blk2, s, wm);
   }
-  // up: prolongation, post-smoothing, mean removal
+  // up: prolongation, post-smoothing (and relaxerr), mean removal or the guard's szero
   for (int l = C.nl - 2; l >= 0; --l) {
     const int n = C.n[l];
     const i64 h3 = cube_d(n / 2);
     for (i64 pp = threadIdx.x; pp < h3; pp += CW) prolong_cell<T>(C.u[l], C.u[l + 1], n, 1, pp);
     __syncthreads();
-    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], C.A[l], n, C.dx[l], C.post, C.symmetric != 0);
-    cw_zerosum<T>(C.u[l], n, C.scal[l], C.guard[l], blk, blk2, s);
+    cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], C.A[l], n, C.dx[l], C.sw[l], C.symmetric != 0);
+    if (C.ref) {
+      const double re = cw_resid_norm<T>(C.u[l], C.rhs[l], C.G[l], C.A[l], n, C.dx[l], wm);
+      if (threadIdx.x == 0) C.scal[l][1] = re;
+      __syncthreads();
+    }
+    const int mode = (C.ref && !(C.top && l == 0)) ? 1 : 0;
+    cw_zerosum<T>(C.u[l], n, C.scal[l], C.guard[l], mode, blk, blk2, s);
   }
 }
 

@@ -50,6 +50,7 @@ struct mmh_ctx {
   int galerkin = 1;        // Krylov preconditioner: Galerkin coarse operators (option "galerkin")
   int nu_top = 0;          // Krylov preconditioner: sweeps on the finest level, 0 = nu_pre (option "nu_top")
   int coarse_fused = 4;    // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
+  int coarse_fused_ref = 16;  // the reference V-cycle's levels <= this size in one workgroup (option "coarse_fused_ref")
   int zero_start = 1;      // first sweep from u = 0 reads only the colour classes already updated (option "zero_start")
   int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
@@ -528,11 +529,17 @@ void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const T* A, T* r, int n, int nr
 // size set by option "coarse_fused" (4 by default: with the rebuilt 19-point operators one
 // workgroup beats the launches up to 16^3, with the 27-point Galerkin operators only at 4^3; at
 // 32^3 a level's sweeps move megabytes through one CU and the launches win).
+// The reference schedule (kind 0: guard, nu = 4 NG/N) fuses its 19-point levels up to the size set
+// by option "coarse_fused_ref" (16 by default: the 16^3..4^3 levels run 128, 256 and 32+32 sweeps,
+// some 3000 launches per V-cycle otherwise; at 512^3 both take 88-89 ms per cycle, and fusing 32^3
+// as well is slower, one CU then carrying the 32^3 level).
 template <class T> bool coarse_cycle_applies(const mmh_ctx* c, const Hier<T>& H, int l, const Sched& S) {
-  return c->coarse_fused && !S.guard && !S.delta && S.kind == 1 && S.nu_pre == S.nu_post &&
-         !(S.nu_top > 0 && H.n[l] == S.ng) && H.n[l] <= std::min(c->coarse_fused, CTOP_MAX) && H.nlev - l <= CLEV_MAX;
+  if (S.delta || H.nlev - l > CLEV_MAX) return false;
+  if (S.kind == 0) return c->coarse_fused_ref && !H.A[l] && H.n[l] <= std::min(c->coarse_fused_ref, CTOP_MAX);
+  return c->coarse_fused && !S.guard && S.kind == 1 && S.nu_pre == S.nu_post && !(S.nu_top > 0 && H.n[l] == S.ng) &&
+         H.n[l] <= std::min(c->coarse_fused, CTOP_MAX);
 }
-template <class T> CoarseCycle<T> coarse_cycle_of(Hier<T>& H, int l, const Sched& S) {
+template <class T> CoarseCycle<T> coarse_cycle_of(Hier<T>& H, int l, const Sched& S, int ng_top) {
   CoarseCycle<T> C{};
   C.nl = H.nlev - l;
   for (int q = 0; q < C.nl; ++q) {
@@ -545,10 +552,11 @@ template <class T> CoarseCycle<T> coarse_cycle_of(Hier<T>& H, int l, const Sched
     C.dx[q] = H.dx[l + q];
     C.n[q] = H.n[l + q];
     C.A[q] = H.A[l + q];
+    C.sw[q] = S.pre(H.n[l + q]);
   }
-  C.pre = S.nu_pre;
-  C.post = S.nu_post;
-  C.coarse = S.nu_coarse;
+  C.coarse = S.coarse();
+  C.ref = S.guard ? 1 : 0;
+  C.top = H.n[l] == ng_top ? 1 : 0;
   C.symmetric = S.symmetric ? 1 : 0;
   return C;
 }
@@ -559,7 +567,7 @@ template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const 
   const i64 n3 = cube(n);
   double* sc = H.scal + 3 * l;
   if (coarse_cycle_applies(c, H, l, S)) {
-    CoarseCycle<T> C = coarse_cycle_of(H, l, S);
+    CoarseCycle<T> C = coarse_cycle_of(H, l, S, ng_top);
     k_coarse_cycle<T><<<1, CW, 0, c->stream>>>(C);
     LAUNCHCHK();
     return;
@@ -1053,7 +1061,7 @@ template <class T> double read_scalar(mmh_ctx* c, const double* d) {
 void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_top, const Sched& S) {
   const int n = Hf.n[l];
   if (coarse_cycle_applies(c, Hf, l, S)) {
-    CoarseCycle<float> C = coarse_cycle_of(Hf, l, S);
+    CoarseCycle<float> C = coarse_cycle_of(Hf, l, S, ng_top);
     const int lb = Hd.nlev - 1;
     C.mixed = 1;
     C.du = Hd.u[lb];
@@ -1720,6 +1728,10 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->zero_start = value ? 1 : 0;
     } else if (k == "batch_streams") {
       c->batch_streams = value ? 1 : 0;
+    } else if (k == "coarse_fused_ref") {
+      require(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
+              "set_option: coarse_fused_ref is 0 (off) or the largest fused level size, 4 to 32");
+      c->coarse_fused_ref = value;
     } else if (k == "pcg_graph") {
       c->pcg_graph = value ? 1 : 0;
     } else if (k == "mixed_delta") {

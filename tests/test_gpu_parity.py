@@ -414,3 +414,29 @@ def test_krylov_solution_solves_the_reference_equation(ctx, n, mixed, galerkin):
     assert rc == 0
     assert np.array_equal(b, bproj)
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("n,ng", [(32, 32), (32, 64), (64, 64)])
+def test_reference_vcycle_fused_coarse_levels_equal_launches(ctx, dt, n, ng):
+    """The reference V-cycle's coarse levels in one workgroup (option coarse_fused_ref: levels up
+    to 32, 16, 8 or 4) against the launch-by-launch cycle: the same iterate, per-level residual norms
+    r1 and relaxerr, and divergence-guard decisions (the finest level is guarded too when NG > N)."""
+    print("case fused reference V-cycle n=%d ng=%d %s" % (n, ng, dt.__name__), flush=True)
+    rho = field(n, 2.0, 7, dt)
+    u = aos(rho)
+    d = smooth_def(n, 2.0, 4, dt)
+    rhs = (1 - rho).astype(dt)
+    R.or_nsmooth(rhs, n)
+    out = []
+    for cf in (32, 16, 8, 4, 0):
+        ctx.set_option("coarse_fused_ref", cf)
+        try:
+            ad, bd = dev(np.zeros(n ** 3, dt)), dev(rhs)
+            st = ctx.multigrid(ad, bd, dev(d), dev(u), 1.0, 2, ng, nu=5)
+        finally:
+            ctx.set_option("coarse_fused_ref", 16)
+        out.append((host(ad), [(st.r1[l], st.relaxerr[l], st.guard[l]) for l in range(st.nlev)]))
+    for a, s in out[:-1]:
+        assert s == out[-1][1]
+        assert np.array_equal(a, out[-1][0])
