@@ -1045,6 +1045,16 @@ template <class S, class D> __global__ void k_convert(D* __restrict__ dst, const
   const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) dst[q] = (D)src[q];
 }
+// The same over a whole coefficient array (all its fields) in one launch: a grid-stride loop over a
+// bounded grid, so any n fits.
+template <class S, class D> __global__ void k_convert_all(D* __restrict__ dst, const S* __restrict__ src, i64 n) {
+  for (i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (i64)gridDim.x * blockDim.x) dst[q] = (D)src[q];
+}
+template <class S, class D> void convert_all(mmh_ctx* c, D* dst, const S* src, i64 n) {
+  const i64 nb = std::min<i64>((n + 255) / 256, 16384);  // bounded grid; the loop covers the rest
+  k_convert_all<S, D><<<(unsigned)std::max<i64>(nb, 1), 256, 0, c->stream>>>(dst, src, n);
+  LAUNCHCHK();
+}
 __global__ void k_add_f2d(double* __restrict__ u, const float* __restrict__ e, i64 n) {
   const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) u[q] = u[q] + (double)e[q];
@@ -1118,15 +1128,11 @@ void build_fp32(mmh_ctx* c, Hier<float>& F, Hier<double>& H) {
     F.r[l] = wsT<float>(c, t + "r", cube(m));
     F.G[l] = wsT<float>(c, t + "G", 6 * cube(m));
     F.def[l] = nullptr; F.u1[l] = nullptr;
-    for (int f = 0; f < 6; ++f)  // one coefficient field per launch (n^3 <= 2^30 work-items)
-      k_convert<double, float><<<nblk(cube(m)), 256, 0, c->stream>>>(F.G[l] + f * cube(m), H.G[l] + f * cube(m),
-                                                                     cube(m));
+    convert_all<double, float>(c, F.G[l], H.G[l], 6 * cube(m));
     F.A[l] = nullptr;
     if (H.A[l]) {
       F.A[l] = wsT<float>(c, t + "A", 27 * cube(m));
-      for (int f = 0; f < 27; ++f)
-        k_convert<double, float><<<nblk(cube(m)), 256, 0, c->stream>>>(F.A[l] + f * cube(m), H.A[l] + f * cube(m),
-                                                                       cube(m));
+      convert_all<double, float>(c, F.A[l], H.A[l], 27 * cube(m));
     }
     LAUNCHCHK();
   }
