@@ -81,8 +81,8 @@ def cpu_baseline(n_gpu, cycles, seconds_budget=12.0):
 
 
 def traffic_from_profile(n):
-    """HBM bytes per k_colour<double> launch from the committed counter summary, or None."""
-    path = os.path.join(REPO, "profiles", "k_colour_f64_%d_hbm_traffic.txt" % n)
+    """HBM bytes per k_colour_col<double> launch from the committed counter summary, or None."""
+    path = os.path.join(REPO, "profiles", "k_colour_col_f64_%d_hbm_traffic.txt" % n)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -92,7 +92,7 @@ def traffic_from_profile(n):
 
 
 def smoother_only(ctx, torch, defn, u, rhs0, n, sweeps):
-    """k_colour<double> launches (8 per sweep) at n^3, then a plain 1 GiB device-to-device copy whose
+    """k_colour_col<double> launches (8 per sweep) at n^3, then a plain 1 GiB device-to-device copy whose
     FETCH_SIZE / WRITE_SIZE give the counters' scale against known bytes."""
     n3 = n ** 3
     G = torch.zeros(6 * n3, dtype=torch.float64, device="cuda")
@@ -298,7 +298,7 @@ def main():
 
     ctx.release_workspace()
     torch.cuda.empty_cache()
-    # roofline of the dominant kernel (k_colour: one colour class per launch), FP64 and FP32
+    # roofline of the dominant kernel (k_colour_col: one colour class per launch), FP64 and FP32
     G = torch.zeros(6 * n3, **f64)
     ctx.metric(G, defn, u, 1.0, 0, nu=1)
     phi = torch.zeros(n3, **f64)
@@ -350,7 +350,8 @@ def main():
                    "batched_realizations": batch},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(n),
-                     "kernel": "k_colour<double> (one of 8 colour classes of a Gauss-Seidel sweep, %d^3)" % n,
+                     "kernel": "k_colour_col<double> (one of 8 colour classes of a Gauss-Seidel sweep, %d^3, "
+                               "8 points per thread along k)" % n,
                      "alg_bytes_per_launch": alg_bytes_launch, "avg_launch_ms": round(launch_ms, 5)},
         "cpu_baseline": cpu,
     }

@@ -340,6 +340,106 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
   colour_point<T, L, DELTA, MASK>(phi, rhs, G, n, dx2, kpar, io, jo, q, live);
 }
 
+// The colour class of k_colour (sub-lattice layout) with each thread updating S consecutive points
+// of its (i, j) column, k = 2 kk + kpar for kk = S*seg .. S*seg + S-1.  Points of one class never
+// read each other, so the order is free; what the column order buys is reuse: the k+1 values one
+// point loads (phi of the z, yz and xz neighbour classes, G6, G5 and G3 of the z class) are the
+// k-1 values of the next point, carried in registers instead of fetched again.  Each point's
+// arithmetic is stencil19's, term for term and in the same order (the build contracts no
+// multiply-add), so the results are bit-identical to k_colour's, MASK and DELTA forms included.
+template <class T, bool DELTA, bool MASK, int S>
+__global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T* __restrict__ rhs,
+                                                    const T* __restrict__ G, int n, T dx2, int kpar, int io, int jo,
+                                                    unsigned live = 0xFFu) {
+  static_assert(!(DELTA && MASK), "the masked stencil is the plain form");
+  const int h = n >> 1, m = n - 1;
+  const i64 hh = (i64)h * h, h3 = hh * h, n3 = 8 * h3;
+  const i64 q = (i64)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (q >= hh * (h / S)) return;
+  const int qi = (int)(q % h), qj = (int)((q / h) % h), kk0 = (int)(q / hh) * S;
+  const int i = 2 * qi + io, j = 2 * qj + jo;
+  const int xp = ((i + 1) & m) >> 1, xm = ((i - 1) & m) >> 1, yp = ((j + 1) & m) >> 1, ym = ((j - 1) & m) >> 1;
+  // sub-lattice classes (i&1 | (j&1)<<1 | (k&1)<<2) of the point and of its neighbours
+  const int c = io | (jo << 1) | (kpar << 2);
+  const T* G1 = G;
+  const T* G2 = G + n3;
+  const T* G3 = G + 2 * n3;
+  const T* G4 = G + 3 * n3;
+  const T* G5 = G + 4 * n3;
+  const T* G6 = G + 5 * n3;
+  const T* px = phi + (c ^ 1) * h3;
+  const T* py = phi + (c ^ 2) * h3;
+  const T* pz = phi + (c ^ 4) * h3;
+  const T* pxy = phi + (c ^ 3) * h3;
+  const T* pyz = phi + (c ^ 6) * h3;
+  const T* pxz = phi + (c ^ 5) * h3;
+  const i64 ox = (c ^ 1) * h3, oy = (c ^ 2) * h3, oz = (c ^ 4) * h3, oc = (i64)c * h3;
+  // which neighbour classes may be non-zero (live is indexed by colour 4 kpar + 2 io + jo)
+  const unsigned cc = 4u * kpar + 2u * io + jo;
+  const bool fx = !MASK || ((live >> (cc ^ 2u)) & 1u), fy = !MASK || ((live >> (cc ^ 1u)) & 1u);
+  const bool fz = !MASK || ((live >> (cc ^ 4u)) & 1u), fxy = !MASK || ((live >> (cc ^ 3u)) & 1u);
+  const bool fyz = !MASK || ((live >> (cc ^ 5u)) & 1u), fxz = !MASK || ((live >> (cc ^ 6u)) & 1u);
+  auto P = [&](bool on, const T* a, i64 o) { return on ? a[o] : T(0); };
+  auto F = [&](bool on, const T* g, i64 o) { return on ? g[o] : T(0); };
+  auto at = [&](int a, int b, int kq) { return a + (i64)h * (b + (i64)h * kq); };
+  // the k-1 plane values of the first point
+  const int km0 = (((2 * kk0 + kpar) - 1) & m) >> 1;
+  T g6m = G6[oz + at(qi, qj, km0)], pzm = P(fz, pz, at(qi, qj, km0));
+  T pyzpm = P(fyz, pyz, at(qi, yp, km0)), pyzmm = P(fyz, pyz, at(qi, ym, km0));
+  T g5m = F(fyz, G5 + oz, at(qi, qj, km0));
+  T pxzpm = P(fxz, pxz, at(xp, qj, km0)), pxzmm = P(fxz, pxz, at(xm, qj, km0));
+  T g3m = F(fxz, G3 + oz, at(qi, qj, km0));
+  T res[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int kk = kk0 + s;
+    const int kpl = (((2 * kk + kpar) + 1) & m) >> 1;
+    const i64 o = at(qi, qj, kk);
+    const T g6p = G6[oz + at(qi, qj, kpl)], pzp = P(fz, pz, at(qi, qj, kpl));
+    const T pyzpp = P(fyz, pyz, at(qi, yp, kpl)), pyzmp = P(fyz, pyz, at(qi, ym, kpl));
+    const T g5p = F(fyz, G5 + oz, at(qi, qj, kpl));
+    const T pxzpp = P(fxz, pxz, at(xp, qj, kpl)), pxzmp = P(fxz, pxz, at(xm, qj, kpl));
+    const T g3p = F(fxz, G3 + oz, at(qi, qj, kpl));
+    const T b11p = (G1[ox + at(xp, qj, kk)] + G1[oc + o]) / T(2);
+    const T b11m = (G1[ox + at(xm, qj, kk)] + G1[oc + o]) / T(2);
+    const T b22p = (G4[oy + at(qi, yp, kk)] + G4[oc + o]) / T(2);
+    const T b22m = (G4[oy + at(qi, ym, kk)] + G4[oc + o]) / T(2);
+    const T b33p = (g6p + G6[oc + o]) / T(2);
+    const T b33m = (g6m + G6[oc + o]) / T(2);
+    const T tdiag = -(b11p + b11m + b22p + b22m + b33p + b33m);
+    const T pxp = P(fx, px, at(xp, qj, kk)), pxm = P(fx, px, at(xm, qj, kk));
+    const T pyp = P(fy, py, at(qi, yp, kk)), pym = P(fy, py, at(qi, ym, kk));
+    T t1;
+    if constexpr (DELTA) {
+      const T pc = phi[oc + o];
+      t1 = (b11p * (pxp - pc) + b11m * (pxm - pc) + b22p * (pyp - pc) + b22m * (pym - pc) + b33p * (pzp - pc) +
+            b33m * (pzm - pc));
+    } else {
+      t1 = (b11p * pxp + b11m * pxm + b22p * pyp + b22m * pym + b33p * pzp + b33m * pzm);
+    }
+    const T pxypp = P(fxy, pxy, at(xp, yp, kk)), pxymp = P(fxy, pxy, at(xm, yp, kk));
+    const T pxypm = P(fxy, pxy, at(xp, ym, kk)), pxymm = P(fxy, pxy, at(xm, ym, kk));
+    const T t2 = ((pxypp - pxymp) * F(fxy, G2 + oy, at(qi, yp, kk))
+                - (pxypm - pxymm) * F(fxy, G2 + oy, at(qi, ym, kk))
+                + (pxypp - pxypm) * F(fxy, G2 + ox, at(xp, qj, kk))
+                - (pxymp - pxymm) * F(fxy, G2 + ox, at(xm, qj, kk))
+                + (pyzpp - pyzmp) * g5p
+                - (pyzpm - pyzmm) * g5m
+                + (pyzpp - pyzpm) * F(fyz, G5 + oy, at(qi, yp, kk))
+                - (pyzmp - pyzmm) * F(fyz, G5 + oy, at(qi, ym, kk))
+                + (pxzpp - pxzmp) * g3p
+                - (pxzpm - pxzmm) * g3m
+                + (pxzpp - pxzpm) * F(fxz, G3 + ox, at(xp, qj, kk))
+                - (pxzmp - pxzmm) * F(fxz, G3 + ox, at(xm, qj, kk))) / T(4);
+    if constexpr (DELTA) res[s] = phi[oc + o] + (rhs[oc + o] * dx2 - t1 - t2) / tdiag;
+    else res[s] = (rhs[oc + o] * dx2 - t1 - t2) / tdiag;
+    g6m = g6p; pzm = pzp; pyzpm = pyzpp; pyzmm = pyzmp; g5m = g5p; pxzpm = pxzpp; pxzmm = pxzmp; g3m = g3p;
+  }
+  // stores after all loads: the compiler cannot tell that they never alias the other classes' phi
+#pragma unroll
+  for (int s = 0; s < S; ++s) phi[oc + at(qi, qj, kk0 + s)] = res[s];
+}
+
 // Residual rhs - L[phi] at point (i, j, k) (calcresid, mmh/gauss1.f90:683-709).
 template <class T, int L, bool DELTA = false>
 __device__ __forceinline__ T resid_point(const T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G,

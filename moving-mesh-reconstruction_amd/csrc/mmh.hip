@@ -52,6 +52,7 @@ struct mmh_ctx {
   int coarse_fused = 4;    // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
   int coarse_fused_ref = 16;  // the reference V-cycle's levels <= this size in one workgroup (option "coarse_fused_ref")
   int zero_start = 1;      // first sweep from u = 0 reads only the colour classes already updated (option "zero_start")
+  int colour_col = 8;      // colour kernels of levels n >= 128: points per thread along k (option "colour_col": 1 .. 16)
   int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
@@ -447,6 +448,24 @@ template <class T> void wave_sweep(mmh_ctx* c, T* out, const T* in, const T* rhs
   LAUNCHCHK();
 }
 
+// One colour class by k_colour_col: S points per thread along k (the same values as k_colour).
+template <class T>
+void colour_col_launch(mmh_ctx* c, int S, bool masked, bool delta, T* phi, const T* rhs, const T* G, int n, T dx2,
+                       int kp, int io, int jo, unsigned live) {
+  const int h = n / 2;
+  require(S > 1 && h % S == 0, "colour_col: points per thread must divide n/2");
+  const unsigned nb = nblk(cube(h) / S);
+#define COL_LAUNCH(SS)                                                                                           \
+  if (masked) k_colour_col<T, false, true, SS><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo, live); \
+  else if (delta) k_colour_col<T, true, false, SS><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);   \
+  else k_colour_col<T, false, false, SS><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+  if (S == 2) { COL_LAUNCH(2) }
+  else if (S == 4) { COL_LAUNCH(4) }
+  else if (S == 8) { COL_LAUNCH(8) }
+  else { COL_LAUNCH(16) }
+#undef COL_LAUNCH
+}
+
 // from_zero: phi is 0 on entry, so the first sweep's colours read only the classes updated before
 // them in that sweep (k_colour's MASK form; same values).
 template <class T>
@@ -466,11 +485,14 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, i
     }
     ns -= 2 * pairs;
   }
+  const int S = (layout == 1 && n >= 128 && h % c->colour_col == 0) ? c->colour_col : 1;
   for (int s = 0; s < ns; ++s)
     for (int cc = 0, done = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
       const int kp = col >> 2, io = (col >> 1) & 1, jo = col & 1;
-      if (s == 0 && from_zero)
+      if (S > 1 && !(s == 0 && from_zero))
+        colour_col_launch<T>(c, S, false, delta, phi, rhs, G, n, dx2, kp, io, jo, 0xFFu);
+      else if (s == 0 && from_zero)
         k_colour<T, 1, false, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo, 0, 1, (unsigned)done);
       else if (layout == 1 && delta) k_colour<T, 1, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
       else if (layout == 1) k_colour<T, 1><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
@@ -1732,6 +1754,10 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->coarse_fused = value;
     } else if (k == "zero_start") {
       c->zero_start = value ? 1 : 0;
+    } else if (k == "colour_col") {
+      require(value == 1 || value == 2 || value == 4 || value == 8 || value == 16,
+              "set_option: colour_col is 1, 2, 4, 8 or 16");
+      c->colour_col = value;
     } else if (k == "batch_streams") {
       c->batch_streams = value ? 1 : 0;
     } else if (k == "coarse_fused_ref") {

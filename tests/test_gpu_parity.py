@@ -290,6 +290,61 @@ def test_wavefront_sweep_equals_colour_kernels(ctx, dt, n, ns):
 
 
 @pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("S", [2, 4, 8, 16])
+def test_column_colour_kernel_equals_colour_kernel(ctx, dt, S):
+    """The colour kernel with S points per thread along k (option colour_col, levels n >= 128)
+    against one point per thread: bit-identical after 3 sweeps from a non-zero start at 128^3 and
+    256^3, through a reference V-cycle, and (FP64 data) through PCG solves with FP64 and FP32
+    preconditioners, the latter in plain and difference form; the solves' first sweeps from zero
+    take the masked form."""
+    print("case colour_col S=%d %s" % (S, dt.__name__), flush=True)
+    res = {1: [], S: []}
+    for n in (128, 256):
+        rho = field(n, 2.0, 3, dt)
+        u = aos(rho)
+        d = smooth_def(n, 2.0, 4, dt)
+        Gd = dev(np.zeros(6 * n ** 3, dt))
+        ctx.metric(Gd, dev(d), dev(u), 1.0, 0, nu=5)
+        rhs = dev((1 - rho).astype(dt))
+        phi0 = smooth_def(n, 0.5, 9, dt)
+        for cc in (1, S):
+            ctx.set_option("colour_col", cc)
+            try:
+                phi = dev(phi0)
+                ctx.time_sweeps(phi, rhs, Gd, 1.0, 3)
+                res[cc].append(host(phi))
+            finally:
+                ctx.set_option("colour_col", 1)
+    n = 128
+    rho = field(n, 2.0, 7, dt)
+    u = aos(rho)
+    d = smooth_def(n, 2.0, 4, dt)
+    rhs = (1 - rho).astype(dt)
+    R.or_nsmooth(rhs, n)
+    for cc in (1, S):
+        ctx.set_option("colour_col", cc)
+        try:
+            ad, bd = dev(np.zeros(n ** 3, dt)), dev(rhs)
+            ctx.multigrid(ad, bd, dev(d), dev(u), 1.0, 2, n, nu=5)
+            res[cc].append(host(ad))
+            if dt == np.float64:
+                for mixed, delta in ((0, 0), (1, 0), (1, 1)):
+                    ctx.set_option("mixed_delta", delta)
+                    phi, b = dev(np.zeros(n ** 3)), dev(rhs)
+                    st, ok = ctx.solve(phi, b, dev(d), dev(u), tol=1e-8, max_cycles=100, schedule=1, nu_pre=2,
+                                       nu_post=2, nu_coarse=8, mixed=mixed, nu=5, krylov=True)
+                    assert ok
+                    res[cc].append(host(phi))
+                    res[cc].append(np.array([st.relres[i] for i in range(st.cycles)]))
+        finally:
+            ctx.set_option("colour_col", 1)
+            ctx.set_option("mixed_delta", 0)
+    assert len(res[1]) == len(res[S])
+    for a, b in zip(res[1], res[S]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dt", DT)
 def test_wavefront_vcycle_bitexact(ctx, dt):
     """With the wavefront kernel switched on, a 128^3 V-cycle (forward sweeps at 128^3) and a
     symmetric-preconditioner PCG solve (reverse sweeps) give the colour kernels' results."""
