@@ -433,11 +433,16 @@ __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T
                 - (pxzmp - pxzmm) * F(fxz, G3 + ox, at(xm, qj, kk))) / T(4);
     if constexpr (DELTA) res[s] = phi[oc + o] + (rhs[oc + o] * dx2 - t1 - t2) / tdiag;
     else res[s] = (rhs[oc + o] * dx2 - t1 - t2) / tdiag;
+    if constexpr (MASK) phi[oc + o] = res[s];
     g6m = g6p; pzm = pzp; pyzpm = pyzpp; pyzmm = pyzmp; g5m = g5p; pxzpm = pxzpp; pxzmm = pxzmp; g3m = g3p;
   }
-  // stores after all loads: the compiler cannot tell that they never alias the other classes' phi
+  // Stores after all loads: the compiler cannot tell that they never alias the other classes' phi,
+  // and would otherwise hold the next point's loads behind each store.  The masked form stores
+  // point by point: its hoisted loads would need more registers than the kernel can have.
+  if constexpr (!MASK) {
 #pragma unroll
-  for (int s = 0; s < S; ++s) phi[oc + at(qi, qj, kk0 + s)] = res[s];
+    for (int s = 0; s < S; ++s) phi[oc + at(qi, qj, kk0 + s)] = res[s];
+  }
 }
 
 // Residual rhs - L[phi] at point (i, j, k) (calcresid, mmh/gauss1.f90:683-709).

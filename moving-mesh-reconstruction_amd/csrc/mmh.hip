@@ -490,8 +490,8 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, i
     for (int cc = 0, done = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
       const int kp = col >> 2, io = (col >> 1) & 1, jo = col & 1;
-      if (S > 1 && !(s == 0 && from_zero))
-        colour_col_launch<T>(c, S, false, delta, phi, rhs, G, n, dx2, kp, io, jo, 0xFFu);
+      if (S > 1)
+        colour_col_launch<T>(c, S, s == 0 && from_zero, delta, phi, rhs, G, n, dx2, kp, io, jo, (unsigned)done);
       else if (s == 0 && from_zero)
         k_colour<T, 1, false, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo, 0, 1, (unsigned)done);
       else if (layout == 1 && delta) k_colour<T, 1, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
