@@ -120,9 +120,10 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 one workgroup (bit-identical)
  *   "coarse_fused_ref" 16 (default), 4, 8, 32 or 0: the same for the reference V-cycle (multigrid,
  *                 schedule 0), its residual norms and divergence guard included (bit-identical)
- *   "colour_col"  8 (default), 1, 2, 4 or 16: Gauss-Seidel colour launches on levels n >= 128 update
- *                 this many points per thread along k, carrying the k+1 plane neighbours of one
- *                 point to the next (1 = one point per thread; same values, less memory traffic)
+ *   "colour_col"  8 (default), 1, 2, 4 or 16: Gauss-Seidel colour launches on levels
+ *                 n >= "colour_col_min_n" (256 by default) update this many points per thread along
+ *                 k, carrying the k+1 plane neighbours of one point to the next (1 = one point per
+ *                 thread; same values, less memory traffic)
  *   "zero_start"  1 (default) / 0: a V-cycle level's first Gauss-Seidel sweep from u = 0 (every
  *                 level below the finest; the finest too in a preconditioner) loads only the colour
  *                 classes updated earlier in that sweep (same values; less memory traffic)

@@ -52,7 +52,8 @@ struct mmh_ctx {
   int coarse_fused = 4;    // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
   int coarse_fused_ref = 16;  // the reference V-cycle's levels <= this size in one workgroup (option "coarse_fused_ref")
   int zero_start = 1;      // first sweep from u = 0 reads only the colour classes already updated (option "zero_start")
-  int colour_col = 8;      // colour kernels of levels n >= 128: points per thread along k (option "colour_col": 1 .. 16)
+  int colour_col = 8;      // colour kernels of levels n >= colour_col_min_n: points per thread along k (option "colour_col": 1 .. 16)
+  int colour_col_min_n = 256;  // below 256^3 the n^3/(8S) threads are too few to fill the GPU
   int pcg_graph = 1;       // single solve: replay the PCG preconditioner as a captured HIP graph
   int batch_graph = 1;     // batched solve: replay one PCG iteration as a captured HIP graph
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
@@ -485,7 +486,7 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, i
     }
     ns -= 2 * pairs;
   }
-  const int S = (layout == 1 && n >= 128 && h % c->colour_col == 0) ? c->colour_col : 1;
+  const int S = (layout == 1 && n >= c->colour_col_min_n && h % c->colour_col == 0) ? c->colour_col : 1;
   for (int s = 0; s < ns; ++s)
     for (int cc = 0, done = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
@@ -1754,6 +1755,9 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->coarse_fused = value;
     } else if (k == "zero_start") {
       c->zero_start = value ? 1 : 0;
+    } else if (k == "colour_col_min_n") {
+      require(value >= 8, "set_option: colour_col_min_n >= 8");
+      c->colour_col_min_n = value;
     } else if (k == "colour_col") {
       require(value == 1 || value == 2 || value == 4 || value == 8 || value == 16,
               "set_option: colour_col is 1, 2, 4, 8 or 16");

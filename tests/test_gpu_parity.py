@@ -292,12 +292,21 @@ def test_wavefront_sweep_equals_colour_kernels(ctx, dt, n, ns):
 @pytest.mark.parametrize("dt", DT)
 @pytest.mark.parametrize("S", [2, 4, 8, 16])
 def test_column_colour_kernel_equals_colour_kernel(ctx, dt, S):
-    """The colour kernel with S points per thread along k (option colour_col, levels n >= 128)
-    against one point per thread: bit-identical after 3 sweeps from a non-zero start at 128^3 and
-    256^3, through a reference V-cycle, and (FP64 data) through PCG solves with FP64 and FP32
-    preconditioners, the latter in plain and difference form; the solves' first sweeps from zero
-    take the masked form."""
+    """The colour kernel with S points per thread along k (option colour_col; here on every level
+    with n/2 divisible by S, option colour_col_min_n = 8) against one point per thread:
+    bit-identical after 3 sweeps from a non-zero start at 128^3 and 256^3, through a reference
+    V-cycle, and (FP64 data) through PCG solves with FP64 and FP32 preconditioners, the latter in
+    plain and difference form; the solves' first sweeps from zero take the masked form."""
     print("case colour_col S=%d %s" % (S, dt.__name__), flush=True)
+    ctx.set_option("colour_col_min_n", 8)
+    try:
+        _column_colour_cases(ctx, dt, S)
+    finally:
+        ctx.set_option("colour_col_min_n", 256)
+        ctx.set_option("colour_col", 8)
+
+
+def _column_colour_cases(ctx, dt, S):
     res = {1: [], S: []}
     for n in (128, 256):
         rho = field(n, 2.0, 3, dt)
