@@ -131,8 +131,17 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 preconditioning V-cycle as a HIP graph on its second call and replays it after;
  *                 on the null stream the solve then runs on a stream of its own, ordered after
  *                 the work already queued
+ *   "mesh_solve"  0 (default) / 1 / 2: the mesh solve inside calcdefp and the reconstruction
+ *                 iteration (FP64 entry points).  0 is the reference's one V-cycle per outer
+ *                 iteration (mmh/limiter.f90:408); 1 repeats V(2,2) cycles of the reference V-cycle
+ *                 (divergence guard, 32 + 32 sweeps on 4^3) and 2 runs the Krylov solve, both until
+ *                 max|rhs - L u| / max|rhs| <= 10^-"mesh_tol" (MMH_ENOCONV if not reached)
+ *   "mesh_tol"    8 (default): the tolerance exponent of "mesh_solve" 1 and 2
  *   "batch_graph", "batch_autotune", "batch_streams": see mmh_solve_batch_d */
 int mmh_set_option(mmh_ctx* ctx, const char* key, int value);
+/* Current value of an option above, or the read-only "mesh_cycles": the cycles (mesh_solve 1) or
+ * iterations (mesh_solve 2) of the last mesh solve inside calcdefp.  No reference counterpart. */
+int mmh_get_option(mmh_ctx* ctx, const char* key, int* value);
 
 /* ---- solver (device pointers) --------------------------------------------------------------- */
 /* relax (mmh/gauss1.f90:64): nrelax colour sweeps of arr; mod(iopt,8) selects residual output:

@@ -59,6 +59,10 @@ struct mmh_ctx {
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
   int batch_autotune = 1;  // batched solve: pick the colour kernel's launch shape by timing
   int batch_streams = 1;   // batched solve: size groups iterate side by side on streams of their own
+  int mesh_solve = 0;      // calcdefp's mesh solve: 0 one reference V-cycle, 1 V(2,2) cycles to 10^-mesh_tol,
+                           // 2 the Krylov solve to 10^-mesh_tol (option "mesh_solve")
+  int mesh_tol = 8;        // option "mesh_tol": the tolerance exponent of mesh_solve 1 and 2
+  int mesh_cycles = 0;     // cycles (or iterations) of the last calcdefp mesh solve
   std::map<std::string, std::pair<int, int>> tune;  // (grid size, batch) -> (block size, XCD remap)
 };
 
@@ -851,6 +855,32 @@ void limiter_rhs_dev(mmh_ctx* c, T* tmp, const T* def, const T* u, T dtold, T dt
   nsmooth_dev(c, tmp, n, 1);
 }
 
+void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
+               const mmh_solve_opts* o, mmh_solve_stats* st);
+
+// calcdefp's mesh solve to a tolerance instead of the reference's one V-cycle (limiter.f90:408):
+// option "mesh_solve" 1 repeats V(2,2) cycles of the reference V-cycle (divergence guard included,
+// 32 + 32 sweeps on 4^3), 2 runs the Krylov solve; both stop at max|rhs - L u| / max|rhs| <=
+// 10^-mesh_tol.  FP64 only.
+inline void mesh_solve_dev(mmh_ctx* c, double* defp, double* rhs, const double* def, const double* u, int n, int nf,
+                           int ng) {
+  mmh_solve_opts o;
+  std::memset(&o, 0, sizeof(o));
+  o.tol = std::pow(10.0, -c->mesh_tol);
+  o.max_cycles = c->mesh_solve == 1 ? 100 : 256;
+  o.schedule = 1;
+  o.nu_pre = 2; o.nu_post = 2;
+  o.nu_coarse = c->mesh_solve == 1 ? 32 : 4;
+  o.ng = ng;
+  o.krylov = c->mesh_solve == 2;
+  mmh_solve_stats st;
+  solve_dev(c, defp, rhs, def, u, n, nf, &o, &st);
+  c->mesh_cycles = st.cycles;
+}
+inline void mesh_solve_dev(mmh_ctx*, float*, float*, const float*, const float*, int, int, int) {
+  throw MmhError(MMH_EINVAL, "calcdefp: mesh_solve (a solve to tolerance) is FP64 only");
+}
+
 // calcdefp (limiter.f90:17-673) on the device.
 template <class T>
 void calcdefp_dev(mmh_ctx* c, T* defp, T* tmp, T* tmp2, const T* def, const T* u, T dtold, T dtaumesh, int nf, int n, T compressmax,
@@ -866,7 +896,8 @@ void calcdefp_dev(mmh_ctx* c, T* defp, T* tmp, T* tmp2, const T* def, const T* u
   limiter_rhs_dev<T>(c, tmp, def, u, dtold, dtaumesh, nf, n, compressmax, hb);
   fill<T>(c, defp, T(0), n3);
   mmh_mg_stats* mst = st ? &st->mg : nullptr;
-  multigrid_dev<T>(c, defp, tmp, def, u, T(1.), n, nf, 2, ng, mst);
+  if (c->mesh_solve) mesh_solve_dev(c, defp, tmp, def, u, n, nf, ng);
+  else multigrid_dev<T>(c, defp, tmp, def, u, T(1.), n, nf, 2, ng, mst);
   nsmooth_dev(c, defp, n, 2);
   const unsigned nb = nblk(n3);
   T* lim = wsT<T>(c, "limL", n3);
@@ -1782,12 +1813,36 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->wave_min_n = value;
     } else if (k == "batch_graph") {
       c->batch_graph = value ? 1 : 0;
+    } else if (k == "mesh_solve") {
+      require(value >= 0 && value <= 2, "set_option: mesh_solve is 0, 1 or 2");
+      c->mesh_solve = value;
+    } else if (k == "mesh_tol") {
+      require(value >= 1 && value <= 14, "set_option: mesh_tol is 1 to 14");
+      c->mesh_tol = value;
     } else if (k == "batch_autotune") {
       c->batch_autotune = value ? 1 : 0;
       if (!value) c->tune.clear();
     } else {
       throw MmhError(MMH_EINVAL, std::string("set_option: unknown key ") + key);
     }
+  });
+}
+
+int mmh_get_option(mmh_ctx* c, const char* key, int* value) {
+  return guarded(c, [&] {
+    require(key != nullptr && value != nullptr, "get_option: null argument");
+    const std::string k(key);
+    const std::map<std::string, int> v = {
+        {"sweep_impl", c->sweep_impl}, {"march_op", c->march_op}, {"march_tile", c->march_tile},
+        {"galerkin", c->galerkin}, {"nu_top", c->nu_top}, {"coarse_fused", c->coarse_fused},
+        {"zero_start", c->zero_start}, {"colour_col_min_n", c->colour_col_min_n}, {"colour_col", c->colour_col},
+        {"batch_streams", c->batch_streams}, {"coarse_fused_ref", c->coarse_fused_ref}, {"pcg_graph", c->pcg_graph},
+        {"mixed_delta", c->mixed_delta}, {"wave", c->wave}, {"wave_p", c->wave_p}, {"wave_min_n", c->wave_min_n},
+        {"batch_graph", c->batch_graph}, {"batch_autotune", c->batch_autotune}, {"mesh_solve", c->mesh_solve},
+        {"mesh_tol", c->mesh_tol}, {"mesh_cycles", c->mesh_cycles}};
+    auto it = v.find(k);
+    if (it == v.end()) throw MmhError(MMH_EINVAL, std::string("get_option: unknown key ") + key);
+    *value = it->second;
   });
 }
 

@@ -151,6 +151,15 @@ class Context:
     def set_option(self, key, value):
         self.call("mmh_set_option", key.encode(), int(value))
 
+    def get_option(self, key):
+        v = C.c_int(0)
+        self.call("mmh_get_option", key.encode(), C.byref(v))
+        return v.value
+
+    def mesh_cycles(self):
+        """Cycles (or Krylov iterations) of the last mesh solve inside calcdefp (option mesh_solve)."""
+        return self.get_option("mesh_cycles")
+
     def release_workspace(self):
         self.call("mmh_release_workspace")
 

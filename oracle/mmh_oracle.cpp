@@ -617,6 +617,18 @@ template <class T> void rlft3(T* data, T* speq, int n1, int n2, int n3, int isig
 }
 
 // ------------------------------------------------------------------------------------------------
+// The mesh solve inside calcdefp.  The reference runs one V-cycle there (limiter.f90:394-408);
+// mode 1 (builder-defined, for a reconstruction whose every mesh solve reaches a tolerance) runs
+// V-cycles of a fixed schedule with the divergence guard until max|rhs - L u| / max|rhs| <= tol
+// (solve() above).  Set with or_set_mesh_solve.
+struct MeshSolve {
+  int mode = 0;
+  double tol = 1e-8;
+  int nu_pre = 2, nu_post = 2, nu_coarse = 32, maxcyc = 100;
+  int last_cycles = 0;
+};
+MeshSolve g_mesh;
+
 // calcdefp (limiter.f90:17-673).  Eigenvalues of the symmetric 3x3 C = I + H by the trigonometric
 // method (limiter.f90:346-359).
 template <class T>
@@ -762,7 +774,16 @@ int calcdefp(T* defp, T* tmp, T* tmp2, const T* def, const T* u, T dtold1, T dta
   // 4. solve (limiter.f90:394-408)
   std::fill(defp, defp + n3, T(0));
   MgStats* mst = st ? &st->mg : nullptr;
-  int rc = multigrid(defp, tmp, def, u, nfluid, T(1.), n, 2, ng, sumorder, mst);
+  int rc;
+  if (g_mesh.mode == 1) {
+    Sched S;
+    S.kind = 1; S.nu_pre = g_mesh.nu_pre; S.nu_post = g_mesh.nu_post; S.nu_coarse = g_mesh.nu_coarse;
+    std::vector<double> relres(256);
+    rc = solve(defp, tmp, def, u, nfluid, n, g_mesh.tol, g_mesh.maxcyc, S, ng, sumorder, relres.data(),
+               &g_mesh.last_cycles);
+  } else {
+    rc = multigrid(defp, tmp, def, u, nfluid, T(1.), n, 2, ng, sumorder, mst);
+  }
   if (rc) return rc;
   nsmooth(defp, n);
   nsmooth(defp, n);
@@ -1294,6 +1315,12 @@ int or_solve_d(double* u, double* rhs, const double* defpot, const double* ubig,
   S.kind = kind; S.nu_pre = nu_pre; S.nu_post = nu_post; S.nu_coarse = nu_coarse;
   return solve(u, rhs, defpot, ubig, nu, n, tol, maxcyc, S, ng > 0 ? ng : n, so, relres, cycles);
 }
+
+void or_set_mesh_solve(int mode, double tol, int nu_pre, int nu_post, int nu_coarse, int maxcyc) {
+  g_mesh.mode = mode; g_mesh.tol = tol; g_mesh.nu_pre = nu_pre; g_mesh.nu_post = nu_post;
+  g_mesh.nu_coarse = nu_coarse; g_mesh.maxcyc = maxcyc;
+}
+int or_mesh_cycles(void) { return g_mesh.last_cycles; }
 
 void or_lognormal(double* rho, int n, double sigma, double slope, double ks_frac, uint64_t seed) {
   lognormal(rho, n, sigma, slope, ks_frac, seed);

@@ -114,6 +114,16 @@ def or_calcdefp(defp, tmp, tmp2, defn, u, n, dtold=1.0, dtaumesh=1.0, nfluid=5, 
     return st
 
 
+def or_set_mesh_solve(mode, tol=1e-8, nu_pre=2, nu_post=2, nu_coarse=32, max_cycles=100):
+    """calcdefp's mesh solve in the oracle: 0 the reference's one V-cycle, 1 V-cycles of a fixed
+    schedule to the relative residual tol."""
+    oracle().or_set_mesh_solve(mode, C.c_double(tol), nu_pre, nu_post, nu_coarse, max_cycles)
+
+
+def or_mesh_cycles():
+    return oracle().or_mesh_cycles()
+
+
 def or_relaxing(u, defn, defp, n, dt=1.0, nfluid=5):
     rc = getattr(oracle(), "or_relaxing_" + _sfx(u.dtype))(ptr(u), ptr(defn), ptr(defp), _T(u.dtype)(dt), nfluid, n)
     assert rc == 0, rc
@@ -159,6 +169,13 @@ def or_delta(phi, n):
     out = np.zeros(n ** 3, phi.dtype)
     getattr(oracle(), "or_delta_" + _sfx(phi.dtype))(ptr(phi), ptr(out), n)
     return out
+
+
+def or_rgzerosum(a, defn, n, dx=1.0, sumorder=2):
+    """rgzerosum (mmh/gauss1.f90:1219-1417) in place: a -= (sum a / sum det) det; returns (dsum, asum)."""
+    sums = np.zeros(2)
+    getattr(oracle(), "or_rgzerosum_" + _sfx(a.dtype))(ptr(a), ptr(defn), _T(a.dtype)(dx), n, sumorder, ptr(sums))
+    return sums
 
 
 def or_sum(x, sumorder):

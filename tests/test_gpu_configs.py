@@ -1,0 +1,140 @@
+"""The workloads of the measured configurations as GPU tests, checked against the oracle.
+
+- 512^3 and 1024^3 (sigma = 2, seed 42): the first outer iteration's mesh equation solved to 1e-8 by
+  the Krylov solve, FP64 and mixed precision, with the preconditioner bench.py times (Galerkin coarse
+  operators, 1+1 sweeps on the finest level, 2+2 below, 4+4 on 4^3).  The true FP64 residual of each
+  result (the oracle's metric and calcresid operator, mmh/gauss1.f90:683-709,1128-1205, at 512^3) is
+  at most 1e-8 of max|rhs|, the right-hand side projected by the solve is bit-identical to the
+  oracle's rgzerosum (mmh/gauss1.f90:1219-1417), and the two precisions agree within 1e-7.
+- The batched set (40 x 32^3 + 40 x 64^3 + 20 x 128^3, sigma uniform in [0.5, 2], seeds 0..99) in
+  one batched call: every realisation's true residual under the oracle's operator is at most 1e-8,
+  and its projected right-hand side is the oracle's rgzerosum bit for bit.
+"""
+import numpy as np
+import pytest
+
+import refbind as R
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+F64 = dict(dtype=torch.float64, device="cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def workload(ctx, n, sigma, seed):
+    """Density (device generator), def = 0, and the limiter right-hand side of the first outer
+    iteration (calcdefp's rhs stage, mmh/limiter.f90:79-392)."""
+    rho = torch.zeros(n ** 3, **F64)
+    ctx.lognormal(rho, sigma, seed)
+    d = torch.zeros(n ** 3, **F64)
+    u5 = torch.zeros(n ** 3, 5, **F64)
+    u5[:, 0] = rho
+    rhs = torch.zeros(n ** 3, **F64)
+    ctx.limiter_rhs(rhs, d, u5.contiguous().view(-1))
+    del u5
+    torch.cuda.synchronize()
+    # the generator's and limiter's buffers are tens of GiB at 1024^3
+    ctx.release_workspace()
+    torch.cuda.empty_cache()
+    return rho, d, rhs
+
+
+def bench_solve(ctx, phi, rhs, d, rho, mixed, tol=1e-8):
+    ctx.set_option("galerkin", 1)
+    ctx.set_option("nu_top", 1)
+    try:
+        st, ok = ctx.solve(phi, rhs, d, rho, tol=tol, max_cycles=200, schedule=1, nu_pre=2, nu_post=2, nu_coarse=4,
+                           mixed=mixed, nu=1, krylov=True)
+    finally:
+        ctx.set_option("nu_top", 0)
+    return st, ok
+
+
+@pytest.mark.parametrize("n", [512, 1024])
+def test_config3_mesh_solve_to_tolerance(ctx, n):
+    print("case config-3 mesh solve n=%d" % n, flush=True)
+    rho, d, rhs0 = workload(ctx, n, 2.0, 42)
+    out = {}
+    for mixed in (False, True):
+        phi, rhs = torch.zeros(n ** 3, **F64), rhs0.clone()
+        st, ok = bench_solve(ctx, phi, rhs, d, rho, mixed)
+        assert ok
+        relres = st.relres[st.cycles - 1]
+        print("  %s: %d iterations, true relres %.3e" % ("mixed" if mixed else "fp64", st.cycles, relres), flush=True)
+        assert relres <= 1e-8
+        # the device's own residual kernel (calcresid's formula) on the result
+        r = torch.zeros(n ** 3, **F64)
+        ctx.residual(r, phi, rhs, d, rho, nu=1)
+        rinf_dev = float(r.abs().max())
+        bmax = float(rhs.abs().max())
+        assert rinf_dev <= 1e-8 * bmax
+        del r
+        out[mixed] = (phi, rhs)
+        ctx.release_workspace()
+        torch.cuda.empty_cache()
+    (p64, b64), (pmx, bmx) = out[False], out[True]
+    err = float((pmx - p64).abs().max() / p64.abs().max())
+    print("  mixed vs fp64 rel err %.3e" % err, flush=True)
+    assert err <= 1e-7
+    assert torch.equal(b64, bmx)
+    if n > 512:
+        return
+    # the oracle on the host: projected rhs and the true residual of both results
+    rhs_h = host(rhs0)
+    d_h = host(d)
+    R.or_rgzerosum(rhs_h, d_h, n, 1.0, 2)
+    bproj = host(b64)
+    assert np.array_equal(rhs_h, bproj)
+    del rhs_h
+    rho_h = host(rho)
+    G = R.or_metric(d_h, rho_h, 1, n, 1.0)
+    for name, phi in (("fp64", p64), ("mixed", pmx)):
+        _, rinf = R.or_residual(host(phi), bproj, G, n, 1.0)
+        rel = rinf / np.abs(bproj).max()
+        print("  oracle operator: %s true relres %.3e" % (name, rel), flush=True)
+        assert rel <= 1e-8
+
+
+def test_config4_batched_set_against_oracle_operator(ctx):
+    print("case config-4 batched set", flush=True)
+    sizes = [32] * 40 + [64] * 40 + [128] * 20
+    sig = np.random.default_rng(2024).uniform(0.5, 2.0, len(sizes))
+    probs = []
+    for i, (n, s) in enumerate(zip(sizes, sig)):
+        rho = torch.zeros(n ** 3, **F64)
+        ctx.lognormal(rho, float(s), i)
+        u = torch.zeros(n ** 3, 5, **F64)
+        u[:, 0] = rho
+        u = u.contiguous().view(-1)
+        d = torch.zeros(n ** 3, **F64)
+        rhs = torch.zeros(n ** 3, **F64)
+        ctx.limiter_rhs(rhs, d, u)
+        probs.append((u, rhs, d))
+    items = [(torch.zeros_like(r), r.clone(), d, u) for (u, r, d) in probs]
+    ctx.set_option("nu_top", 1)
+    try:
+        st, ok = ctx.solve_batch(items, tol=1e-8, max_cycles=200, nu_pre=2, nu_coarse=4)
+    finally:
+        ctx.set_option("nu_top", 0)
+    assert ok
+    worst = 0.0
+    for i, ((phi, bproj, d, u), (_, rhs0, _)) in enumerate(zip(items, probs)):
+        n = sizes[i]
+        ph, bp, dh, uh = host(phi), host(bproj), host(d), host(u)
+        b = host(rhs0)
+        R.or_rgzerosum(b, dh, n, 1.0, 2)
+        assert np.array_equal(b, bp), "realisation %d: projected rhs differs from the oracle's rgzerosum" % i
+        G = R.or_metric(dh, uh, 5, n, 1.0)
+        _, rinf = R.or_residual(ph, bp, G, n, 1.0)
+        rel = rinf / np.abs(bp).max()
+        worst = max(worst, rel)
+        assert rel <= 1e-8, "realisation %d (n=%d): true relres %.3e" % (i, n, rel)
+        assert st[i].cycles >= 1
+    print("  worst true relres over the set %.3e; iterations %d..%d" % (
+        worst, min(s.cycles for s in st), max(s.cycles for s in st)), flush=True)

@@ -66,6 +66,57 @@ def test_reconstruct_matches_oracle_loop(ctx, dt, n, sigma, seed):
         assert a.lim.ncompress == b.ncompress and a.lim.nexpand == b.nexpand
 
 
+def test_config1_reconstruction_to_tolerance(ctx):
+    """The periodic 64^3 reconstruction from a Cartesian mesh, sigma = 1.0, seed 1234, FP64, limiter
+    on, every mesh solve run to 1e-8 relative residual by V(2,2) cycles of the reference V-cycle
+    (option mesh_solve = 1; 32 + 32 sweeps on 4^3, divergence guard on).  Three outer iterations
+    against the oracle's loop with the same solve: deformation potential, density and defp bit for
+    bit, the same cycle count per solve, and the mesh positions (scalar2gradient) and reconstructed
+    density (defp2delta) of the final potential within 1e-10 (mmh/main.f90:89-110,133-164)."""
+    n, dt, niter = 64, np.float64, 3
+    print("case config-1 reconstruction n=%d" % n, flush=True)
+    rho = R.or_lognormal(n, 1.0, 1234)
+    R.or_set_mesh_solve(1, 1e-8, 2, 2, 32, 100)
+    try:
+        u = np.zeros((n ** 3, 5))
+        u[:, 0] = rho
+        u = u.ravel()
+        d = np.zeros(n ** 3)
+        cyc_o = []
+        for _ in range(niter):
+            p, t, t2 = np.zeros(n ** 3), np.zeros(n ** 3), np.zeros((n + 2) * n * n)
+            st = R.or_calcdefp(p, t, t2, d, u, n, sumorder=2)
+            assert st.ncompress == 0
+            cyc_o.append(R.or_mesh_cycles())
+            d += p
+            R.or_relaxing(u, d, p, n)
+    finally:
+        R.or_set_mesh_solve(0)
+    ctx.set_option("mesh_solve", 1)
+    ctx.set_option("mesh_tol", 8)
+    try:
+        ud, dd, pd = dev(np.concatenate([rho[:, None], np.zeros((n ** 3, 4))], 1).ravel()), dev(np.zeros(n ** 3)), \
+            dev(np.zeros(n ** 3))
+        cyc_d = []
+        for _ in range(niter):
+            ctx.reconstruct(ud, dd, pd, 1)
+            cyc_d.append(ctx.mesh_cycles())
+    finally:
+        ctx.set_option("mesh_solve", 0)
+    print("cycles per mesh solve: oracle %s device %s" % (cyc_o, cyc_d))
+    assert cyc_d == cyc_o
+    assert np.array_equal(host(dd), d)
+    assert np.array_equal(host(ud), u)
+    assert np.array_equal(host(pd), p)
+    # mesh positions and reconstructed density of the converged potential
+    g_o, dl_o = R.or_gradient(d, n), R.or_delta(d, n)
+    g_d, dl_d = dev(np.zeros(3 * n ** 3)), dev(np.zeros(n ** 3))
+    ctx.scalar2gradient(dd, g_d)
+    ctx.defp2delta(dd, dl_d)
+    assert np.abs(host(g_d) - g_o).max() <= 1e-10 * np.abs(g_o).max()
+    assert np.abs(host(dl_d) - dl_o).max() <= 1e-10 * np.abs(dl_o).max()
+
+
 def test_reconstruct_with_compression_limiter(ctx):
     """compressmax = 1.1 makes the compression limiter and its FFT Poisson solve fire in both
     iterations; the FFTs round differently from the oracle's, so the check is relative (FP64)."""

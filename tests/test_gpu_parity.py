@@ -7,6 +7,8 @@ per-level mean removal summing the level arrays in their sub-lattice storage ord
 each with its tolerance in the test: the limiter's eigenvalue gate uses device acos/cos
 (calcdefp with ncompress > 0: rel 1e-12 FP64 / 1e-5 FP32), and the lognormal generator uses device
 log/cos/exp (rel 1e-12)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -504,3 +506,56 @@ def test_reference_vcycle_fused_coarse_levels_equal_launches(ctx, dt, n, ng):
     for a, s in out[:-1]:
         assert s == out[-1][1]
         assert np.array_equal(a, out[-1][0])
+
+
+GUARD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mg_guard_32.npz")
+
+
+@pytest.mark.parametrize("dt", DT)
+@pytest.mark.parametrize("ng", [32, 64])
+@pytest.mark.parametrize("cf", [16, 0])
+def test_multigrid_divergence_guard(ctx, dt, ng, cf):
+    """V-cycles whose divergence guard fires: the second outer iteration of a sigma = 3 field at
+    32^3, where the reference reports 'relaxation diverged' on the 16^3, 8^3 and 4^3 levels
+    (mmh/mgtemplate.f90:46-55; 4^3 through mg4's 1.001 factor and r1 > 1e-5 test,
+    mmh/multigrid.f90:113-119).  With NG = 64 the 32^3 grid is a coarse level and is guarded too.
+    Both forms of the coarse levels (one workgroup up to 16^3, and launch by launch) must give the
+    oracle's iterate, norms and guard decisions bit for bit; with NG = 32 the decisions must also be
+    the levels of the committed reference run."""
+    print("case guard ng=%d coarse_fused_ref=%d %s" % (ng, cf, dt.__name__), flush=True)
+    z = np.load(GUARD, allow_pickle=False)
+    n = int(z["n"])
+    var = "f64" if dt == np.float64 else "f32"
+    if dt == np.float32 and ng == 64:
+        pytest.skip("the FP32 cycle with a guarded 32^3 level overflows to inf in the oracle and the reference")
+    u = aos(z["rho"].astype(dt))
+    d = z["def"].astype(dt)
+    a, b = np.zeros(n ** 3, dt), z["rhs_in"].astype(dt)
+    ad, bd, dd, ud = dev(a), dev(b), dev(d), dev(u)
+    ctx.set_option("coarse_fused_ref", cf)
+    try:
+        for c in range(int(z["cycles_" + var])):
+            st = R.or_multigrid(a, b, d, u, 5, 1.0, n, 2, ng, 2)
+            sd = ctx.multigrid(ad, bd, dd, ud, 1.0, 2, ng, nu=5)
+            assert sd.nlev == st.nlev
+            assert [sd.guard[l] for l in range(sd.nlev)] == [st.guard[l] for l in range(st.nlev)]
+            assert [sd.r1[l] for l in range(sd.nlev)] == [st.r1[l] for l in range(st.nlev)]
+            assert [sd.relaxerr[l] for l in range(sd.nlev)] == [st.relaxerr[l] for l in range(st.nlev)]
+            assert sd.entry_zerosum == st.zerosum_entry
+            fired = sorted(n >> l for l in range(st.nlev) if st.guard[l])
+            if ng == n:
+                assert fired == list(z["diverged_levels_" + var][c])
+            else:
+                assert n in fired
+            assert np.array_equal(host(ad), a)
+            assert np.array_equal(host(bd), b)
+    finally:
+        ctx.set_option("coarse_fused_ref", 16)
+    if ng == n:
+        # the reference sums sequentially, the device in blocked trees: FP32 agrees bit for bit
+        # here, FP64 to the rounding of the mean removals
+        ref = z["phi_out_" + var]
+        if dt == np.float32:
+            assert np.array_equal(a, ref)
+        else:
+            assert np.abs(a - ref).max() <= 1e-12 * np.abs(ref).max()
