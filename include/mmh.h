@@ -158,6 +158,16 @@ int mmh_multigrid_d(mmh_ctx*, double* u, double* rhs, const double* defpot, cons
 int mmh_multigrid_f(mmh_ctx*, float* u, float* rhs, const float* defpot, const float* ubig, float dx, int n, int nu,
                     int iopt, int ng, mmh_mg_stats* st);
 
+/* Allocation-free multigrid: the same V-cycle as mmh_multigrid_d (identical results), for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Every workspace is carved from a
+ * static 32 MiB scratch array that each GPU holds from library load to process end; the call makes
+ * no hipMalloc/hipFree.  n <= 64 (MMH_EINVAL beyond; a 64^3 FP64 cycle needs about 27 MiB).  All
+ * calls on one GPU share that GPU's array, which keeps the last call's contents: calls that overlap
+ * on one GPU (other host threads, other contexts) race. */
+int mmh_multigrid_noalloc_d(mmh_ctx*, double* u, double* rhs, const double* defpot, const double* ubig, double dx,
+                            int n, int nu, int iopt, int ng, mmh_mg_stats* st);
+
 /* solve to tolerance with repeated V-cycles (FP64 or mixed precision); rhs projected in place. */
 int mmh_solve_d(mmh_ctx*, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
                 const mmh_solve_opts* opts, mmh_solve_stats* st);

@@ -64,9 +64,45 @@ struct mmh_ctx {
   int mesh_tol = 8;        // option "mesh_tol": the tolerance exponent of mesh_solve 1 and 2
   int mesh_cycles = 0;     // cycles (or iterations) of the last calcdefp mesh solve
   std::map<std::string, std::pair<int, int>> tune;  // (grid size, batch) -> (block size, XCD remap)
+  struct Arena* arena = nullptr;  // set while an allocation-free entry point runs: ws_get carves from it
 };
 
+// Workspace of the allocation-free entry points: buffers carved in request order from the
+// device's static scratch array; nothing outlives the call.
+struct Arena {
+  unsigned char* base = nullptr;
+  size_t cap = 0, off = 0;
+  std::map<std::string, std::pair<void*, size_t>> ws;
+};
+
+// Scratch memory of the allocation-free entry point (mmh_multigrid_noalloc_d): one array per GPU,
+// placed by the code object loader and held until the process ends.  Host code never passes the
+// array's name to a kernel or a copy: the name is a host-side stand-in, and the device address is
+// taken once per device with hipGetSymbolAddress (scratch_setup) and used instead.
+#define MMH_SCRATCH_BYTES (((size_t)32) << 20)
+static __device__ __align__(256) unsigned char scratch[MMH_SCRATCH_BYTES];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;    // device -> address of that device's scratch array
+std::map<int, hipError_t> g_scratch_rc;  // device -> result of the address lookup
+
+// Called by mmh_create with `device` current: the first context on a device looks the address up.
+void scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_rc.count(device)) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_rc[device] = e;
+  if (e == hipSuccess) g_scratch_ptr[device] = ptr;
+}
+
+void* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  return it == g_scratch_ptr.end() ? nullptr : it->second;
+}
 
 struct MmhError : std::runtime_error {
   int code;
@@ -96,6 +132,19 @@ inline unsigned nblk(i64 n, int b = 256) {
 }
 
 void* ws_get(mmh_ctx* c, const std::string& key, size_t bytes) {
+  if (c->arena) {
+    Arena& A = *c->arena;
+    auto ai = A.ws.find(key);
+    if (ai != A.ws.end() && ai->second.second >= bytes) return ai->second.first;
+    const size_t b = (std::max<size_t>(bytes, 256) + 255) / 256 * 256;
+    if (A.off + b > A.cap)
+      throw MmhError(MMH_EINVAL, "allocation-free entry: the problem needs more than the " +
+                                     std::to_string(A.cap >> 20) + " MiB scratch array");
+    void* p = A.base + A.off;
+    A.off += b;
+    A.ws[key] = {p, bytes};
+    return p;
+  }
   auto it = c->ws.find(key);
   if (it != c->ws.end() && it->second.second >= bytes) return it->second.first;
   if (it != c->ws.end()) {
@@ -112,6 +161,7 @@ void* ws_get(mmh_ctx* c, const std::string& key, size_t bytes) {
 }
 // Frees one cached buffer (after the queued work that may use it).
 void ws_free(mmh_ctx* c, const std::string& key) {
+  if (c->arena) return;  // the arena is dropped as a whole when the call ends
   auto it = c->ws.find(key);
   if (it == c->ws.end()) return;
   if (c->graphs_live) throw MmhError(MMH_EDEVICE, "internal: workspace '" + key + "' freed while a graph is live");
@@ -1739,6 +1789,8 @@ int mmh_create(mmh_ctx** out, int device) {
   if (!out) return MMH_EINVAL;
   int nd = 0;
   if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0 || device < 0 || device >= nd) return MMH_EDEVICE;
+  if (hipSetDevice(device) != hipSuccess) return MMH_EDEVICE;
+  scratch_setup(device);
   mmh_ctx* c = new mmh_ctx();
   c->device = device;
   *out = c;
@@ -1934,6 +1986,33 @@ mmh_ctx* mmh_default_ctx(void) {
   }
 
 DEF_DEV(double, d)
+
+/* multigrid without allocation: the V-cycle of mmh_multigrid_d with every workspace carved from
+ * the device's static scratch array instead of the context's cached buffers. */
+int mmh_multigrid_noalloc_d(mmh_ctx* c, double* u, double* rhs, const double* dp, const double* ub, double dx, int n,
+                            int nu, int iopt, int ng, mmh_mg_stats* st) {
+  return guarded(c, [&] {
+    void* base = scratch_ptr(c->device);
+    if (!base) {
+      std::lock_guard<std::mutex> lk(g_scratch_mu);
+      const hipError_t e = g_scratch_rc.count(c->device) ? g_scratch_rc[c->device] : hipErrorNotInitialized;
+      throw MmhError(MMH_EDEVICE, std::string("multigrid_noalloc: no scratch address for this device: ") +
+                                      hipGetErrorString(e));
+    }
+    require(is_pow2(n) && n <= 64, "multigrid_noalloc: n must be a power of two in [4, 64]");
+    Arena A;
+    A.base = (unsigned char*)base;
+    A.cap = MMH_SCRATCH_BYTES;
+    c->arena = &A;
+    try {
+      multigrid_dev<double>(c, u, rhs, dp, ub, dx, n, nu, iopt, ng, st);
+    } catch (...) {
+      c->arena = nullptr;
+      throw;
+    }
+    c->arena = nullptr;
+  });
+}
 DEF_DEV(float, f)
 
 int mmh_solve_d(mmh_ctx* c, double* u, double* rhs, const double* defpot, const double* ubig, int n, int nu,
