@@ -275,8 +275,8 @@ __global__ void k_rlft_pairs(C2<T>* __restrict__ z, const C2<T>* __restrict__ wt
 
 // Spectral operations on the rlft3 layout: data (n x n x n reals = complex h x n x n) + speq (n x n).
 // mode 0: divide by the 7-point symbol 2cos kx + 2cos ky + 2cos kz - 6 (mmh/limiter.f90:560-575)
-// from host cosine tables; mode 1: multiply by |k|^2 (mmh/reconstruct.f90:225-247);
-// mode 2/3/4: multiply by i k_x, i k_y, i k_z (mmh/gradient.f90:153-206).
+// from host cosine tables; mode 1: multiply by |k|^2 (mmh/reconstruct.f90:14-36);
+// mode 2/3/4: multiply by i k_x, i k_y, i k_z (mmh/gradient.f90:16-26, 34-48, 56-70).
 template <class T>
 __global__ void k_spectral(T* __restrict__ data, T* __restrict__ speq, int n, int mode, const T* __restrict__ cosx,
                            const T* __restrict__ cosy, const T* __restrict__ cosz, T pi) {

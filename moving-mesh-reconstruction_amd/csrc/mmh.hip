@@ -843,7 +843,7 @@ template <class T> void rlft3_dev(mmh_ctx* c, T* data, T* speq, int n, int isign
 // Spectral multiplier pass on (data, speq); mode as in k_spectral.
 template <class T> void spectral(mmh_ctx* c, T* data, T* speq, int n, int mode) {
   const T pi2 = T(3.14159265358979323846264338328);      // limiter.f90:59
-  const T piG = (T)(atan(1.0) * 4.0);                   // gradient.f90:147, reconstruct.f90:220
+  const T piG = (T)(atan(1.0) * 4.0);                   // gradient.f90:10, reconstruct.f90:9
   T* cs = wsT<T>(c, "spec_cos" + std::to_string(n), 3 * (i64)n);
   if (mode == 0) {
     std::vector<T> h(3 * (size_t)n);

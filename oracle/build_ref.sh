@@ -10,6 +10,8 @@
 # Outputs (git-ignored): oracle/_ref/libmmhref_ng<NG>_f32.so   FP32 as shipped (-O2, serial)
 #                        oracle/_ref/libmmhref_ng<NG>_f64.so   -fdefault-real-8 promotion
 #                        oracle/_ref/libmmhref_ng<NG>_f32omp.so FP32, -O3 -fopenmp (CPU baseline)
+#                        oracle/_ref/dropin_ref_ng<NG>        tests/dropin/dropin_driver.F90 linked
+#                                                             against libmmhref_ng<NG>_f32.so
 #                        oracle/_ref/libmmhref_fisher.so      calcfisherinfo.f90 (its calccov,
 #                        calerror and inverse subroutines are called; the program itself is not run)
 set -euo pipefail
@@ -42,6 +44,10 @@ for NG in $NGS; do
     $FC -shared -fPIC $FL $objs -o "$OUT/libmmhref_ng${NG}_$V.so"
     echo "built $OUT/libmmhref_ng${NG}_$V.so"
   done
+  # the reference-style caller of tests/dropin/, linked against this build's objects (FP32 as shipped)
+  $FC -cpp -DNG=$NG -O2 -w "$HERE/../tests/dropin/dropin_driver.F90" -o "$OUT/dropin_ref_ng$NG" \
+    -L "$OUT" -lmmhref_ng${NG}_f32 -Wl,-rpath,'$ORIGIN'
+  echo "built $OUT/dropin_ref_ng$NG"
 done
 # the Fisher-matrix statistics (independent of NG)
 B=$OUT/build/fisher

@@ -68,6 +68,8 @@ def test_config3_mesh_solve_to_tolerance(ctx, n):
         relres = st.relres[st.cycles - 1]
         print("  %s: %d iterations, true relres %.3e" % ("mixed" if mixed else "fp64", st.cycles, relres), flush=True)
         assert relres <= 1e-8
+        ctx.release_workspace()
+        torch.cuda.empty_cache()
         # the device's own residual kernel (calcresid's formula) on the result
         r = torch.zeros(n ** 3, **F64)
         ctx.residual(r, phi, rhs, d, rho, nu=1)

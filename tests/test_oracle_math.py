@@ -188,6 +188,44 @@ def test_lognormal_generator_statistics():
     assert abs(np.var(g) - 4.0) < 1e-9
 
 
+def _shape_ratio(g, n, slope, ks_frac):
+    """Per-bin mean of |FFT g|^2 / (k^slope exp(-(k/k_s)^2)) in 12 log-spaced |k| shells."""
+    G = g.reshape(n, n, n)
+    P = np.abs(np.fft.fftn(G - G.mean())) ** 2
+    f = np.fft.fftfreq(n) * 2 * np.pi
+    kz, ky, kx = np.meshgrid(f, f, f, indexing="ij")
+    kk = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2).ravel()
+    edges = np.geomspace(2 * np.pi / n, np.pi, 13)
+    idx = np.digitize(kk, edges) - 1
+    ok = (idx >= 0) & (idx < 12)
+    shape = kk[ok] ** slope * np.exp(-(kk[ok] / (ks_frac * np.pi)) ** 2)
+    nb = np.bincount(idx[ok], minlength=12)
+    rb = np.bincount(idx[ok], weights=P.ravel()[ok] / shape, minlength=12) / nb
+    kb = np.bincount(idx[ok], weights=kk[ok], minlength=12) / nb
+    return rb, kb, nb
+
+
+@pytest.mark.parametrize("slope", [-1.5, -1.0])
+def test_lognormal_generator_spectrum(slope):
+    """The Gaussian field under the lognormal density has the spectrum the workload names,
+    P(k) ~ k^slope exp(-(k/k_s)^2), k_s = k_Nyquist/2: |FFT log rho|^2 divided by that shape is flat
+    over 12 log-spaced shells from 2 pi/N to pi (each shell's mean within its sampling scatter of the
+    median), and the residual slope of the ratio is 0.  A generator with another slope or k_s fails:
+    the same data divided by the shape of slope - 0.5 must not be flat."""
+    n = 64
+    g = np.log(R.or_lognormal(n, 2.0, 42, slope=slope))
+    rb, kb, nb = _shape_ratio(g, n, slope, 0.5)
+    med = np.median(rb)
+    tol = 5 * np.sqrt(2.0 / nb) + 0.03
+    assert np.all(np.abs(rb / med - 1) <= tol), (rb / med, tol)
+    s = np.polyfit(np.log(kb), np.log(rb), 1, w=np.sqrt(nb))[0]
+    assert abs(s) < 0.05, s
+    wrong, _, _ = _shape_ratio(g, n, slope - 0.5, 0.5)
+    assert not np.all(np.abs(wrong / np.median(wrong) - 1) <= tol)
+    wrong_ks, _, _ = _shape_ratio(g, n, slope, 0.25)
+    assert not np.all(np.abs(wrong_ks / np.median(wrong_ks) - 1) <= tol)
+
+
 def test_sum_order_effect_is_below_1e12():
     """The device sums FP64 reductions as blocked trees (oracle sumorder 2); the reference sums
     sequentially at one thread (sumorder 0).  Over two outer iterations at 64^3 (sigma = 2) the
