@@ -30,54 +30,59 @@ PEAK_HBM_GBS = 8000.0
 B_ALG_WORDS = 5  # phi read, rhs, def, rho-source, phi write per point-sweep
 
 
-def cpu_baseline(n_gpu, cycles, seconds_budget=12.0):
-    """Times the reference Fortran (FP32, OpenMP build under oracle/_ref) V-cycle on a 128^3 sample
-    of the same field recipe and scales it to seconds-to-tolerance at n_gpu^3."""
-    ng = 128
-    so = os.path.join(REPO, "oracle", "_ref", "libmmhref_ng%d_f32omp.so" % ng)
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(n, rho_host, rhs_host, cycles=2):
+    """Times the reference itself, built from its sources at NG = n (oracle/_ref, FP32 as shipped,
+    -O3 -fopenmp -mcmodel=large), for `cycles` V-cycles of its multigrid_ on the same equation the GPU
+    solves (the density and limiter right-hand side of the workload, rounded to FP32), from phi = 0.
+    Returns seconds per reference V-cycle; the GPU's reference V-cycle is timed beside it."""
+    import numpy as np
+    so = os.path.join(REPO, "oracle", "_ref", "libmmhref_ng%d_f32omp.so" % n)
     if not os.path.exists(so):
         return None
     out = {}
 
     def work():
         import ctypes as C
-        import numpy as np
         lib = C.CDLL(so)
         sys.path.insert(0, os.path.join(REPO, "tests"))
         import refbind as R
-        rho = R.or_lognormal(ng, 2.0, 42).astype(np.float32)
-        u = np.zeros((ng ** 3, 5), np.float32)
-        u[:, 0] = rho
+        u = np.zeros((n ** 3, 5), np.float32)
+        u[:, 0] = rho_host
         u = u.ravel()
-        rhs0 = (1 - rho).astype(np.float32)
-        for _ in range(4):
-            R.or_nsmooth(rhs0, ng)
-        d = np.zeros(ng ** 3, np.float32)
-        a = np.zeros(ng ** 3, np.float32)
-        k, t0 = 0, time.perf_counter()
-        while True:
-            rhs = rhs0.copy()
-            R.ref_multigrid(lib, a, rhs, d, u, 5, 1.0, ng)
-            k += 1
-            if time.perf_counter() - t0 > seconds_budget:
-                break
-        dt = (time.perf_counter() - t0) / k
-        out["t"] = dt
-        out["k"] = k
+        d = np.zeros(n ** 3, np.float32)
+        a = np.zeros(n ** 3, np.float32)
+        times = []
+        for _ in range(cycles):
+            rhs = rhs_host.astype(np.float32)
+            t0 = time.perf_counter()
+            R.ref_multigrid(lib, a, rhs, d, u, 5, 1.0, n)
+            times.append(time.perf_counter() - t0)
+        out["t"] = times
 
-    threading.stack_size(1 << 30)
+    threading.stack_size(1 << 30)  # mg512's level arrays are automatic (stack) arrays
     th = threading.Thread(target=work)
     th.start()
     th.join()
     if "t" not in out:
         return None
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    scale = (n_gpu / ng) ** 3 * cycles
-    return {"value": out["t"] * scale, "unit": "s", "cores": threads, "kind": "reference",
-            "sample": "%d V-cycles of the reference multigrid_ (FP32, OpenMP, NG=128 build) on the sigma=2 lognormal "
-                      "field at 128^3, %.3f s per cycle, scaled by (%d/128)^3 and by %d cycles (the FP64 GPU solve's "
-                      "iteration count; the reference's own V-cycle iteration does not reach 1e-8 at this size)"
-                      % (out["k"], out["t"], n_gpu, cycles)}
+    t = sum(out["t"]) / len(out["t"])
+    return {"value": round(t, 3), "unit": "s per reference V-cycle", "cores": threads, "kind": "reference",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "sample": "%d V-cycles of the reference's multigrid_ built from its sources at NG=%d (FP32, flang -O3 "
+                      "-fopenmp -mcmodel=large, OMP_NUM_THREADS=%d) on the workload's density and limiter rhs "
+                      "(rounded to FP32), from phi = 0; per-cycle times %s s" % (
+                          cycles, n, threads, [round(x, 3) for x in out["t"]])}
 
 
 def traffic_from_profile(n):
@@ -124,6 +129,82 @@ def krylov_probe(ctx, torch, defn, u, rhs0, n, iters=2):
         y.copy_(x)
     torch.cuda.synchronize()
     print("krylov-probe: %d^3, %d PCG iterations per precision; copy 1 GiB x3 done" % (n, iters), flush=True)
+
+
+def curved_leg(ctx, torch, rho, args, outer=2):
+    """The mesh equation at a curved mesh: outer iteration outer + 1 of the same field, after `outer`
+    reconstruction iterations (calcdefp, def += defp, relaxing) whose mesh solves ran to the bench
+    tolerance (option mesh_solve = 2), so that the metric's cross terms G12, G13, G23 are live.  FP64
+    and mixed-precision solves are timed as the first-iteration ones.  calcdefp's step is scaled by
+    its dtold argument (the reference's main passes 1, mmh/main.f90:95); where a full step leaves a
+    mesh whose discrete operator the solve reports as not definite, the record says so and the next
+    step size (2, then 4) is used."""
+    import mmh_amd
+    n = args.n
+    n3 = n ** 3
+    f64 = dict(dtype=torch.float64, device="cuda")
+    attempts = []
+    for dtold in (1.0, 2.0, 4.0):
+        u5 = torch.zeros(n3, 5, **f64)
+        u5[:, 0] = rho
+        u5 = u5.view(-1)
+        d, p = torch.zeros(n3, **f64), torch.zeros(n3, **f64)
+        ctx.set_option("mesh_solve", 2)
+        ctx.set_option("mesh_tol", int(round(-__import__("math").log10(args.tol))))
+        ctx.set_option("nu_top", args.nu_top)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        try:
+            st = ctx.reconstruct(u5, d, p, outer, dtold=dtold)
+        except mmh_amd.MmhError as e:
+            attempts.append({"dtold": dtold, "failed": str(e)})
+            print("bench: curved mesh, dtold %g: %s" % (dtold, e), file=sys.stderr, flush=True)
+            del u5, d, p
+            ctx.release_workspace()
+            torch.cuda.empty_cache()
+            continue
+        finally:
+            ctx.set_option("mesh_solve", 0)
+        t_outer = time.perf_counter() - t0
+        rhs0 = torch.zeros(n3, **f64)
+        ctx.limiter_rhs(rhs0, d, u5, dtold1=dtold)
+        u1 = u5.view(n3, 5)[:, 0].contiguous()
+        del u5, p
+        torch.cuda.synchronize()
+        ctx.release_workspace()
+        torch.cuda.empty_cache()
+        print("bench: curved mesh after %d outer iterations, dtold %g (%.1f s)" % (outer, dtold, t_outer),
+              file=sys.stderr, flush=True)
+        res = {}
+        for mixed in (False, True):
+            for _ in range(args.warmup):
+                phi, rhs = torch.zeros(n3, **f64), rhs0.clone()
+                ctx.solve(phi, rhs, d, u1, tol=args.tol, max_cycles=200, schedule=1, nu_pre=args.nu,
+                          nu_post=args.nu, nu_coarse=args.nu_coarse, mixed=mixed, nu=1, krylov=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                phi, rhs = torch.zeros(n3, **f64), rhs0.clone()
+                s2, ok = ctx.solve(phi, rhs, d, u1, tol=args.tol, max_cycles=200, schedule=1, nu_pre=args.nu,
+                                   nu_post=args.nu, nu_coarse=args.nu_coarse, mixed=mixed, nu=1, krylov=True)
+                if not ok:
+                    raise RuntimeError("curved-mesh solve did not converge")
+            torch.cuda.synchronize()
+            res[mixed] = ((time.perf_counter() - t0) / args.steps, s2.cycles, s2.relres[s2.cycles - 1], phi)
+        err = float((res[True][3] - res[False][3]).abs().max() / res[False][3].abs().max())
+        out = {"outer_iterations_before": outer, "dtold": dtold, "attempts": attempts,
+               "def_sigma": round(st[-1].def_sigma, 4), "ncompress": [s.lim.ncompress for s in st],
+               "detmin_detmax_last_limiter": [st[-1].lim.detmin, st[-1].lim.detmax],
+               "fp64_s": round(res[False][0], 4), "fp64_cycles": res[False][1], "final_relres_fp64": res[False][2],
+               "mixed_s": round(res[True][0], 4), "mixed_cycles": res[True][1], "final_relres_mixed": res[True][2],
+               "speedup_mixed_vs_fp64": round(res[False][0] / res[True][0], 3), "mixed_rel_err_vs_fp64": err}
+        del res, d, rhs0, u1
+        ctx.set_option("nu_top", 0)
+        ctx.release_workspace()
+        torch.cuda.empty_cache()
+        return out
+    ctx.set_option("nu_top", 0)
+    return {"outer_iterations_before": outer, "attempts": attempts}
 
 
 def batched_leg(ctx, torch, nu, nu_top, nu_coarse):
@@ -201,6 +282,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweeps", type=int, default=20, help="sweeps timed for the roofline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-realisations leg")
+    ap.add_argument("--no-curved", action="store_true", help="skip the curved-mesh leg")
     ap.add_argument("--krylov-probe", action="store_true",
                     help="only two PCG iterations per precision plus the 1 GiB copy (counter runs)")
     ap.add_argument("--smoother-only", action="store_true",
@@ -292,12 +374,19 @@ def main():
         del phi, rhs
         print("bench: reference V-cycle %.1f ms" % ref_ms, file=sys.stderr, flush=True)
 
+    curved = None
+    if not args.no_curved:
+        curved = curved_leg(ctx, torch, rho, args)
+
     batch = None
     if not args.no_batch:
         batch = batched_leg(ctx, torch, args.nu, args.nu_top, args.nu_coarse)
 
     ctx.release_workspace()
     torch.cuda.empty_cache()
+    # host copies for the CPU leg (the reference's precision is FP32)
+    rho_host = rho.cpu().numpy().astype("float32") if not args.no_cpu else None
+    rhs_host = rhs0.cpu().numpy().astype("float32") if not args.no_cpu else None
     # roofline of the dominant kernel (k_colour_col: one colour class per launch), FP64 and FP32
     G = torch.zeros(6 * n3, **f64)
     ctx.metric(G, defn, u, 1.0, 0, nu=1)
@@ -319,7 +408,10 @@ def main():
     dn = os.open(os.devnull, os.O_WRONLY)
     os.dup2(dn, 1)
     if not args.no_cpu:
-        cpu = cpu_baseline(n, c64)
+        cpu = cpu_baseline(n, rho_host, rhs_host)
+        if cpu and ref_ms:
+            cpu["gpu_reference_vcycle_s"] = round(ref_ms / 1e3, 4)
+            cpu["gpu_vs_cpu_reference_vcycle"] = round(cpu["value"] / (ref_ms / 1e3), 1)
 
     line = {
         "metric": "s to 1e-8 rel residual at 512^3 (mixed vs FP64 GPU); smoother HBM GB/s",
@@ -347,6 +439,7 @@ def main():
                    "reference_vcycle_ms_fp64": ref_ms, "reference_relres_after_%d_cycles" % args.ref_cycles: ref_relres,
                    "sweep_ms_fp64": round(ms64, 4), "sweep_ms_fp32": round(ms32, 4),
                    "smoother_alg_GBs_fp32": round(B_ALG_WORDS * 4 * n3 / (ms32 * 1e-3) / 1e9, 1),
+                   "curved_mesh": curved,
                    "batched_realizations": batch},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(n),

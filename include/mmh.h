@@ -71,6 +71,10 @@ typedef struct {
   int cycles;
   double relres[256];  /* relative residual after each cycle */
   double rhs_norm;
+  /* Krylov solve: iterations where p.Lp >= 0 or z.r >= 0.  L is negative definite on mean-free
+   * fields and so is the V-cycle preconditioner, so both must stay negative; a count > 0 means the
+   * discrete operator (or the preconditioner) lost definiteness, where CG cannot converge. */
+  int nonneg_pLp, nonneg_zr;
 } mmh_solve_stats;
 
 /* Per outer iteration: the diagnostics the reference prints in its iteration loop
@@ -245,6 +249,19 @@ int mmh_coarsen_f(mmh_ctx*, const float* den, int n, int levels, float* out);
  * ng/2 .. ng/2 + nplot - 1, the column's ends moved by the mesh's z displacement from def.
  * Device arrays, Fortran order. */
 int mmh_column_density_f(mmh_ctx*, const float* den, const float* def, int ng, int nplot, float* proj);
+
+/* Mesh picture of gridfigure.f90:41-131 (device pointers, FP32 as the reference).  def: the
+ * deformation potential of a coarse reconstruction (nf^3); den: density on a finer grid (ng^3,
+ * ng a multiple of nf).  pos (2 nf^2): the new (x, y) positions of the coarse nodes in the fine
+ * mid-plane, newposition(1:2, ratio::ratio, ratio::ratio, ng/2) with newposition = index + 0.5 +
+ * trilinearly interpolated cell-corner displacement; proj (ng^2): the column density of cells
+ * ng/2 .. ng/2 + nplot - 1 through the deformed mesh, den scaled to mean 1 first (FP64 total; the
+ * reference divides by an FP32 sum); dsp_slab (2 (nf+1)^2 4, or NULL): the reference's
+ * dsp_proj.dat, the x/y corner displacements of planes nf/2 .. nf/2+3 before its ghost fill.
+ * The ghost layer is filled as the reference does (a copy shifted along all three axes at once).
+ * nplot <= ng/2 - ng/nf keeps every corner read inside the displacement array. */
+int mmh_gridfigure_f(mmh_ctx*, const float* def, int nf, const float* den, int ng, int nplot, float* pos,
+                     float* proj, float* dsp_slab);
 
 /* Binned power spectrum of a real N^3 field (no reference counterpart: the reference reads P(k)
  * from text files, mmh/calcfisherinfo.f90:26-36).  P_b = sum over modes k in bin b of |F(k)|^2 /

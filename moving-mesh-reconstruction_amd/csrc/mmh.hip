@@ -1325,6 +1325,14 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   precond();
   double rz, unused;
   dots(r, r, rz, unused);
+  // definiteness monitor: p.Lp < 0 and z.r < 0 on a negative definite operator and preconditioner
+  int nonneg_pq = 0, nonneg_zr = rz >= 0.0 ? 1 : 0;
+  auto indefinite_note = [&] {
+    if (st) { st->nonneg_pLp = nonneg_pq; st->nonneg_zr = nonneg_zr; }
+    if (!nonneg_pq && !nonneg_zr) return std::string();
+    return " (p.Lp >= 0 in " + std::to_string(nonneg_pq) + " and z.r >= 0 in " + std::to_string(nonneg_zr) +
+           " iterations: the discrete operator or its preconditioner is not definite)";
+  };
   if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(nullptr, p, z32, 0.0, 0.0, n3, zsum, 0);
   else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(nullptr, p, z, 0.0, 0.0, n3, zsum, 0);
   LAUNCHCHK();
@@ -1332,6 +1340,7 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
     const unsigned nq = op_apply<double>(c, q, bs, p, H.G[0], n, 1.0);
     reduce<double>(c, bs, nq, 0, sums + 5);
     const double pq = read_scalar<double>(c, sums + 5);
+    if (pq >= 0.0) ++nonneg_pq;
     const double alpha = rz / pq;
     k_cg_update<<<nb, RB, 0, c->stream>>>(r, ro, q, alpha, bm, n3, r32);
     LAUNCHCHK();
@@ -1343,11 +1352,13 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
       k_axpy<<<nblk(n3), 256, 0, c->stream>>>(x, p, alpha, n3);
       LAUNCHCHK();
       if (rr <= o->tol) break;
-      throw MmhError(MMH_ENOCONV, "solve: tolerance not reached");
+      const std::string note = indefinite_note();
+      throw MmhError(MMH_ENOCONV, "solve: tolerance not reached" + note);
     }
     precond();
     double rz_new, zro;
     dots(r, ro, rz_new, zro);
+    if (rz_new >= 0.0) ++nonneg_zr;
     const double beta = (rz_new - zro) / rz;
     if (F) k_cg_dir<float><<<nblk(n3), 256, 0, c->stream>>>(x, p, z32, alpha, beta, n3, zsum);
     else k_cg_dir<double><<<nblk(n3), 256, 0, c->stream>>>(x, p, z, alpha, beta, n3, zsum);
@@ -1358,6 +1369,7 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   const unsigned nr = op_resid<double>(c, (double*)nullptr, bm, x, b, H.G[0], n, 1.0);
   reduce<double>(c, bm, nr, 1, sums + 6);
   if (st && st->cycles > 0) st->relres[st->cycles - 1] = read_scalar<double>(c, sums + 6) / rden;
+  (void)indefinite_note();
   reduce<double>(c, x, n3, 0, sums + 2);
   k_zerosum_guard<double><<<nblk(n3), 256, 0, c->stream>>>(x, sums + 2, H.scal, n3, 0, nullptr);
   LAUNCHCHK();
@@ -2140,6 +2152,32 @@ int mmh_column_density_f(mmh_ctx* c, const float* den, const float* def, int ng,
     reduce<float>(c, d, n3, 0, sum);
     k_scale_mean1<<<nblk(n3), 256, 0, c->stream>>>(d, sum, n3);
     k_column_density<<<nblk((i64)ng * ng), 256, 0, c->stream>>>(proj, d, def, ng, nplot);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmh_gridfigure_f(mmh_ctx* c, const float* def, int nf, const float* den, int ng, int nplot, float* pos,
+                     float* proj, float* dsp_slab) {
+  return guarded(c, [&] {
+    require(def && den && pos && proj && nf >= 4 && ng >= nf && ng % nf == 0 && nplot >= 1 &&
+                nplot <= ng / 2 - ng / nf,
+            "gridfigure: nf >= 4, ng a multiple of nf, 1 <= nplot <= ng/2 - ng/nf");
+    const int ratio = ng / nf, E = nf + 1;
+    const i64 e3 = (i64)E * E * E, n3 = cube(ng);
+    float* t = wsT<float>(c, "gf_t", 3 * e3);
+    fill<float>(c, t, 0.f, 3 * e3);
+    k_grid_corner_dsp<<<nblk(cube(nf)), 256, 0, c->stream>>>(t, def, nf, ratio);
+    if (dsp_slab) k_grid_dsp_slab<<<nblk((i64)E * E * 4), 256, 0, c->stream>>>(dsp_slab, t, nf);
+    k_grid_ghosts<<<nblk((i64)E * E), 256, 0, c->stream>>>(t, nf);
+    k_grid_positions<<<nblk((i64)nf * nf), 256, 0, c->stream>>>(pos, t, nf, ratio, ng);
+    LAUNCHCHK();
+    float* d = wsT<float>(c, "gf_den", n3);
+    HIPCHK(hipMemcpyAsync(d, den, sizeof(float) * n3, hipMemcpyDeviceToDevice, c->stream));
+    double* sum = wsT<double>(c, "gf_sum", 1);
+    reduce<float>(c, d, n3, 0, sum);
+    k_scale_mean1<<<nblk(n3), 256, 0, c->stream>>>(d, sum, n3);
+    k_grid_column<<<nblk((i64)ng * ng), 256, 0, c->stream>>>(proj, d, t, nf, ratio, ng, nplot);
     LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(c->stream));
   });

@@ -65,7 +65,8 @@ class SolveOpts(C.Structure):
 
 
 class SolveStats(C.Structure):
-    _fields_ = [("cycles", C.c_int), ("relres", C.c_double * 256), ("rhs_norm", C.c_double)]
+    _fields_ = [("cycles", C.c_int), ("relres", C.c_double * 256), ("rhs_norm", C.c_double),
+                ("nonneg_pLp", C.c_int), ("nonneg_zr", C.c_int)]
 
 
 _lib = None
@@ -255,6 +256,13 @@ class Context:
     def column_density(self, den, def_, ng, nplot, proj):
         """column_density.f90: density integrated over the deformed column (float32 device)."""
         self.call("mmh_column_density_f", _ptr(den), _ptr(def_), ng, nplot, _ptr(proj))
+
+    def gridfigure(self, def_, den, nplot, pos, proj, dsp_slab=None):
+        """gridfigure.f90:41-131 (FP32 device tensors): def_ nf^3, den ng^3 -> pos (2 nf^2), proj (ng^2),
+        optionally dsp_slab (2 (nf+1)^2 4)."""
+        nf, ng = _cube_side(def_), _cube_side(den)
+        self.call("mmh_gridfigure_f", _ptr(def_), nf, _ptr(den), ng, nplot, _ptr(pos), _ptr(proj),
+                  _ptr(dsp_slab) if dsp_slab is not None else None)
 
     def power(self, field, nbins=16):
         """Binned power spectrum of an N^3 FP64 device field: (P_b, mode counts) as numpy arrays."""

@@ -26,7 +26,8 @@ SRCS="gauss1 mgutil multigrid limiter relaxing recipes12 gradient reconstruct"
 INCS="dimen relaxgl globalpa cold mg8 mg16 mg32 mg64 mg128 mg256 mg512 mg1024"
 mkdir -p "$OUT/build"
 for NG in $NGS; do
-  for V in f32 f64 f32omp; do
+  # REF_VARIANTS selects the builds (the CPU baseline at NG = 512 needs only f32omp)
+  for V in ${REF_VARIANTS:-f32 f64 f32omp}; do
     B=$OUT/build/ng${NG}_$V
     rm -rf "$B"; mkdir -p "$B"
     for f in $SRCS $INCS; do ln -s "$REF/$f.f90" "$B/$f.f90"; done
@@ -34,6 +35,9 @@ for NG in $NGS; do
     case $V in
       f32) FL="-O2";; f64) FL="-O2 -fdefault-real-8";; f32omp) FL="-O3 -fopenmp";;
     esac
+    # the reference's own Makefile builds with -mcmodel=large (mmh/Makefile:7); it is needed from
+    # NG = 512 up, where the static work arrays pass 2 GiB
+    [ "$NG" -ge 512 ] && FL="$FL -mcmodel=large"
     objs=""
     for f in $SRCS; do
       # gradient/reconstruct declare explicit real(4) arrays and cannot be promoted consistently
@@ -45,6 +49,7 @@ for NG in $NGS; do
     echo "built $OUT/libmmhref_ng${NG}_$V.so"
   done
   # the reference-style caller of tests/dropin/, linked against this build's objects (FP32 as shipped)
+  [ -f "$OUT/libmmhref_ng${NG}_f32.so" ] || continue
   $FC -cpp -DNG=$NG -O2 -w "$HERE/../tests/dropin/dropin_driver.F90" -o "$OUT/dropin_ref_ng$NG" \
     -L "$OUT" -lmmhref_ng${NG}_f32 -Wl,-rpath,'$ORIGIN'
   echo "built $OUT/dropin_ref_ng$NG"

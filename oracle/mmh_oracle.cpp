@@ -1270,6 +1270,117 @@ static void column_density(const float* den_in, const float* def, int ng, int np
     }
 }
 
+// gridfigure.f90:41-131: the mesh picture of a coarse reconstruction (nf^3 deformation potential)
+// on a finer density grid (ng^3, ratio = ng/nf), FP32 as the reference.
+//  1. Cell-corner displacements of the coarse mesh (48-50): tmp_dsp(c, i, j, k), c = x, y, z, is
+//     ratio/4 times the sum of the four def differences across the cell along axis c; stored with
+//     a ghost layer, tmp_dsp(3, 0:nf, 0:nf, 0:nf), whose static storage starts at zero.
+//  2. dsp_proj.dat (57-59): tmp_dsp(1:2, :, :, nf/2:nf/2+3), before the ghost layer is filled.
+//  3. Ghost layer (61-67): tmp = cshift(cshift(cshift(tmp_dsp,-1,2),-1,3),-1,4) over the 0:nf
+//     extents, then the faces i = 0 (j, k >= 1), k = 0 (i >= 1) and j = 0 (i, k >= 1) of tmp_dsp
+//     are copied from tmp.  The shift is taken in all three axes at once, so a face ghost holds the
+//     value diagonally across (for i = 0: tmp_dsp(nf, j-1, k-1)); reproduced as written.
+//  4. Trilinear interpolation onto the fine grid (70-97): ib = mod(i, ratio)/ratio, ip =
+//     floor(i/ratio) + 1, eight weighted corners summed in the reference's order; index ip = nf + 1
+//     addresses the next row's ghost (Fortran's column-major storage), which the linear offset
+//     below reproduces.  newposition = i + 0.5 + dsp.
+//  5. Outputs (104-121): proj2(1:2, nf, nf) = newposition(1:2, ratio::ratio, ratio::ratio, ng/2),
+//     and the column density of column_density.f90 with dsp(3) from the interpolation, den first
+//     scaled to mean 1 (FP64 total, as scale_mean1 above).
+// Only the fine planes ng/2 - 1, ng/2 and ng/2 + nplot - 1 are interpolated (the outputs read no
+// others); nplot <= ng/2 - ratio keeps every corner read inside tmp_dsp's storage.
+static int gridfigure(const float* def, int nf, const float* den_in, int ng, int nplot, float* proj2, float* proj,
+                      float* dsp_slab) {
+  if (nf < 4 || ng < nf || ng % nf || nplot < 1 || nplot > ng / 2 - ng / nf) return -1;
+  const int ratio = ng / nf, E = nf + 1;
+  const size_t E3 = (size_t)E * E * E;
+  std::vector<float> t(3 * E3, 0.f), tmp(3 * E3);
+  auto at = [&](std::vector<float>& a, int c, long ia, long jb, long kd) -> float& {
+    return a[(size_t)(c + 3 * (ia + (long)E * (jb + (long)E * kd)))];
+  };
+  auto D = [&](int i, int j, int k) { return def[(i - 1) + (size_t)nf * ((j - 1) + (size_t)nf * (k - 1))]; };
+  const float s4 = (float)ratio * 1.f / 4.f;
+  for (int k = 1; k <= nf; ++k)
+    for (int j = 1; j <= nf; ++j)
+      for (int i = 1; i <= nf; ++i) {
+        const int ip = i % nf + 1, jp = j % nf + 1, kp = k % nf + 1;
+        at(t, 0, i, j, k) = s4 * (D(ip, j, k) - D(i, j, k) + D(ip, jp, k) - D(i, jp, k) + D(ip, j, kp) - D(i, j, kp) +
+                                  D(ip, jp, kp) - D(i, jp, kp));
+        at(t, 1, i, j, k) = s4 * (D(i, jp, k) - D(i, j, k) + D(ip, jp, k) - D(ip, j, k) + D(i, jp, kp) - D(i, j, kp) +
+                                  D(ip, jp, kp) - D(ip, j, kp));
+        at(t, 2, i, j, k) = s4 * (D(i, j, kp) - D(i, j, k) + D(ip, j, kp) - D(ip, j, k) + D(i, jp, kp) - D(i, jp, k) +
+                                  D(ip, jp, kp) - D(ip, jp, k));
+      }
+  if (dsp_slab) {
+    size_t o = 0;
+    for (int kd = nf / 2; kd <= nf / 2 + 3; ++kd)
+      for (int jb = 0; jb <= nf; ++jb)
+        for (int ia = 0; ia <= nf; ++ia)
+          for (int c = 0; c < 2; ++c) dsp_slab[o++] = at(t, c, ia, jb, kd);
+  }
+  // cshift(a, -1, dim): result(m) = a(m - 1), circular over the dimension's extent E
+  for (int kd = 0; kd < E; ++kd)
+    for (int jb = 0; jb < E; ++jb)
+      for (int ia = 0; ia < E; ++ia)
+        for (int c = 0; c < 3; ++c) at(tmp, c, ia, jb, kd) = at(t, c, (ia + E - 1) % E, (jb + E - 1) % E, (kd + E - 1) % E);
+  for (int c = 0; c < 3; ++c) {
+    for (int kd = 1; kd <= nf; ++kd)
+      for (int jb = 1; jb <= nf; ++jb) at(t, c, 0, jb, kd) = at(tmp, c, 0, jb, kd);
+    for (int jb = 0; jb <= nf; ++jb)
+      for (int ia = 1; ia <= nf; ++ia) at(t, c, ia, jb, 0) = at(tmp, c, ia, jb, 0);
+    for (int kd = 1; kd <= nf; ++kd)
+      for (int ia = 1; ia <= nf; ++ia) at(t, c, ia, 0, kd) = at(tmp, c, ia, 0, kd);
+  }
+  auto dsp = [&](int c, int i, int j, int k) {
+    const float ib = (float)(i % ratio) / (float)ratio, jb = (float)(j % ratio) / (float)ratio,
+                kb = (float)(k % ratio) / (float)ratio;
+    const long ip = (long)std::floor((float)i / (float)ratio) + 1 - (long)std::floor(ib);
+    const long jp = (long)std::floor((float)j / (float)ratio) + 1 - (long)std::floor(jb);
+    const long kp = (long)std::floor((float)k / (float)ratio) + 1 - (long)std::floor(kb);
+    const float p111 = ib * jb * kb, p211 = (1 - ib) * jb * kb, p121 = ib * (1 - jb) * kb, p112 = ib * jb * (1 - kb);
+    const float p221 = (1 - ib) * (1 - jb) * kb, p212 = (1 - ib) * jb * (1 - kb), p122 = ib * (1 - jb) * (1 - kb);
+    const float p222 = (1 - ib) * (1 - jb) * (1 - kb);
+    float v = 0.f;
+    v = v + at(t, c, ip - 1, jp - 1, kp - 1) * p111;
+    v = v + at(t, c, ip, jp - 1, kp - 1) * p211;
+    v = v + at(t, c, ip - 1, jp, kp - 1) * p121;
+    v = v + at(t, c, ip - 1, jp - 1, kp) * p112;
+    v = v + at(t, c, ip, jp, kp - 1) * p221;
+    v = v + at(t, c, ip, jp - 1, kp) * p212;
+    v = v + at(t, c, ip - 1, jp, kp) * p122;
+    v = v + at(t, c, ip, jp, kp) * p222;
+    return v;
+  };
+  const int kmid = ng / 2;
+  for (int mj = 1; mj <= nf; ++mj)
+    for (int mi = 1; mi <= nf; ++mi)
+      for (int c = 0; c < 2; ++c) {
+        const int i = ratio * mi, j = ratio * mj;
+        const int ii = c == 0 ? i : j;
+        proj2[c + 2 * ((mi - 1) + (size_t)nf * (mj - 1))] = (float)ii + 0.5f + dsp(c, i, j, kmid);
+      }
+  const long n3 = (long)ng * ng * ng;
+  std::vector<float> den(den_in, den_in + n3);
+  scale_mean1(den.data(), n3);
+  auto w = [&](int a) { return ((a - 1) % ng + ng) % ng; };
+  const int k1 = ng / 2, k2 = k1 + nplot - 1;
+  const int k1m = ((k1 - 2) % ng + ng) % ng + 1;
+  for (int j = 1; j <= ng; ++j)
+    for (int i = 1; i <= ng; ++i) {
+      auto dn = [&](int k) { return den[(i - 1) + (size_t)ng * ((j - 1) + (size_t)ng * w(k))]; };
+      const int jm = ((j - 2) % ng + ng) % ng + 1, im = ((i - 2) % ng + ng) % ng + 1;
+      const float kk1 = (float)(k1 - 1) + (dsp(2, im, jm, k1m) + dsp(2, i, jm, k1m) + dsp(2, im, j, k1m) + dsp(2, i, j, k1m)) / 4.f;
+      const float kk2 = (float)k2 + (dsp(2, im, jm, k2) + dsp(2, i, jm, k2) + dsp(2, im, j, k2) + dsp(2, i, j, k2)) / 4.f;
+      const int nk1 = (int)std::floor(kk1) + 1, nk2 = (int)std::floor(kk2) + 1;
+      const float r1 = (float)nk1 - kk1, r2 = kk2 - (float)nk2 + 1.f;
+      float s1 = 0.f, s2 = 0.f;
+      for (int c = nk1 + 1; c <= nk2 - 1; ++c) s1 = s1 + dn(c);
+      for (int c = nk2; c <= nk1; ++c) s2 = s2 + dn(c);
+      proj[(i - 1) + (size_t)ng * (j - 1)] = (r1 * dn(nk1) + s1 + r2 * dn(nk2) - s2) / (float)nplot;
+    }
+  return 0;
+}
+
 extern "C" {
 
 #define OR_BOTH(X) X(double, d) X(float, f)
@@ -1336,6 +1447,10 @@ void or_particle_density(const float* xp, long np, int ng, float nf, float* den,
 void or_coarsen(const float* den, int n, int levels, float* out) { coarsen(den, n, levels, out); }
 void or_column_density(const float* den, const float* def, int ng, int nplot, float* proj) {
   column_density(den, def, ng, nplot, proj);
+}
+int or_gridfigure(const float* def, int nf, const float* den, int ng, int nplot, float* proj2, float* proj,
+                  float* dsp_slab) {
+  return gridfigure(def, nf, den, ng, nplot, proj2, proj, dsp_slab);
 }
 int or_mgstats_size(void) { return (int)sizeof(MgStats); }
 int or_limstats_size(void) { return (int)sizeof(LimStats); }

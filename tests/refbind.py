@@ -315,3 +315,15 @@ def or_column_density(den, defn, ng, nplot):
     proj = np.zeros(ng * ng, np.float32)
     oracle().or_column_density(ptr(den), ptr(defn), ng, nplot, ptr(proj))
     return proj
+
+
+def or_gridfigure(defn, nf, den, ng, nplot=1):
+    """Oracle of gridfigure.f90:41-131: (pos (2 nf^2), proj (ng^2), dsp_slab (2 (nf+1)^2 4))."""
+    defn = np.ascontiguousarray(defn, dtype=np.float32)
+    den = np.ascontiguousarray(den, dtype=np.float32)
+    pos = np.zeros(2 * nf * nf, np.float32)
+    proj = np.zeros(ng * ng, np.float32)
+    slab = np.zeros(2 * (nf + 1) ** 2 * 4, np.float32)
+    rc = oracle().or_gridfigure(ptr(defn), nf, ptr(den), ng, nplot, ptr(pos), ptr(proj), ptr(slab))
+    assert rc == 0, rc
+    return pos, proj, slab

@@ -53,3 +53,25 @@ def test_column_density_matches_oracle(ctx, ng, nplot):
     proj = torch.zeros(ng * ng, dtype=torch.float32, device="cuda")
     ctx.column_density(torch.from_numpy(den).cuda(), torch.from_numpy(defn).cuda(), ng, nplot, proj)
     assert np.allclose(proj.cpu().numpy(), ref, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("nf,ng,nplot", [(8, 32, 1), (16, 128, 3), (32, 256, 1)])
+def test_gridfigure_matches_oracle(ctx, nf, ng, nplot):
+    """gridfigure.f90:41-131 on the device against the oracle: corner displacements (dsp_proj.dat)
+    and node positions bit for bit; the column density to 1e-6 (den's mean-1 scaling uses an FP64
+    total summed in another order)."""
+    print("case gridfigure nf=%d ng=%d nplot=%d" % (nf, ng, nplot), flush=True)
+    rng = np.random.default_rng(nf + ng)
+    d = rng.standard_normal(nf ** 3).astype(np.float32)
+    for _ in range(3):
+        R.or_nsmooth(d, nf)
+    defn = (d * np.float32(3.0)).astype(np.float32)
+    den = (rng.random(ng ** 3) + 0.5).astype(np.float32)
+    pos_o, proj_o, slab_o = R.or_gridfigure(defn, nf, den, ng, nplot)
+    f32 = dict(dtype=torch.float32, device="cuda")
+    pos, proj, slab = torch.zeros(2 * nf * nf, **f32), torch.zeros(ng * ng, **f32), torch.zeros(slab_o.size, **f32)
+    ctx.gridfigure(torch.from_numpy(defn).cuda(), torch.from_numpy(den).cuda(), nplot, pos, proj, slab)
+    torch.cuda.synchronize()
+    assert np.array_equal(slab.cpu().numpy(), slab_o)
+    assert np.array_equal(pos.cpu().numpy(), pos_o)
+    assert np.allclose(proj.cpu().numpy(), proj_o, rtol=1e-6, atol=0)
