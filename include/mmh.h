@@ -135,6 +135,10 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 preconditioning V-cycle as a HIP graph on its second call and replays it after;
  *                 on the null stream the solve then runs on a stream of its own, ordered after
  *                 the work already queued
+ *   "precond_bf16" 1 (default) / 0: in the mixed Krylov solve (mixed = 1, krylov = 1) the FP32
+ *                 preconditioner reads the finest level's metric and the Galerkin operators of levels
+ *                 n >= 32 from 16-bit (bfloat16) copies; its arithmetic stays FP32.  Changes the
+ *                 preconditioner, not the FP64 tolerance reached
  *   "mesh_solve"  0 (default) / 1 / 2: the mesh solve inside calcdefp and the reconstruction
  *                 iteration (FP64 entry points).  0 is the reference's one V-cycle per outer
  *                 iteration (mmh/limiter.f90:408); 1 repeats V(2,2) cycles of the reference V-cycle

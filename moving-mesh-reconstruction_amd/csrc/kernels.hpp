@@ -13,6 +13,23 @@ namespace mmh {
 
 typedef int64_t i64;
 
+// 16-bit storage of preconditioner coefficients (bfloat16: the top half of an FP32 word).  The
+// arithmetic stays in T; a coefficient array of type TG is read through gl<T>, which widens bf16
+// exactly and is the plain load when TG is T.
+struct bf16 {
+  unsigned short b;
+};
+template <class T, class TG> __device__ __forceinline__ T gl(const TG* __restrict__ p, i64 o) { return p[o]; }
+template <> __device__ __forceinline__ float gl<float, bf16>(const bf16* __restrict__ p, i64 o) {
+  return __uint_as_float((unsigned)p[o].b << 16);
+}
+// Round to the nearest bf16 (ties to even) of the FP32 rounding of a double.
+__device__ __forceinline__ bf16 to_bf16(double x) {
+  unsigned u = __float_as_uint((float)x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return bf16{(unsigned short)(u >> 16)};
+}
+
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 __device__ __forceinline__ i64 ix3(int i, int j, int k, int n) { return (i64)i + (i64)n * ((i64)j + (i64)n * (i64)k); }
 template <class T> __device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
@@ -237,25 +254,25 @@ __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __re
 // (j&1)) of phi may be non-zero; values of the other classes are taken as 0 without being loaded,
 // and so are the G2/G3/G5 factors of cross terms whose phi difference is then 0.  The result has
 // the value the unmasked stencil gives when those classes hold zeros (a zero may differ in sign).
-template <class T, int L, bool DELTA = false, bool MASK = false>
-__device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __restrict__ G, i64 n3, int n, int i, int j,
+template <class T, int L, bool DELTA = false, bool MASK = false, class TG = T>
+__device__ __forceinline__ void stencil19(const T* __restrict__ phi, const TG* __restrict__ G, i64 n3, int n, int i, int j,
                                           int k, T& t1, T& t2, T& tdiag, unsigned live = 0xFFu) {
-  const T* G1 = G;
-  const T* G2 = G + n3;
-  const T* G3 = G + 2 * n3;
-  const T* G4 = G + 3 * n3;
-  const T* G5 = G + 4 * n3;
-  const T* G6 = G + 5 * n3;
+  const TG* G1 = G;
+  const TG* G2 = G + n3;
+  const TG* G3 = G + 2 * n3;
+  const TG* G4 = G + 3 * n3;
+  const TG* G5 = G + 4 * n3;
+  const TG* G6 = G + 5 * n3;
   const int m = n - 1;
   const int ip = (i + 1) & m, im = (i - 1) & m, jp = (j + 1) & m, jm = (j - 1) & m, kp = (k + 1) & m, km = (k - 1) & m;
   const i64 c = lx<L>(i, j, k, n);
 #define X_(a, b, cc) lx<L>(a, b, cc, n)
-  T b11p = (G1[X_(ip, j, k)] + G1[c]) / T(2);
-  T b11m = (G1[X_(im, j, k)] + G1[c]) / T(2);
-  T b22p = (G4[X_(i, jp, k)] + G4[c]) / T(2);
-  T b22m = (G4[X_(i, jm, k)] + G4[c]) / T(2);
-  T b33p = (G6[X_(i, j, kp)] + G6[c]) / T(2);
-  T b33m = (G6[X_(i, j, km)] + G6[c]) / T(2);
+  T b11p = (gl<T>(G1, X_(ip, j, k)) + gl<T>(G1, c)) / T(2);
+  T b11m = (gl<T>(G1, X_(im, j, k)) + gl<T>(G1, c)) / T(2);
+  T b22p = (gl<T>(G4, X_(i, jp, k)) + gl<T>(G4, c)) / T(2);
+  T b22m = (gl<T>(G4, X_(i, jm, k)) + gl<T>(G4, c)) / T(2);
+  T b33p = (gl<T>(G6, X_(i, j, kp)) + gl<T>(G6, c)) / T(2);
+  T b33m = (gl<T>(G6, X_(i, j, km)) + gl<T>(G6, c)) / T(2);
   tdiag = -(b11p + b11m + b22p + b22m + b33p + b33m);
   if constexpr (MASK) {
     static_assert(!DELTA, "the masked stencil is the plain form");
@@ -263,7 +280,7 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __
     const bool fx = (live >> (cc ^ 2u)) & 1u, fy = (live >> (cc ^ 1u)) & 1u, fz = (live >> (cc ^ 4u)) & 1u;
     const bool fxy = (live >> (cc ^ 3u)) & 1u, fyz = (live >> (cc ^ 5u)) & 1u, fxz = (live >> (cc ^ 6u)) & 1u;
     auto P = [&](bool on, i64 a) { return on ? phi[a] : T(0); };
-    auto F = [&](bool on, const T* g, i64 a) { return on ? g[a] : T(0); };
+    auto F = [&](bool on, const TG* g, i64 a) { return on ? gl<T>(g, a) : T(0); };
     t1 = (b11p * P(fx, X_(ip, j, k)) + b11m * P(fx, X_(im, j, k)) + b22p * P(fy, X_(i, jp, k)) +
           b22m * P(fy, X_(i, jm, k)) + b33p * P(fz, X_(i, j, kp)) + b33m * P(fz, X_(i, j, km)));
     t2 = ((P(fxy, X_(ip, jp, k)) - P(fxy, X_(im, jp, k))) * F(fxy, G2, X_(i, jp, k))
@@ -288,31 +305,31 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const T* __
     t1 = (b11p * phi[X_(ip, j, k)] + b11m * phi[X_(im, j, k)] + b22p * phi[X_(i, jp, k)] +
           b22m * phi[X_(i, jm, k)] + b33p * phi[X_(i, j, kp)] + b33m * phi[X_(i, j, km)]);
   }
-  t2 = ((phi[X_(ip, jp, k)] - phi[X_(im, jp, k)]) * G2[X_(i, jp, k)]
-      - (phi[X_(ip, jm, k)] - phi[X_(im, jm, k)]) * G2[X_(i, jm, k)]
-      + (phi[X_(ip, jp, k)] - phi[X_(ip, jm, k)]) * G2[X_(ip, j, k)]
-      - (phi[X_(im, jp, k)] - phi[X_(im, jm, k)]) * G2[X_(im, j, k)]
-      + (phi[X_(i, jp, kp)] - phi[X_(i, jm, kp)]) * G5[X_(i, j, kp)]
-      - (phi[X_(i, jp, km)] - phi[X_(i, jm, km)]) * G5[X_(i, j, km)]
-      + (phi[X_(i, jp, kp)] - phi[X_(i, jp, km)]) * G5[X_(i, jp, k)]
-      - (phi[X_(i, jm, kp)] - phi[X_(i, jm, km)]) * G5[X_(i, jm, k)]
-      + (phi[X_(ip, j, kp)] - phi[X_(im, j, kp)]) * G3[X_(i, j, kp)]
-      - (phi[X_(ip, j, km)] - phi[X_(im, j, km)]) * G3[X_(i, j, km)]
-      + (phi[X_(ip, j, kp)] - phi[X_(ip, j, km)]) * G3[X_(ip, j, k)]
-      - (phi[X_(im, j, kp)] - phi[X_(im, j, km)]) * G3[X_(im, j, k)]) / T(4);
+  t2 = ((phi[X_(ip, jp, k)] - phi[X_(im, jp, k)]) * gl<T>(G2, X_(i, jp, k))
+      - (phi[X_(ip, jm, k)] - phi[X_(im, jm, k)]) * gl<T>(G2, X_(i, jm, k))
+      + (phi[X_(ip, jp, k)] - phi[X_(ip, jm, k)]) * gl<T>(G2, X_(ip, j, k))
+      - (phi[X_(im, jp, k)] - phi[X_(im, jm, k)]) * gl<T>(G2, X_(im, j, k))
+      + (phi[X_(i, jp, kp)] - phi[X_(i, jm, kp)]) * gl<T>(G5, X_(i, j, kp))
+      - (phi[X_(i, jp, km)] - phi[X_(i, jm, km)]) * gl<T>(G5, X_(i, j, km))
+      + (phi[X_(i, jp, kp)] - phi[X_(i, jp, km)]) * gl<T>(G5, X_(i, jp, k))
+      - (phi[X_(i, jm, kp)] - phi[X_(i, jm, km)]) * gl<T>(G5, X_(i, jm, k))
+      + (phi[X_(ip, j, kp)] - phi[X_(im, j, kp)]) * gl<T>(G3, X_(i, j, kp))
+      - (phi[X_(ip, j, km)] - phi[X_(im, j, km)]) * gl<T>(G3, X_(i, j, km))
+      + (phi[X_(ip, j, kp)] - phi[X_(ip, j, km)]) * gl<T>(G3, X_(ip, j, k))
+      - (phi[X_(im, j, kp)] - phi[X_(im, j, km)]) * gl<T>(G3, X_(im, j, k))) / T(4);
 #undef X_
 }
 
 // Gauss-Seidel update of point q (0 <= q < (n/2)^3) of the colour class (kpar, io, jo), in place
 // (relaxplane, mmh/gauss1.f90:954-980).
-template <class T, int L, bool DELTA = false, bool MASK = false>
-__device__ __forceinline__ void colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G,
+template <class T, int L, bool DELTA = false, bool MASK = false, class TG = T>
+__device__ __forceinline__ void colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ G,
                                              int n, T dx2, int kpar, int io, int jo, i64 q, unsigned live = 0xFFu) {
   const int h = n >> 1;
   const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
   const i64 n3 = (i64)n * n * n;
   T t1, t2, tdiag;
-  stencil19<T, L, DELTA, MASK>(phi, G, n3, n, i, j, k, t1, t2, tdiag, live);
+  stencil19<T, L, DELTA, MASK, TG>(phi, G, n3, n, i, j, k, t1, t2, tdiag, live);
   const i64 c = lx<L>(i, j, k, n);
   if (DELTA) phi[c] = phi[c] + (rhs[c] * dx2 - t1 - t2) / tdiag;
   else phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
@@ -325,8 +342,8 @@ __device__ __forceinline__ void colour_point(T* __restrict__ phi, const T* __res
 // Batched launches: gridDim.y = B grids of the same size stored one after another (bstride = n^3
 // values; G holds 6 n^3 per grid).  remap = 0 keeps the hardware's block order (a tuning choice).
 // MASK: the first sweep from phi = 0, where live holds the classes updated earlier in the sweep.
-template <class T, int L, bool DELTA = false, bool MASK = false>
-__global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G, int n,
+template <class T, int L, bool DELTA = false, bool MASK = false, class TG = T>
+__global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ G, int n,
                                                 T dx2, int kpar, int io, int jo, i64 bstride = 0, int remap = 1,
                                                 unsigned live = 0xFFu, const int* __restrict__ act = nullptr) {
   if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
@@ -337,7 +354,7 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
   const unsigned blk = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   const i64 q = (i64)blk * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
-  colour_point<T, L, DELTA, MASK>(phi, rhs, G, n, dx2, kpar, io, jo, q, live);
+  colour_point<T, L, DELTA, MASK, TG>(phi, rhs, G, n, dx2, kpar, io, jo, q, live);
 }
 
 // The colour class of k_colour (sub-lattice layout) with each thread updating S consecutive points
@@ -347,9 +364,9 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
 // k-1 values of the next point, carried in registers instead of fetched again.  Each point's
 // arithmetic is stencil19's, term for term and in the same order (the build contracts no
 // multiply-add), so the results are bit-identical to k_colour's, MASK and DELTA forms included.
-template <class T, bool DELTA, bool MASK, int S>
+template <class T, bool DELTA, bool MASK, int S, class TG = T>
 __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T* __restrict__ rhs,
-                                                    const T* __restrict__ G, int n, T dx2, int kpar, int io, int jo,
+                                                    const TG* __restrict__ G, int n, T dx2, int kpar, int io, int jo,
                                                     unsigned live = 0xFFu) {
   static_assert(!(DELTA && MASK), "the masked stencil is the plain form");
   const int h = n >> 1, m = n - 1;
@@ -361,12 +378,12 @@ __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T
   const int xp = ((i + 1) & m) >> 1, xm = ((i - 1) & m) >> 1, yp = ((j + 1) & m) >> 1, ym = ((j - 1) & m) >> 1;
   // sub-lattice classes (i&1 | (j&1)<<1 | (k&1)<<2) of the point and of its neighbours
   const int c = io | (jo << 1) | (kpar << 2);
-  const T* G1 = G;
-  const T* G2 = G + n3;
-  const T* G3 = G + 2 * n3;
-  const T* G4 = G + 3 * n3;
-  const T* G5 = G + 4 * n3;
-  const T* G6 = G + 5 * n3;
+  const TG* G1 = G;
+  const TG* G2 = G + n3;
+  const TG* G3 = G + 2 * n3;
+  const TG* G4 = G + 3 * n3;
+  const TG* G5 = G + 4 * n3;
+  const TG* G6 = G + 5 * n3;
   const T* px = phi + (c ^ 1) * h3;
   const T* py = phi + (c ^ 2) * h3;
   const T* pz = phi + (c ^ 4) * h3;
@@ -380,11 +397,11 @@ __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T
   const bool fz = !MASK || ((live >> (cc ^ 4u)) & 1u), fxy = !MASK || ((live >> (cc ^ 3u)) & 1u);
   const bool fyz = !MASK || ((live >> (cc ^ 5u)) & 1u), fxz = !MASK || ((live >> (cc ^ 6u)) & 1u);
   auto P = [&](bool on, const T* a, i64 o) { return on ? a[o] : T(0); };
-  auto F = [&](bool on, const T* g, i64 o) { return on ? g[o] : T(0); };
+  auto F = [&](bool on, const TG* g, i64 o) { return on ? gl<T>(g, o) : T(0); };
   auto at = [&](int a, int b, int kq) { return a + (i64)h * (b + (i64)h * kq); };
   // the k-1 plane values of the first point
   const int km0 = (((2 * kk0 + kpar) - 1) & m) >> 1;
-  T g6m = G6[oz + at(qi, qj, km0)], pzm = P(fz, pz, at(qi, qj, km0));
+  T g6m = gl<T>(G6, oz + at(qi, qj, km0)), pzm = P(fz, pz, at(qi, qj, km0));
   T pyzpm = P(fyz, pyz, at(qi, yp, km0)), pyzmm = P(fyz, pyz, at(qi, ym, km0));
   T g5m = F(fyz, G5 + oz, at(qi, qj, km0));
   T pxzpm = P(fxz, pxz, at(xp, qj, km0)), pxzmm = P(fxz, pxz, at(xm, qj, km0));
@@ -395,17 +412,17 @@ __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T
     const int kk = kk0 + s;
     const int kpl = (((2 * kk + kpar) + 1) & m) >> 1;
     const i64 o = at(qi, qj, kk);
-    const T g6p = G6[oz + at(qi, qj, kpl)], pzp = P(fz, pz, at(qi, qj, kpl));
+    const T g6p = gl<T>(G6, oz + at(qi, qj, kpl)), pzp = P(fz, pz, at(qi, qj, kpl));
     const T pyzpp = P(fyz, pyz, at(qi, yp, kpl)), pyzmp = P(fyz, pyz, at(qi, ym, kpl));
     const T g5p = F(fyz, G5 + oz, at(qi, qj, kpl));
     const T pxzpp = P(fxz, pxz, at(xp, qj, kpl)), pxzmp = P(fxz, pxz, at(xm, qj, kpl));
     const T g3p = F(fxz, G3 + oz, at(qi, qj, kpl));
-    const T b11p = (G1[ox + at(xp, qj, kk)] + G1[oc + o]) / T(2);
-    const T b11m = (G1[ox + at(xm, qj, kk)] + G1[oc + o]) / T(2);
-    const T b22p = (G4[oy + at(qi, yp, kk)] + G4[oc + o]) / T(2);
-    const T b22m = (G4[oy + at(qi, ym, kk)] + G4[oc + o]) / T(2);
-    const T b33p = (g6p + G6[oc + o]) / T(2);
-    const T b33m = (g6m + G6[oc + o]) / T(2);
+    const T b11p = (gl<T>(G1, ox + at(xp, qj, kk)) + gl<T>(G1, oc + o)) / T(2);
+    const T b11m = (gl<T>(G1, ox + at(xm, qj, kk)) + gl<T>(G1, oc + o)) / T(2);
+    const T b22p = (gl<T>(G4, oy + at(qi, yp, kk)) + gl<T>(G4, oc + o)) / T(2);
+    const T b22m = (gl<T>(G4, oy + at(qi, ym, kk)) + gl<T>(G4, oc + o)) / T(2);
+    const T b33p = (g6p + gl<T>(G6, oc + o)) / T(2);
+    const T b33m = (g6m + gl<T>(G6, oc + o)) / T(2);
     const T tdiag = -(b11p + b11m + b22p + b22m + b33p + b33m);
     const T pxp = P(fx, px, at(xp, qj, kk)), pxm = P(fx, px, at(xm, qj, kk));
     const T pyp = P(fy, py, at(qi, yp, kk)), pym = P(fy, py, at(qi, ym, kk));
@@ -438,7 +455,8 @@ __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T
   }
   // Stores after all loads: the compiler cannot tell that they never alias the other classes' phi,
   // and would otherwise hold the next point's loads behind each store.  The masked form stores
-  // point by point: its hoisted loads would need more registers than the kernel can have.
+  // point by point: its hoisted loads would need more registers than the kernel can have (with
+  // FP32 values and 16-bit coefficients too: 156 VGPRs and three times the time per launch).
   if constexpr (!MASK) {
 #pragma unroll
     for (int s = 0; s < S; ++s) phi[oc + at(qi, qj, kk0 + s)] = res[s];
@@ -446,22 +464,22 @@ __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T
 }
 
 // Residual rhs - L[phi] at point (i, j, k) (calcresid, mmh/gauss1.f90:683-709).
-template <class T, int L, bool DELTA = false>
-__device__ __forceinline__ T resid_point(const T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ G,
+template <class T, int L, bool DELTA = false, class TG = T>
+__device__ __forceinline__ T resid_point(const T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ G,
                                         int n, T di2, int i, int j, int k) {
   const i64 n3 = (i64)n * n * n;
   const i64 p = lx<L>(i, j, k, n);
   T t1, t2, tdiag;
-  stencil19<T, L, DELTA>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+  stencil19<T, L, DELTA, false, TG>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
   return DELTA ? rhs[p] - (t1 + t2) * di2 : rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
 }
 
 // Residual r = rhs - L[phi] (calcresid, mmh/gauss1.f90:683-709); optional store; per-block max|r|.
 // In layout 1 a thread owns one 2x2x2 cell and evaluates its 8 points, one per sub-lattice, so
 // the 8 class arrays and their neighbours are read in lockstep at unit stride.
-template <class T, int L, bool DELTA = false>
+template <class T, int L, bool DELTA = false, class TG = T>
 __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restrict__ bmax, const T* __restrict__ phi,
-                                              const T* __restrict__ rhs, const T* __restrict__ G, int n, T di2,
+                                              const T* __restrict__ rhs, const TG* __restrict__ G, int n, T di2,
                                               int batched = 0, const int* __restrict__ act = nullptr) {
   if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const i64 n3 = (i64)n * n * n;
@@ -485,7 +503,7 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
     for (int c = 0; c < 8; ++c) {
       const int i = 2 * ci + (c & 1), j = 2 * cj + ((c >> 1) & 1), k = 2 * ck + (c >> 2);
       const i64 p = lx<L>(i, j, k, n);
-      const T rv = resid_point<T, L, DELTA>(phi, rhs, G, n, di2, i, j, k);
+      const T rv = resid_point<T, L, DELTA, TG>(phi, rhs, G, n, di2, i, j, k);
       if (r) r[p] = rv;
       a = dmax(a, (double)fabs(rv));
     }
@@ -493,7 +511,7 @@ __global__ __launch_bounds__(RB) void k_resid(T* __restrict__ r, double* __restr
     int i, j, k;
     point_of<L>(pp, n, i, j, k);
     const i64 p = lx<L>(i, j, k, n);
-    const T rv = resid_point<T, L>(phi, rhs, G, n, di2, i, j, k);
+    const T rv = resid_point<T, L, false, TG>(phi, rhs, G, n, di2, i, j, k);
     if (r) r[p] = rv;
     a = (double)fabs(rv);
   }

@@ -175,8 +175,8 @@ __global__ __launch_bounds__(256) void k_rap27(double* __restrict__ Ac, const do
 // With MASK, neighbours in colour classes b (4 (k&1) + 2 (i&1) + (j&1)) whose bit of live is clear
 // hold zeros: their terms are left out, coefficient and value unread (the sum keeps its value; a
 // zero may differ in sign).
-template <class T, bool MASK = false>
-__device__ __forceinline__ T gal_offsum(const T* __restrict__ phi, const T* __restrict__ A, int n, int i, int j, int k,
+template <class T, bool MASK = false, class TG = T>
+__device__ __forceinline__ T gal_offsum(const T* __restrict__ phi, const TG* __restrict__ A, int n, int i, int j, int k,
                                        i64 p, unsigned live = 0xFFu) {
   const int m = n - 1;
   const i64 n3 = (i64)n * n * n;
@@ -187,35 +187,35 @@ __device__ __forceinline__ T gal_offsum(const T* __restrict__ phi, const T* __re
     if (e == 13) continue;
     const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
     if (MASK && !((live >> (cc ^ (4u * (ez != 0) + 2u * (ex != 0) + (ey != 0)))) & 1u)) continue;
-    s = s + A[(i64)e * n3 + p] * phi[lx<1>((i + ex) & m, (j + ey) & m, (k + ez) & m, n)];
+    s = s + gl<T>(A, (i64)e * n3 + p) * phi[lx<1>((i + ex) & m, (j + ey) & m, (k + ez) & m, n)];
   }
   return s;
 }
 
 // Gauss-Seidel update of point q of colour class (kpar, io, jo) on a 27-point level (the 8
 // classes decouple the 27-point stencil as they do the 19-point one).
-template <class T, bool MASK = false>
-__device__ __forceinline__ void gal_colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
+template <class T, bool MASK = false, class TG = T>
+__device__ __forceinline__ void gal_colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ A,
                                                  int n, int kpar, int io, int jo, i64 q, unsigned live = 0xFFu) {
   const int h = n >> 1;
   const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
   const i64 p = lx<1>(i, j, k, n);
   const i64 n3 = (i64)n * n * n;
-  phi[p] = (rhs[p] - gal_offsum<T, MASK>(phi, A, n, i, j, k, p, live)) / A[13 * n3 + p];
+  phi[p] = (rhs[p] - gal_offsum<T, MASK, TG>(phi, A, n, i, j, k, p, live)) / gl<T>(A, 13 * n3 + p);
 }
 
-template <class T>
-__device__ __forceinline__ T gal_resid_point(const T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
+template <class T, class TG = T>
+__device__ __forceinline__ T gal_resid_point(const T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ A,
                                             int n, int i, int j, int k) {
   const i64 p = lx<1>(i, j, k, n);
   const i64 n3 = (i64)n * n * n;
-  return rhs[p] - (gal_offsum<T>(phi, A, n, i, j, k, p) + A[13 * n3 + p] * phi[p]);
+  return rhs[p] - (gal_offsum<T, false, TG>(phi, A, n, i, j, k, p) + gl<T>(A, 13 * n3 + p) * phi[p]);
 }
 
 // Batched launches: gridDim.y = B grids stored one after another (bstride = n^3 values; A holds
 // 27 n^3 per grid).  MASK: the first sweep from phi = 0 (live: classes updated earlier in it).
-template <class T, bool MASK = false>
-__global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T* __restrict__ rhs, const T* __restrict__ A,
+template <class T, bool MASK = false, class TG = T>
+__global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ A,
                                                     int n, int kpar, int io, int jo, i64 bstride = 0,
                                                     unsigned live = 0xFFu, const int* __restrict__ act = nullptr) {
   if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
@@ -226,13 +226,13 @@ __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T
   const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
   const i64 q = (i64)blk * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
-  gal_colour_point<T, MASK>(phi, rhs, A, n, kpar, io, jo, q, live);
+  gal_colour_point<T, MASK, TG>(phi, rhs, A, n, kpar, io, jo, q, live);
 }
 
 // r = rhs - A phi at every point (one thread per point, sub-lattice order); r stored.
-template <class T>
+template <class T, class TG = T>
 __global__ __launch_bounds__(256) void k_gal_resid(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
-                                                   const T* __restrict__ A, int n, i64 bstride = 0, const int* __restrict__ act = nullptr) {
+                                                   const TG* __restrict__ A, int n, i64 bstride = 0, const int* __restrict__ act = nullptr) {
   if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   r += (i64)blockIdx.y * bstride;
   phi += (i64)blockIdx.y * bstride;
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(256) void k_gal_resid(T* __restrict__ r, const T* _
   if (p >= (i64)n * n * n) return;
   int i, j, k;
   point_of<1>(p, n, i, j, k);
-  r[p] = gal_resid_point<T>(phi, rhs, A, n, i, j, k);
+  r[p] = gal_resid_point<T, TG>(phi, rhs, A, n, i, j, k);
 }
 
 }  // namespace mmh

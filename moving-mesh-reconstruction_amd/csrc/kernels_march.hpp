@@ -26,10 +26,10 @@ namespace mmh {
 // batch when batched.  MODE 0: out = rhs - L phi (out may be null), bres = max |out|.  MODE 1:
 // out = L phi, bres = sum phi * out.  Requires n % TI == n % TJ == n % KS == 0, TI >= 2, TJ >= 2,
 // and 2 (TI + TJ) + 4 <= TI * TJ (one halo point per thread at most).
-template <class T, int MODE, int TI, int TJ>
+template <class T, int MODE, int TI, int TJ, class TG = T>
 __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, double* __restrict__ bres,
                                                       const T* __restrict__ phi, const T* __restrict__ rhs,
-                                                      const T* __restrict__ G, int n, T di2, int KS, int batched, const int* __restrict__ act = nullptr) {
+                                                      const TG* __restrict__ G, int n, T di2, int KS, int batched, const int* __restrict__ act = nullptr) {
   if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   constexpr int NT = TI * TJ, RI = TI + 2, RJ = TJ + 2, PL = RI * RJ, NH = PL - NT;
   static_assert(NH <= NT, "one halo point per thread at most");
@@ -83,13 +83,17 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
   }
   const uint32_t oh = BO(IJ((i0 + hx) & m, (j0 + hy) & m));
   auto LD = [](const T* base, uint32_t off) -> T { return *(const T*)((const char*)base + off); };
+  // the metric's byte offsets (its element type TG may be narrower than T)
+  auto BOG = [&](i64 e) -> uint32_t { return (uint32_t)(e * (i64)sizeof(TG)); };
+  const uint32_t ocg = BOG(IJ(i, j)), ohg = BOG(IJ((i0 + hx) & m, (j0 + hy) & m));
+  auto LDG = [](const TG* base, uint32_t off) -> T { return gl<T>((const TG*)((const char*)base + off), 0); };
   const int own = (ty + 1) * RI + tx + 1, hslot = (hy + 1) * RI + hx + 1;
-  const T* G1 = G;
-  const T* G2 = G + n3;
-  const T* G3 = G + 2 * n3;
-  const T* G4 = G + 3 * n3;
-  const T* G5 = G + 4 * n3;
-  const T* G6 = G + 5 * n3;
+  const TG* G1 = G;
+  const TG* G2 = G + n3;
+  const TG* G3 = G + 2 * n3;
+  const TG* G4 = G + 3 * n3;
+  const TG* G5 = G + 4 * n3;
+  const TG* G6 = G + 5 * n3;
 
   // halo points in the left/right columns supply G1, G2, G3 (i-neighbours); in the top/bottom rows
   // G2, G4, G5 (j-neighbours); corners no metric value
@@ -110,17 +114,17 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
   }
   auto load_next = [&](int kn, int kphi) {  // G1/G2/G4 and halo G of plane kn, phi of plane kphi
     const i64 q = K(kn);
-    n1 = LD(G1 + q, oc);
-    n2 = LD(G2 + q, oc);
-    n4 = LD(G4 + q, oc);
+    n1 = LDG(G1 + q, ocg);
+    n2 = LDG(G2 + q, ocg);
+    n4 = LDG(G4 + q, ocg);
     if (hcol) {
-      nh0 = LD(G1 + q, oh);
-      nh1 = LD(G2 + q, oh);
-      nh2 = LD(G3 + q, oh);
+      nh0 = LDG(G1 + q, ohg);
+      nh1 = LDG(G2 + q, ohg);
+      nh2 = LDG(G3 + q, ohg);
     } else if (hrow) {
-      nh0 = LD(G2 + q, oh);
-      nh1 = LD(G4 + q, oh);
-      nh2 = LD(G5 + q, oh);
+      nh0 = LDG(G2 + q, ohg);
+      nh1 = LDG(G4 + q, ohg);
+      nh2 = LDG(G5 + q, ohg);
     }
     const T* pz = phi + K(kphi);
     nphi = LD(pz, oc);
@@ -129,9 +133,9 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
   load_next(k0, k0 + 1);
   T rhs_c = MODE == 0 ? LD(rhs + K(k0), oc) : T(0);  // rhs of the current plane, loaded a step ahead
   const i64 km0 = K(k0 - 1), kc0 = K(k0), kp0 = K(k0 + 1);
-  T g3m = LD(G3 + km0, oc), g5m = LD(G5 + km0, oc), g6m = LD(G6 + km0, oc);
-  T g3c = LD(G3 + kc0, oc), g5c = LD(G5 + kc0, oc), g6c = LD(G6 + kc0, oc);
-  T g3p = LD(G3 + kp0, oc), g5p = LD(G5 + kp0, oc), g6p = LD(G6 + kp0, oc);
+  T g3m = LDG(G3 + km0, ocg), g5m = LDG(G5 + km0, ocg), g6m = LDG(G6 + km0, ocg);
+  T g3c = LDG(G3 + kc0, ocg), g5c = LDG(G5 + kc0, ocg), g6c = LDG(G6 + kc0, ocg);
+  T g3p = LDG(G3 + kp0, ocg), g5p = LDG(G5 + kp0, ocg), g6p = LDG(G6 + kp0, ocg);
 
   double acc = 0.0;
 #pragma unroll 1
@@ -166,9 +170,9 @@ __global__ __launch_bounds__(TI * TJ) void k_op_march(T* __restrict__ out, doubl
     if (t + 1 < KS) {
       load_next(k + 1, k + 2);
       const i64 k2 = K(k + 2);
-      g3n = LD(G3 + k2, oc);
-      g5n = LD(G5 + k2, oc);
-      g6n = LD(G6 + k2, oc);
+      g3n = LDG(G3 + k2, ocg);
+      g5n = LDG(G5 + k2, ocg);
+      g6n = LDG(G6 + k2, ocg);
       if (MODE == 0) rhs_n = LD(rhs + K(k + 1), oc);
     }
     __syncthreads();

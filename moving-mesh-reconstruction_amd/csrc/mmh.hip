@@ -17,6 +17,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mmh.h"
@@ -59,6 +60,7 @@ struct mmh_ctx {
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
   int batch_autotune = 1;  // batched solve: pick the colour kernel's launch shape by timing
   int batch_streams = 1;   // batched solve: size groups iterate side by side on streams of their own
+  int precond_bf16 = 1;    // mixed Krylov solve: 16-bit coefficients in the FP32 preconditioner (option "precond_bf16")
   int mesh_solve = 0;      // calcdefp's mesh solve: 0 one reference V-cycle, 1 V(2,2) cycles to 10^-mesh_tol,
                            // 2 the Krylov solve to 10^-mesh_tol (option "mesh_solve")
   int mesh_tol = 8;        // option "mesh_tol": the tolerance exponent of mesh_solve 1 and 2
@@ -285,6 +287,10 @@ template <class T> struct Hier {
   T* G[MMH_MAXLEV];
   T* r[MMH_MAXLEV];
   T* A[MMH_MAXLEV] = {};  // Galerkin coarse operators (27 fields per level), null where not built
+  // 16-bit copies of G / A that the FP32 preconditioner reads instead, where set (option
+  // "precond_bf16"); the arithmetic stays FP32
+  const bf16* Gb[MMH_MAXLEV] = {};
+  const bf16* Ab[MMH_MAXLEV] = {};
   double* scal;  // per level: [0] r1, [1] relaxerr, [2] sum; then [3*MAXLEV] rhs sums
   int* guard;    // per level, plus entry flag at [MAXLEV]
 };
@@ -413,38 +419,39 @@ template <class T, int V> int march_ks(const mmh_ctx* c, int n) {
 
 // One k_op_march launch (MODE 0 residual, 1 operator) if the level qualifies; returns the number
 // of block results, 0 when it does not apply.
-template <class T, int MODE, int V>
-unsigned march_launch(mmh_ctx* c, T* out, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2, int B,
+template <class T, int MODE, int V, class TG = T>
+unsigned march_launch(mmh_ctx* c, T* out, double* bres, const T* phi, const T* rhs, const TG* G, int n, T di2, int B,
                       int batched, const int* act) {
   constexpr int TI = MarchTile<T, V>::TI, TJ = MarchTile<T, V>::TJ;
   const int ks = march_ks<T, V>(c, n);
   if (!ks) return 0;
   const unsigned nb = (unsigned)(cube(n) / ((i64)TI * TJ * ks));
-  k_op_march<T, MODE, TI, TJ><<<dim3(nb, B), TI * TJ, 0, c->stream>>>(out, bres, phi, rhs, G, n, di2, ks, batched, act);
+  k_op_march<T, MODE, TI, TJ, TG><<<dim3(nb, B), TI * TJ, 0, c->stream>>>(out, bres, phi, rhs, G, n, di2, ks, batched,
+                                                                          act);
   LAUNCHCHK();
   return nb;
 }
-template <class T, int MODE>
-unsigned march_op(mmh_ctx* c, T* out, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2, int B,
+template <class T, int MODE, class TG = T>
+unsigned march_op(mmh_ctx* c, T* out, double* bres, const T* phi, const T* rhs, const TG* G, int n, T di2, int B,
                   int batched, const int* act = nullptr) {
   switch (c->march_tile) {
-    case 1: return march_launch<T, MODE, 1>(c, out, bres, phi, rhs, G, n, di2, B, batched, act);
-    case 2: return march_launch<T, MODE, 2>(c, out, bres, phi, rhs, G, n, di2, B, batched, act);
-    default: return march_launch<T, MODE, 0>(c, out, bres, phi, rhs, G, n, di2, B, batched, act);
+    case 1: return march_launch<T, MODE, 1, TG>(c, out, bres, phi, rhs, G, n, di2, B, batched, act);
+    case 2: return march_launch<T, MODE, 2, TG>(c, out, bres, phi, rhs, G, n, di2, B, batched, act);
+    default: return march_launch<T, MODE, 0, TG>(c, out, bres, phi, rhs, G, n, di2, B, batched, act);
   }
 }
 
 // r = rhs - L phi on a sub-lattice level (r may be null) with per-block max|r| into bres, for B
 // grids stored one after another when batched (act: grids y with act[y] = 0 are skipped); returns
 // the number of block results per grid.
-template <class T>
-unsigned op_resid(mmh_ctx* c, T* r, double* bres, const T* phi, const T* rhs, const T* G, int n, T di2,
+template <class T, class TG = T>
+unsigned op_resid(mmh_ctx* c, T* r, double* bres, const T* phi, const T* rhs, const TG* G, int n, T di2,
                   bool delta = false, int B = 1, int batched = 0, const int* act = nullptr) {
   if (!delta)
-    if (const unsigned nb = march_op<T, 0>(c, r, bres, phi, rhs, G, n, di2, B, batched, act)) return nb;
+    if (const unsigned nb = march_op<T, 0, TG>(c, r, bres, phi, rhs, G, n, di2, B, batched, act)) return nb;
   const unsigned nb = stencil_grid(n, 1);
-  if (delta) k_resid<T, 1, true><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched, act);
-  else k_resid<T, 1><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched, act);
+  if (delta) k_resid<T, 1, true, TG><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched, act);
+  else k_resid<T, 1, false, TG><<<dim3(nb, B), RB, 0, c->stream>>>(r, bres, phi, rhs, G, n, di2, batched, act);
   LAUNCHCHK();
   return nb;
 }
@@ -504,16 +511,16 @@ template <class T> void wave_sweep(mmh_ctx* c, T* out, const T* in, const T* rhs
 }
 
 // One colour class by k_colour_col: S points per thread along k (the same values as k_colour).
-template <class T>
-void colour_col_launch(mmh_ctx* c, int S, bool masked, bool delta, T* phi, const T* rhs, const T* G, int n, T dx2,
+template <class T, class TG = T>
+void colour_col_launch(mmh_ctx* c, int S, bool masked, bool delta, T* phi, const T* rhs, const TG* G, int n, T dx2,
                        int kp, int io, int jo, unsigned live) {
   const int h = n / 2;
   require(S > 1 && h % S == 0, "colour_col: points per thread must divide n/2");
   const unsigned nb = nblk(cube(h) / S);
 #define COL_LAUNCH(SS)                                                                                           \
-  if (masked) k_colour_col<T, false, true, SS><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo, live); \
-  else if (delta) k_colour_col<T, true, false, SS><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);   \
-  else k_colour_col<T, false, false, SS><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+  if (masked) k_colour_col<T, false, true, SS, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo, live); \
+  else if (delta) k_colour_col<T, true, false, SS, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);   \
+  else k_colour_col<T, false, false, SS, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
   if (S == 2) { COL_LAUNCH(2) }
   else if (S == 4) { COL_LAUNCH(4) }
   else if (S == 8) { COL_LAUNCH(8) }
@@ -523,13 +530,14 @@ void colour_col_launch(mmh_ctx* c, int S, bool masked, bool delta, T* phi, const
 
 // from_zero: phi is 0 on entry, so the first sweep's colours read only the classes updated before
 // them in that sweep (k_colour's MASK form; same values).
-template <class T>
-void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, int layout = 1, bool reverse = false,
+template <class T, class TG = T>
+void sweeps(mmh_ctx* c, T* phi, const T* rhs, const TG* G, int n, T dx, int ns, int layout = 1, bool reverse = false,
             bool delta = false, bool from_zero = false) {
   const T dx2 = dx * dx;
   const int h = n / 2;
   const unsigned nb = nblk(cube(h));
   from_zero = from_zero && layout == 1 && !delta;
+  if constexpr (std::is_same<T, TG>::value)
   if (layout == 1 && ns >= 2 && !delta && !from_zero && wave_fits<T>(c, n)) {
     // pairs of sweeps through a second buffer; an odd last sweep goes colour by colour below
     T* tmp = wsT<T>(c, "wave_tmp" + std::to_string(n), cube(n));
@@ -546,12 +554,14 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, i
       const int col = reverse ? 7 - cc : cc;
       const int kp = col >> 2, io = (col >> 1) & 1, jo = col & 1;
       if (S > 1)
-        colour_col_launch<T>(c, S, s == 0 && from_zero, delta, phi, rhs, G, n, dx2, kp, io, jo, (unsigned)done);
+        colour_col_launch<T, TG>(c, S, s == 0 && from_zero, delta, phi, rhs, G, n, dx2, kp, io, jo, (unsigned)done);
       else if (s == 0 && from_zero)
-        k_colour<T, 1, false, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo, 0, 1, (unsigned)done);
-      else if (layout == 1 && delta) k_colour<T, 1, true><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
-      else if (layout == 1) k_colour<T, 1><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
-      else k_colour<T, 0><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+        k_colour<T, 1, false, true, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo, 0, 1,
+                                                                   (unsigned)done);
+      else if (layout == 1 && delta)
+        k_colour<T, 1, true, false, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+      else if (layout == 1) k_colour<T, 1, false, false, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
+      else k_colour<T, 0, false, false, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);
       done |= 1 << col;
     }
   LAUNCHCHK();
@@ -559,13 +569,13 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G, int n, T dx, int ns, i
 
 // relax on the device (level arrays in sub-lattice layout): sweeps, then residual (stored when
 // rc != null), max-norm into *rinf_dev, and the calcresid restriction into rc.
-template <class T>
-void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n, T dx, int nrelax, int want_norm, T* rc,
+template <class T, class TG = T>
+void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const TG* G, T* rscratch, int n, T dx, int nrelax, int want_norm, T* rc,
                double* rinf_dev, bool reverse = false, bool delta = false, bool from_zero = false) {
-  sweeps(c, phi, rhs, G, n, dx, nrelax, 1, reverse, delta, from_zero);
+  sweeps<T, TG>(c, phi, rhs, G, n, dx, nrelax, 1, reverse, delta, from_zero);
   if (!want_norm && !rc) return;
   double* bmax = (double*)ws_get(c, "bmax", sizeof(double) * stencil_grid(n, 1));
-  const unsigned nb = op_resid<T>(c, rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx), delta);
+  const unsigned nb = op_resid<T, TG>(c, rc ? rscratch : nullptr, bmax, phi, rhs, G, n, T(1) / (dx * dx), delta);
   if (want_norm) reduce<double>(c, bmax, nb, 1, rinf_dev);
   if (rc) {
     const int nrmax = std::min(4, std::max(1, n / 4));
@@ -577,24 +587,24 @@ void relax_dev(mmh_ctx* c, T* phi, const T* rhs, const T* G, T* rscratch, int n,
 
 // Sweeps and relax of a Galerkin (27-point) level: the colour order of the 19-point sweeps, the
 // residual stored in r and restricted into rc with calcresid's plane order rule.
-template <class T>
-void gal_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* A, int n, int ns, bool reverse, bool from_zero = false) {
+template <class T, class TG = T>
+void gal_sweeps(mmh_ctx* c, T* phi, const T* rhs, const TG* A, int n, int ns, bool reverse, bool from_zero = false) {
   const unsigned nb = nblk(cube(n / 2));
   for (int s = 0; s < ns; ++s)
     for (int cc = 0, done = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
       if (s == 0 && from_zero)
-        k_gal_colour<T, true><<<nb, 256, 0, c->stream>>>(phi, rhs, A, n, col >> 2, (col >> 1) & 1, col & 1, 0,
-                                                         (unsigned)done);
-      else k_gal_colour<T><<<nb, 256, 0, c->stream>>>(phi, rhs, A, n, col >> 2, (col >> 1) & 1, col & 1);
+        k_gal_colour<T, true, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, A, n, col >> 2, (col >> 1) & 1, col & 1, 0,
+                                                             (unsigned)done);
+      else k_gal_colour<T, false, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, A, n, col >> 2, (col >> 1) & 1, col & 1);
       done |= 1 << col;
     }
   LAUNCHCHK();
 }
-template <class T>
-void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const T* A, T* r, int n, int nrelax, T* rc, bool from_zero = false) {
-  gal_sweeps<T>(c, phi, rhs, A, n, nrelax, false, from_zero);
-  k_gal_resid<T><<<nblk(cube(n)), 256, 0, c->stream>>>(r, phi, rhs, A, n);
+template <class T, class TG = T>
+void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const TG* A, T* r, int n, int nrelax, T* rc, bool from_zero = false) {
+  gal_sweeps<T, TG>(c, phi, rhs, A, n, nrelax, false, from_zero);
+  k_gal_resid<T, TG><<<nblk(cube(n)), 256, 0, c->stream>>>(r, phi, rhs, A, n);
   const int nrmax = std::min(4, std::max(1, n / 4));
   const int nrl = nrelax > nrmax ? (nrelax % nrmax ? nrelax % nrmax : nrmax) : nrelax;
   k_restrict_resid<T, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(rc, r, n, nrl % (n / 2));
@@ -612,6 +622,8 @@ void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const T* A, T* r, int n, int nr
 // as well is slower, one CU then carrying the 32^3 level).
 template <class T> bool coarse_cycle_applies(const mmh_ctx* c, const Hier<T>& H, int l, const Sched& S) {
   if (S.delta || H.nlev - l > CLEV_MAX) return false;
+  for (int q = l; q < H.nlev; ++q)
+    if (H.Gb[q] || H.Ab[q]) return false;  // the one-workgroup cycle reads FP32 coefficients
   if (S.kind == 0) return c->coarse_fused_ref && !H.A[l] && H.n[l] <= std::min(c->coarse_fused_ref, CTOP_MAX);
   return c->coarse_fused && !S.guard && S.kind == 1 && S.nu_pre == S.nu_post && !(S.nu_top > 0 && H.n[l] == S.ng) &&
          H.n[l] <= std::min(c->coarse_fused, CTOP_MAX);
@@ -1202,14 +1214,22 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   const i64 n3 = cube(n);
   double* sc = Hf.scal + 3 * l;
   const bool zero = c->zero_start && (l > 0 || S.top_zero);
-  if (Hf.A[l]) gal_relax<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1], zero);
+  if (Hf.Ab[l]) gal_relax<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Ab[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1], zero);
+  else if (Hf.A[l]) gal_relax<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1], zero);
+  else if (Hf.Gb[l])
+    relax_dev<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Gb[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1],
+                           sc + 0, false, S.delta, zero);
   else relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1], sc + 0, false,
                  S.delta, zero);
   fill<float>(c, Hf.u[l + 1], 0.f, cube(n / 2));
   vcycle_mixed(c, Hf, Hd, l + 1, ng_top, S);
   k_prolong<float, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
   LAUNCHCHK();
-  if (Hf.A[l]) gal_sweeps<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], n, S.post(n), S.symmetric);
+  if (Hf.Ab[l]) gal_sweeps<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Ab[l], n, S.post(n), S.symmetric);
+  else if (Hf.A[l]) gal_sweeps<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], n, S.post(n), S.symmetric);
+  else if (Hf.Gb[l])
+    relax_dev<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Gb[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr,
+                           sc + 1, S.symmetric, S.delta);
   else relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1,
                  S.symmetric, S.delta);
   reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
@@ -1219,7 +1239,25 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   LAUNCHCHK();
 }
 
-void build_fp32(mmh_ctx* c, Hier<float>& F, Hier<double>& H) {
+// FP64 -> bf16 copy of n values (the coefficients of the 16-bit preconditioner).
+__global__ void k_to_bf16(bf16* __restrict__ dst, const double* __restrict__ src, i64 n) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) dst[p] = to_bf16(src[p]);
+}
+void convert_bf16(mmh_ctx* c, bf16* dst, const double* src, i64 n) {
+  constexpr i64 CH = (i64)1 << 30;  // one launch per 2^30 values (32-bit dispatch sizes)
+  for (i64 o = 0; o < n; o += CH) {
+    const i64 m = std::min(CH, n - o);
+    k_to_bf16<<<nblk(m), 256, 0, c->stream>>>(dst + o, src + o, m);
+  }
+  LAUNCHCHK();
+}
+
+// The FP32 preconditioner's hierarchy.  bf16 (option "precond_bf16", levels of n >= 32): the
+// metric of the finest level and the Galerkin operators are stored in 16 bits, read and widened
+// by the FP32 sweeps and residuals; smaller levels keep FP32 copies (the fused coarse cycle reads
+// those).
+void build_fp32(mmh_ctx* c, Hier<float>& F, Hier<double>& H, bool bf = false) {
   F.nlev = H.nlev;
   F.scal = (double*)ws_get(c, "sfscal", sizeof(double) * 4 * MMH_MAXLEV);
   F.guard = (int*)ws_get(c, "sfguard", sizeof(int) * (MMH_MAXLEV + 1));
@@ -1230,13 +1268,26 @@ void build_fp32(mmh_ctx* c, Hier<float>& F, Hier<double>& H) {
     F.u[l] = wsT<float>(c, t + "u", cube(m));
     F.rhs[l] = wsT<float>(c, t + "rhs", cube(m));
     F.r[l] = wsT<float>(c, t + "r", cube(m));
-    F.G[l] = wsT<float>(c, t + "G", 6 * cube(m));
     F.def[l] = nullptr; F.u1[l] = nullptr;
-    convert_all<double, float>(c, F.G[l], H.G[l], 6 * cube(m));
-    F.A[l] = nullptr;
+    F.A[l] = nullptr; F.Ab[l] = nullptr; F.Gb[l] = nullptr; F.G[l] = nullptr;
+    const bool b16 = bf && m >= 32;
     if (H.A[l]) {
-      F.A[l] = wsT<float>(c, t + "A", 27 * cube(m));
-      convert_all<double, float>(c, F.A[l], H.A[l], 27 * cube(m));
+      if (b16) {
+        bf16* a = (bf16*)ws_get(c, t + "Ab", sizeof(bf16) * 27 * cube(m));
+        convert_bf16(c, a, H.A[l], 27 * cube(m));
+        F.Ab[l] = a;
+      } else {
+        F.A[l] = wsT<float>(c, t + "A", 27 * cube(m));
+        convert_all<double, float>(c, F.A[l], H.A[l], 27 * cube(m));
+      }
+    }
+    if (b16 && !H.A[l]) {
+      bf16* g = (bf16*)ws_get(c, t + "Gb", sizeof(bf16) * 6 * cube(m));
+      convert_bf16(c, g, H.G[l], 6 * cube(m));
+      F.Gb[l] = g;
+    } else if (!b16) {
+      F.G[l] = wsT<float>(c, t + "G", 6 * cube(m));
+      convert_all<double, float>(c, F.G[l], H.G[l], 6 * cube(m));
     }
     LAUNCHCHK();
   }
@@ -1509,13 +1560,20 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
   if (o->krylov) {
     Hier<float> F;
     if (o->mixed) {
-      build_fp32(c, F, H);
-      // the FP32 cycle reads FP32 copies; only the FP64 4^3 level keeps its FP64 operator
-      for (int l = 1; l + 1 < H.nlev; ++l)
-        if (H.A[l]) {
-          ws_free(c, "sd" + std::to_string(l) + "A:8");
-          H.A[l] = nullptr;
-        }
+      build_fp32(c, F, H, o->mixed == 1 && c->precond_bf16);
+      // The FP32 cycle reads its own copies; only the FP64 4^3 level's operator is read in FP64.
+      // The FP64 operators of the other levels stay cached for the next solve unless the Krylov
+      // vectors would not fit beside them (1024^3).
+      size_t freeb = 0, total = 0;
+      HIPCHK(hipMemGetInfo(&freeb, &total));
+      size_t vec = 0;
+      for (const char* k : {"cg_r:8", "cg_ro:8", "cg_p:8", "cg_q:8"}) vec += ws_need(c, k, (size_t)n3 * sizeof(double));
+      if (freeb < vec + ((size_t)8 << 30))
+        for (int l = 1; l + 1 < H.nlev; ++l)
+          if (H.A[l]) {
+            ws_free(c, "sd" + std::to_string(l) + "A:8");
+            H.A[l] = nullptr;
+          }
     }
     try {
       if (o->mixed == 2) refine_fp32(c, H, F, us, rs, n, S, o, st, rden, sums);
@@ -1877,6 +1935,8 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->wave_min_n = value;
     } else if (k == "batch_graph") {
       c->batch_graph = value ? 1 : 0;
+    } else if (k == "precond_bf16") {
+      c->precond_bf16 = value ? 1 : 0;
     } else if (k == "mesh_solve") {
       require(value >= 0 && value <= 2, "set_option: mesh_solve is 0, 1 or 2");
       c->mesh_solve = value;
@@ -1903,7 +1963,7 @@ int mmh_get_option(mmh_ctx* c, const char* key, int* value) {
         {"batch_streams", c->batch_streams}, {"coarse_fused_ref", c->coarse_fused_ref}, {"pcg_graph", c->pcg_graph},
         {"mixed_delta", c->mixed_delta}, {"wave", c->wave}, {"wave_p", c->wave_p}, {"wave_min_n", c->wave_min_n},
         {"batch_graph", c->batch_graph}, {"batch_autotune", c->batch_autotune}, {"mesh_solve", c->mesh_solve},
-        {"mesh_tol", c->mesh_tol}, {"mesh_cycles", c->mesh_cycles}};
+        {"mesh_tol", c->mesh_tol}, {"mesh_cycles", c->mesh_cycles}, {"precond_bf16", c->precond_bf16}};
     auto it = v.find(k);
     if (it == v.end()) throw MmhError(MMH_EINVAL, std::string("get_option: unknown key ") + key);
     *value = it->second;
