@@ -89,7 +89,7 @@ typedef struct {
   mmh_lim_stats lim;
 } mmh_iter_stats;
 
-/* Record of a loss/gradient evaluation (mmh_recon_power_grad_d). */
+/* Record of a loss/gradient evaluation (mmh_recon_power_grad_d, mmh_recon_power_grad_mesh_d). */
 typedef struct {
   double loss;
   int cycles_forward, cycles_adjoint;      /* PCG iterations of the two solves */
@@ -283,6 +283,22 @@ int mmh_power_d(mmh_ctx*, const double* field, int n, int nbins, double* pk, dou
  * (MMH_EINVAL) an input where the compression limiter fires.  rho, grad: device; w, pk: host. */
 int mmh_recon_power_grad_d(mmh_ctx*, const double* rho, int n, int nbins, const double* w, double tol, double* pk,
                            double* grad, mmh_adjoint_stats* st);
+
+/* The same loss and gradient at a curved mesh (no reference counterpart): the outer iteration
+ * that starts from the mesh potential def (device, N^3; the earlier iterations' result, held
+ * fixed) with density rho.  Forward map: the calcdefp right-hand side at (def, rho), whose
+ * expansion limiter, compression gate and +-1/dtaumesh clamp are taken on the branches the
+ * forward pass takes (mmh/limiter.f90:335-375); rgzerosum with det(def) (mmh/gauss1.f90:1259-1407);
+ * L(def, rho) phi = rhs to tol with the curvilinear metric G = rhobar adj(A)/det, A = I + H(def)
+ * (mmh/gauss1.f90:1128-1204, the tangle clamp det <= 1e-5 frozen); defp = nsmooth^2(phi);
+ * delta = defp2delta(def + defp) (the reconstructed density after the iteration, mmh/main.f90:98,
+ * 164).  grad_rho and grad_def (device, N^3, either may be NULL) receive dloss/drho and
+ * dloss/d(def): the implicit-function adjoint (one solve of the symmetric operator) with the
+ * transposed curvilinear operator's metric sensitivity, dG/d(def) through adj(A) and det, the
+ * sensitivity of rgzerosum's det, and the direct dependence of delta on def.  Refuses
+ * (MMH_EINVAL) where the compression limiter fires. */
+int mmh_recon_power_grad_mesh_d(mmh_ctx*, const double* rho, const double* def, int n, int nbins, const double* w,
+                                double tol, double* pk, double* grad_rho, double* grad_def, mmh_adjoint_stats* st);
 
 /* Fisher information as mmh/calcfisherinfo.f90 computes it from nsim realisations of nk binned
  * spectra pk[s*nk + b]: cov (nk x nk, column-major) and covnorm = cov / (mean_i mean_j)

@@ -83,7 +83,8 @@ __global__ void k_mismatch(T* __restrict__ tmp, const T* __restrict__ u, int nf,
 template <class T>
 __global__ __launch_bounds__(256) void k_eflux(T* __restrict__ eflux, const T* __restrict__ tmp, const T* __restrict__ B,
                                                const T* __restrict__ def, const T* __restrict__ u, int nf, int n, T dtold,
-                                               T dtaumesh, T cmpmaxglobal, T pi, double* __restrict__ bstat) {
+                                               T dtaumesh, T cmpmaxglobal, T pi, double* __restrict__ bstat,
+                                               T* __restrict__ gate = nullptr) {
   __shared__ double s[5][256];
   const i64 n3 = (i64)n * n * n;
   const i64 pk = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -113,7 +114,10 @@ __global__ __launch_bounds__(256) void k_eflux(T* __restrict__ eflux, const T* _
 #undef UU
     T ef = (s1p + s2p + s3p - s1m - s2m - s3m) / T(2);
     ef = ef + tmp[pkm];
+    // d eflux / d tmp with every gate frozen at its branch (the adjoint's use; see gate below)
+    T dm = T(1);
     if (det > T(10.0f)) {
+      dm = ef > T(0) ? -T(2) : (ef < T(0) ? T(2) : T(0));
       ef = -T(2) * fabs(ef);
       st[0] = 1.0;
     }
@@ -121,12 +125,20 @@ __global__ __launch_bounds__(256) void k_eflux(T* __restrict__ eflux, const T* _
     T compression = compression_of(T(1) + h.xx, h.xy, h.xz, T(1) + h.yy, h.yz, T(1) + h.zz, pi);
     T cmpmax = cmpmaxglobal * T(1);
     T tlim = T(3) * (T(1) / cmpmax - T(1) / compression) / dtold;
-    if (tlim > T(0)) ef = dmax(T(0.), ef);
+    if (tlim > T(0)) {
+      if (!(ef > T(0))) dm = T(0);
+      ef = dmax(T(0.), ef);
+    }
     st[1] = (double)det;
     st[2] = (double)det;
+    if (!(ef < T(1.) / dtaumesh && ef > -T(1.) / dtaumesh)) dm = T(0);
     ef = dmin(T(1.) / dtaumesh, ef);
     ef = dmax(-T(1.) / dtaumesh, ef);
     eflux[pkm] = ef;
+    // gate: the derivative of this eflux with respect to the smoothed mismatch tmp[pkm] at fixed
+    // def (the expansion limiter's -2|.|, the compression gate's max(0, .) and the clamp, each
+    // taken on the branch the forward pass took; the e.v terms vanish with u(2:4) = 0)
+    if (gate) gate[pkm] = dm;
     st[3] = (double)(ef * dtold);
     st[4] = (double)(ef * dtold);
   }

@@ -898,7 +898,7 @@ template <class T> __global__ void k_pad_speq(T* __restrict__ dst, const T* __re
 // receives the per-block [nexpand, detmax, detmin, fluxmax, fluxmin] (valid after a sync).
 template <class T>
 void limiter_rhs_dev(mmh_ctx* c, T* tmp, const T* def, const T* u, T dtold, T dtaumesh, int nf, int n, T compressmax,
-                     std::vector<double>& hb) {
+                     std::vector<double>& hb, T* gate = nullptr) {
   const T pi = T(3.14159265358979323846264338328);
   const i64 n3 = cube(n);
   k_mismatch<T><<<nblk(n3), 256, 0, c->stream>>>(tmp, u, nf, n3, dtold);
@@ -909,7 +909,7 @@ void limiter_rhs_dev(mmh_ctx* c, T* tmp, const T* def, const T* u, T dtold, T dt
   T* ef = wsT<T>(c, "limE", n3);
   const unsigned nb = nblk(n3);
   double* bst = (double*)ws_get(c, "limstat", sizeof(double) * 5 * nb);
-  k_eflux<T><<<nb, 256, 0, c->stream>>>(ef, tmp, B, def, u, nf, n, dtold, dtaumesh, compressmax, pi, bst);
+  k_eflux<T><<<nb, 256, 0, c->stream>>>(ef, tmp, B, def, u, nf, n, dtold, dtaumesh, compressmax, pi, bst, gate);
   LAUNCHCHK();
   hb.assign(5 * (size_t)nb, 0.0);
   HIPCHK(hipMemcpyAsync(hb.data(), bst, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, c->stream));
@@ -1393,6 +1393,11 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
     const double pq = read_scalar<double>(c, sums + 5);
     if (pq >= 0.0) ++nonneg_pq;
     const double alpha = rz / pq;
+    if (!std::isfinite(alpha)) {
+      // p.Lp = 0 (or not a number): the recurrence has broken down; keep the iterate as it is
+      if (st) { st->cycles = it; st->nonneg_pLp = nonneg_pq; st->nonneg_zr = nonneg_zr; }
+      throw MmhError(MMH_ENOCONV, "solve: breakdown, p.Lp = " + std::to_string(pq));
+    }
     k_cg_update<<<nb, RB, 0, c->stream>>>(r, ro, q, alpha, bm, n3, r32);
     LAUNCHCHK();
     reduce<double>(c, bm, nb, 1, sums + 6);
@@ -1681,27 +1686,39 @@ void power_dev(mmh_ctx* c, const double* f, int n, int nb, double* pk, double* n
   }
 }
 
-void recon_power_dev(mmh_ctx* c, const double* rho, int n, int nb, const double* w, double tol, double* pk_out,
-                     double* grad, mmh_adjoint_stats* st) {
+void recon_power_dev(mmh_ctx* c, const double* rho, const double* defin, int n, int nb, const double* w, double tol,
+                     double* pk_out, double* grad, double* grad_def, mmh_adjoint_stats* st) {
   require(is_pow2(n) && n >= 16 && n <= 1024, "recon_power: n must be a power of two in [16, 1024]");
   require(nb >= 1 && nb <= 64 && w != nullptr, "recon_power: nbins in [1, 64] and weights required");
   require(tol > 0 && tol < 1, "recon_power: tol in (0, 1)");
   const i64 n3 = cube(n);
   const double pi = 3.14159265358979323846264338328;
   double* u = wsT<double>(c, "ad_u", 5 * n3);
-  double* d0 = wsT<double>(c, "ad_def", n3);
   double* rhs = wsT<double>(c, "ad_rhs", n3);
   double* phi = wsT<double>(c, "ad_phi", n3);
   double* w1 = wsT<double>(c, "ad_w1", n3);
   double* dl = wsT<double>(c, "ad_delta", n3);
   double* sd = wsT<double>(c, "pw_sd", n3);
   double* sq = wsT<double>(c, "pw_sq", 2 * (i64)n * n);
+  double* gate = wsT<double>(c, "ad_gate", n3);
+  double* det = wsT<double>(c, "ad_det", n3);
+  double* sums = (double*)ws_get(c, "ad_sums", sizeof(double) * 8);
   k_to_aos<double><<<nblk(n3), 256, 0, c->stream>>>(u, rho, n3);
   LAUNCHCHK();
-  fill<double>(c, d0, 0.0, n3);
-  // forward: rhs, solve, defp = S^2 phi, compression check, delta, binned power
+  const double* d0 = defin;
+  if (!d0) {
+    double* z = wsT<double>(c, "ad_def", n3);
+    fill<double>(c, z, 0.0, n3);
+    d0 = z;
+  }
+  // forward: rhs (gates recorded), its sum (rgzerosum's), solve, defp = S^2 phi, compression
+  // check, delta = D(def + defp), binned power
   std::vector<double> hb;
-  limiter_rhs_dev<double>(c, rhs, d0, u, 1.0, 1.0, 5, n, 10.0, hb);
+  limiter_rhs_dev<double>(c, rhs, d0, u, 1.0, 1.0, 5, n, 10.0, hb, gate);
+  reduce<double>(c, rhs, n3, 0, sums + 0);  // sum r
+  k_det<double><<<nblk(n3), 256, 0, c->stream>>>(det, d0, n, 1.0);
+  LAUNCHCHK();
+  reduce<double>(c, det, n3, 0, sums + 1);  // sum det (rgzerosum's, unclamped)
   fill<double>(c, phi, 0.0, n3);
   mmh_solve_opts o{};
   o.tol = tol; o.max_cycles = 256; o.schedule = 1; o.nu_pre = 2; o.nu_post = 2; o.nu_coarse = 8; o.ng = 0;
@@ -1729,6 +1746,7 @@ void recon_power_dev(mmh_ctx* c, const double* rho, int n, int nb, const double*
   if (ncomp > 0)
     throw MmhError(MMH_EINVAL, "recon_power: the compression limiter fires at this input; the gradient is defined "
                                "only where it is inactive");
+  if (defin) k_add<double><<<nblk(n3), 256, 0, c->stream>>>(w1, defin, n3);  // def + defp
   spectral_out<double>(c, w1, dl, n, 1);
   double* e = upload_edges(c, n, nb);
   std::vector<double> S, N;
@@ -1742,8 +1760,9 @@ void recon_power_dev(mmh_ctx* c, const double* rho, int n, int nb, const double*
     sym[b] = N[b] > 0 ? 2.0 * w[b] / N[b] : 0.0;  // d loss / d delta = 2 N^3 IFFT(W F), W = w/(N_b N^3)
   }
   if (st) st->loss = loss;
-  if (!grad) return;
-  // backward: g_delta = IFFT(sym FFT delta) with sym = 2 w_b / N_b; g_defp = D g_delta; g_phi = S^2 g_defp
+  if (!grad && !grad_def) return;
+  // backward: g_delta = IFFT(sym FFT delta) with sym = 2 w_b / N_b; g_defp = D g_delta (also the
+  // direct part of g_def, delta being D(def + defp)); g_phi = S^2 g_defp, mean removed
   double* symd = (double*)ws_get(c, "ad_sym", sizeof(double) * nb);
   HIPCHK(hipMemcpyAsync(symd, sym.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
   const i64 cnt = (i64)(n / 2 + 1) * n * n;
@@ -1753,8 +1772,13 @@ void recon_power_dev(mmh_ctx* c, const double* rho, int n, int nb, const double*
   k_scale2<double><<<nblk(n3), 256, 0, c->stream>>>(w1, sd, n, n3);
   LAUNCHCHK();
   spectral_out<double>(c, w1, rhs, n, 1);
+  if (grad_def) HIPCHK(hipMemcpyAsync(grad_def, rhs, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
   nsmooth_dev<double>(c, rhs, n, 2);
-  // adjoint solve L lambda = Q g_phi (the operator is symmetric; the solve removes the mean)
+  reduce<double>(c, rhs, n3, 0, sums + 2);
+  k_sub_mean<<<nblk(n3), 256, 0, c->stream>>>(rhs, sums + 2, n3);
+  LAUNCHCHK();
+  // adjoint solve L lambda = P g_phi (L symmetric; the projected right-hand side is mean-free, so
+  // the solve's own rgzerosum leaves it unchanged)
   double* lam = dl;
   fill<double>(c, lam, 0.0, n3);
   mmh_solve_stats s2;
@@ -1763,13 +1787,38 @@ void recon_power_dev(mmh_ctx* c, const double* rho, int n, int nb, const double*
     st->cycles_adjoint = s2.cycles;
     st->relres_adjoint = s2.cycles > 0 ? s2.relres[s2.cycles - 1] : 0.0;
   }
-  // through the metric: g_rhobar, then the 27-point box mean's adjoint
-  k_adj_rhobar<<<nblk(n3), 256, 0, c->stream>>>(w1, lam, phi, n, 1.0);
+  // rgzerosum: s = sum r / sum det; ld = sum lambda det
+  k_mul<<<nblk(n3), 256, 0, c->stream>>>(w1, lam, det, n3);
+  LAUNCHCHK();
+  reduce<double>(c, w1, n3, 0, sums + 3);
+  double hs[4];
+  HIPCHK(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double sfac[2] = {hs[0] / hs[1], hs[3] / hs[1]};
+  HIPCHK(hipMemcpyAsync(sums + 4, sfac, sizeof(sfac), hipMemcpyHostToDevice, c->stream));
+  // through the metric (and rgzerosum's det): g_rhobar, S = dJ/dA
+  double* Sb = wsT<double>(c, "ad_S", 6 * n3);
+  k_adj_metric<<<nblk(n3), 256, 0, c->stream>>>(w1, Sb, lam, phi, d0, rho, n, sums + 4);
+  LAUNCHCHK();
+  if (grad_def) {
+    k_adj_hess<<<nblk(n3), 256, 0, c->stream>>>(grad_def, Sb, n);
+    LAUNCHCHK();
+  }
+  if (!grad) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return;
+  }
   k_box27_adj<<<nblk(n3), 256, 0, c->stream>>>(grad, w1, n);
   LAUNCHCHK();
-  // through the right-hand side: S^4 lambda, the clip mask, 1/30
+  // through the right-hand side: g_r = Q^T lambda = lambda - ld / sum det; then nsmooth, the
+  // eflux gates, nsmooth^3, the clip mask and 1/30
   HIPCHK(hipMemcpyAsync(w1, lam, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
-  nsmooth_dev<double>(c, w1, n, 4);
+  k_sub_val<<<nblk(n3), 256, 0, c->stream>>>(w1, sums + 5, n3);
+  LAUNCHCHK();
+  nsmooth_dev<double>(c, w1, n, 1);
+  k_mul<<<nblk(n3), 256, 0, c->stream>>>(w1, w1, gate, n3);
+  LAUNCHCHK();
+  nsmooth_dev<double>(c, w1, n, 3);
   k_rhs_adj<<<nblk(n3), 256, 0, c->stream>>>(grad, w1, rho, n3, 1.0);
   LAUNCHCHK();
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -2249,7 +2298,14 @@ int mmh_power_d(mmh_ctx* c, const double* field, int n, int nbins, double* pk, d
 
 int mmh_recon_power_grad_d(mmh_ctx* c, const double* rho, int n, int nbins, const double* w, double tol, double* pk,
                            double* grad, mmh_adjoint_stats* st) {
-  return guarded(c, [&] { recon_power_dev(c, rho, n, nbins, w, tol, pk, grad, st); });
+  return guarded(c, [&] { recon_power_dev(c, rho, nullptr, n, nbins, w, tol, pk, grad, nullptr, st); });
+}
+int mmh_recon_power_grad_mesh_d(mmh_ctx* c, const double* rho, const double* def, int n, int nbins, const double* w,
+                                double tol, double* pk, double* grad_rho, double* grad_def, mmh_adjoint_stats* st) {
+  return guarded(c, [&] {
+    require(def != nullptr, "recon_power_grad_mesh: def required (mmh_recon_power_grad_d is the Cartesian mesh)");
+    recon_power_dev(c, rho, def, n, nbins, w, tol, pk, grad_rho, grad_def, st);
+  });
 }
 
 int mmh_fisher_d(const double* pk, int nsim, int nk, const double* cv, double* cov, double* covnorm, double* fisher,

@@ -284,6 +284,19 @@ class Context:
                   pk.ctypes.data_as(C.c_void_p), _ptr(grad), C.byref(st))
         return st.loss, pk, st
 
+    def recon_power_grad_mesh(self, rho, def_, weights, grad_rho=None, grad_def=None, tol=1e-12):
+        """The loss of recon_power_grad for the outer iteration that starts from the mesh potential
+        def_ (held fixed): delta = defp2delta(def_ + defp).  dloss/drho into grad_rho and dloss/d(def)
+        into grad_def where given.  Returns (loss, pk, stats)."""
+        import numpy as np
+        n = _cube_side(rho)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        pk = np.zeros(w.size)
+        st = AdjointStats()
+        self.call("mmh_recon_power_grad_mesh_d", _ptr(rho), _ptr(def_), n, w.size, w.ctypes.data_as(C.c_void_p),
+                  C.c_double(tol), pk.ctypes.data_as(C.c_void_p), _ptr(grad_rho), _ptr(grad_def), C.byref(st))
+        return st.loss, pk, st
+
     def relaxing(self, u, def_, defp, cmax=0.0, dt=1.0, nfluidcmp=5):
         s, T = _sfx(u)
         n = _cube_side(def_)

@@ -207,3 +207,158 @@ def test_fisher_pipeline(ctx):
     assert np.all(np.isfinite(out["fisher"]))
     assert np.allclose(out["cov"], np.cov(pk_all.T, ddof=1), rtol=1e-10, atol=0)
     print("cumulative Fisher information:", np.array2string(out["fisher"], precision=4), flush=True)
+
+
+def curved_state(ctx, n, sigma=1.5, seed=99):
+    """The start of the second outer iteration: def and density after one reconstruction iteration
+    whose mesh solve ran to 1e-10 (option mesh_solve = 2)."""
+    f64 = dict(dtype=torch.float64, device="cuda:0")
+    rho = dev(R.or_lognormal(n, sigma, seed))
+    u = torch.zeros(n ** 3, 5, **f64)
+    u[:, 0] = rho
+    u = u.contiguous().view(-1)
+    d, p = torch.zeros(n ** 3, **f64), torch.zeros(n ** 3, **f64)
+    ctx.set_option("mesh_solve", 2)
+    ctx.set_option("mesh_tol", 10)
+    try:
+        ctx.reconstruct(u, d, p, 1)
+    finally:
+        ctx.set_option("mesh_solve", 0)
+        ctx.set_option("mesh_tol", 8)
+    return u.view(n ** 3, 5)[:, 0].contiguous(), d
+
+
+def fd_mesh(ctx, rho_d, d_d, n, w, cells, eps, phi0, which):
+    """Central differences of the curved-mesh loss (delta = defp2delta(def + defp)) with respect to
+    rho or def at the given cells, formed as in fd_gpu: perturbed solutions phi0 + e with the
+    correction e solved from the perturbed residual, the loss difference from the sum and the
+    difference of the two reconstructed densities."""
+    f64 = dict(dtype=torch.float64, device="cuda:0")
+    b = A.mode_bins(n, NB)
+    nm = np.array([(b == q).sum() for q in range(NB)], dtype=np.float64)
+    W = np.zeros(b.shape)
+    for q in range(NB):
+        W[b == q] = w[q] / (nm[q] * n ** 3)
+    Wd = dev(W)
+    out = []
+    for x in cells:
+        fields = []
+        for sgn in (1.0, -1.0):
+            r, dd = rho_d.clone(), d_d.clone()
+            if which == "rho":
+                r[x] += sgn * eps
+            else:
+                dd[x] += sgn * eps
+            u = torch.zeros(n ** 3, 5, **f64)
+            u[:, 0] = r
+            u = u.contiguous().view(-1)
+            rhs = torch.zeros(n ** 3, **f64)
+            ctx.limiter_rhs(rhs, dd, u)
+            # the correction equation L e = rhs - L phi0; the solve projects its right-hand side with
+            # rgzerosum at this def, which leaves L phi0 (sum 0) unchanged, so e + phi0 solves the
+            # projected equation of the perturbed input
+            res = torch.zeros(n ** 3, **f64)
+            ctx.residual(res, phi0, rhs, dd, u)
+            e = torch.zeros(n ** 3, **f64)
+            # the correction by two rounds of iterative refinement: 1e-7 of the right-hand side, then
+            # 1e-4 of the first round's residual (recomputed in FP64).  One round leaves the finite
+            # differences of d/d(def) a bias of about 1e-9 absolute (4e-5 of the largest at 128^3),
+            # and single solves floor near 1e-8 in FP64
+            for rnd, ctol in ((0, 1e-7), (1, 1e-4)):
+                de = torch.zeros(n ** 3, **f64)
+                r = res
+                if rnd:
+                    r = torch.zeros(n ** 3, **f64)
+                    ctx.residual(r, e, res, dd, u)
+                ctx.set_option("galerkin", 0)  # as fd_gpu: its floor on such right-hand sides is known
+                try:
+                    st, ok = ctx.solve(de, r, dd, u, tol=ctol, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
+                                       nu_coarse=8, nu=5, krylov=True)
+                finally:
+                    ctx.set_option("galerkin", 1)
+                assert ok, st.relres[st.cycles - 1]
+                e = e + de
+            fields.append((e, dd))
+        (ep, dp), (em, dm) = fields
+        diff, summ = ep - em, 2 * phi0 + ep + em
+        for v in (diff, summ):
+            ctx.nsmooth(v)
+            ctx.nsmooth(v)
+        diff = diff + (dp - dm)
+        summ = summ + (dp + dm)
+        ddl, sdl = torch.zeros_like(diff), torch.zeros_like(summ)
+        ctx.defp2delta(diff, ddl)
+        ctx.defp2delta(summ, sdl)
+        Fd = torch.fft.fftn(ddl.view(n, n, n))
+        Fs = torch.fft.fftn(sdl.view(n, n, n))
+        out.append(float((Wd * (Fd * Fs.conj()).real).sum()) / (2 * eps))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_curved_mesh_gradient_central_differences(ctx, n):
+    """The adjoint at a curved mesh (mmh_recon_power_grad_mesh_d, the second outer iteration of the
+    sigma = 1.5, seed 99 field): dloss/drho and dloss/d(def) against central differences
+    (eps = 1e-6) at 16 random cells each, within 1e-5 of the largest finite difference.  The
+    Cartesian-mesh entry and the mesh entry at def = 0 give the same loss and dloss/drho."""
+    print("case curved-mesh adjoint FD n=%d" % n, flush=True)
+    f64 = dict(dtype=torch.float64, device="cuda:0")
+    rho_d, d_d = curved_state(ctx, n)
+    w = weights()
+    gr, gd = torch.zeros(n ** 3, **f64), torch.zeros(n ** 3, **f64)
+    loss, pk, st = ctx.recon_power_grad_mesh(rho_d, d_d, w, gr, gd, tol=1e-12)
+    print("loss %.6e cycles fwd %d adj %d ncompress %d; |def| max %.3f" % (
+        loss, st.cycles_forward, st.cycles_adjoint, st.ncompress, float(d_d.abs().max())), flush=True)
+    assert st.ncompress == 0
+    # the unperturbed solution
+    u = torch.zeros(n ** 3, 5, **f64)
+    u[:, 0] = rho_d
+    u = u.contiguous().view(-1)
+    rhs = torch.zeros(n ** 3, **f64)
+    ctx.limiter_rhs(rhs, d_d, u)
+    phi0 = torch.zeros(n ** 3, **f64)
+    st0, ok = ctx.solve(phi0, rhs, d_d, u, tol=1e-13, max_cycles=250, schedule=1, nu_pre=2, nu_post=2,
+                        nu_coarse=8, nu=5, krylov=True)
+    assert ok
+    rng = np.random.default_rng(n)
+    for which, g in (("rho", gr), ("def", gd)):
+        cells = rng.choice(n ** 3, 16, replace=False)
+        fd = fd_mesh(ctx, rho_d, d_d, n, w, cells, 1e-6, phi0, which)
+        ga = host(g)[cells]
+        err = np.abs(ga - fd).max() / np.abs(fd).max()
+        print("  d/d%s: max |adjoint - FD| / max |FD| = %.2e over 16 cells" % (which, err), flush=True)
+        assert err <= 1e-5
+    # def = 0 reduces to the Cartesian entry
+    z = torch.zeros(n ** 3, **f64)
+    g1, g2 = torch.zeros(n ** 3, **f64), torch.zeros(n ** 3, **f64)
+    l1, _, _ = ctx.recon_power_grad(rho_d, w, g1, tol=1e-12)
+    l2, _, _ = ctx.recon_power_grad_mesh(rho_d, z, w, g2, None, tol=1e-12)
+    assert abs(l1 - l2) <= 1e-12 * abs(l1)
+    assert float((g1 - g2).abs().max()) <= 1e-9 * float(g1.abs().max())
+
+
+def test_torch_autograd_harness_curved_mesh(ctx):
+    """backward() through the curved-mesh autograd.Function gives both adjoint gradients, and the
+    directional derivative along smooth directions in (rho, def) jointly matches central differences
+    (eps = 1e-6) to 1e-5."""
+    from torch_recon import recon_power_loss_mesh
+    n = 64
+    print("case torch harness curved mesh n=%d" % n, flush=True)
+    rho0, d0 = curved_state(ctx, n)
+    rho, d = rho0.clone().requires_grad_(), d0.clone().requires_grad_()
+    w = torch.tensor(weights(), dtype=torch.float64)
+    loss = recon_power_loss_mesh(rho, d, w, ctx)
+    loss.backward()
+    rng = np.random.default_rng(11)
+    vr, vd = rng.standard_normal(n ** 3), rng.standard_normal(n ** 3)
+    for _ in range(3):
+        R.or_nsmooth(vr, n)
+        R.or_nsmooth(vd, n)
+    vr, vd = dev(vr), dev(vd * 0.1)
+    eps = 1e-6
+    lp, _, _ = ctx.recon_power_grad_mesh(rho0 + eps * vr, d0 + eps * vd, w.numpy(), None, None, tol=1e-13)
+    lm, _, _ = ctx.recon_power_grad_mesh(rho0 - eps * vr, d0 - eps * vd, w.numpy(), None, None, tol=1e-13)
+    fd = (lp - lm) / (2 * eps)
+    gv = float((rho.grad * vr).sum() + (d.grad * vd).sum())
+    print("directional: adjoint %.10e FD %.10e" % (gv, fd), flush=True)
+    assert abs(gv - fd) <= 1e-5 * abs(fd)

@@ -20,3 +20,25 @@ class ReconPowerLoss(torch.autograd.Function):
 
 def recon_power_loss(rho, weights, lib_ctx, tol=1e-12):
     return ReconPowerLoss.apply(rho, weights, lib_ctx, tol)
+
+
+class ReconPowerLossMesh(torch.autograd.Function):
+    """The loss at a curved mesh (mmh_recon_power_grad_mesh_d): inputs rho and def, both
+    differentiable."""
+
+    @staticmethod
+    def forward(ctx, rho, defn, weights, lib_ctx, tol):
+        gr, gd = torch.zeros_like(rho), torch.zeros_like(defn)
+        loss, pk, st = lib_ctx.recon_power_grad_mesh(rho.detach().contiguous(), defn.detach().contiguous(),
+                                                     weights.detach().cpu().numpy(), gr, gd, tol=tol)
+        ctx.save_for_backward(gr, gd)
+        return rho.new_tensor(loss)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        gr, gd = ctx.saved_tensors
+        return grad_out * gr, grad_out * gd, None, None, None
+
+
+def recon_power_loss_mesh(rho, defn, weights, lib_ctx, tol=1e-12):
+    return ReconPowerLossMesh.apply(rho, defn, weights, lib_ctx, tol)
