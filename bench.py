@@ -255,6 +255,29 @@ def batched_leg(ctx, torch, nu, nu_top, nu_coarse):
     batched()  # warm-up: workspaces and launch-shape tuning
     tb, st = batched()
     print("bench: batched 100 realisations %.3f s" % tb, file=sys.stderr, flush=True)
+    # the preconditioner's coarse levels one workgroup per grid up to 8^3 and 16^3 (the default: 4^3)
+    tfused = {}
+    for fz in (0, 8, 16):
+        ctx.set_option("batch_coarse_fused", fz)
+        try:
+            batched()
+            tfused[fz] = batched()[0]
+        finally:
+            ctx.set_option("batch_coarse_fused", 4)
+        print("bench: batched 100 realisations, coarse levels <= %d^3 one workgroup per grid: %.3f s" % (fz, tfused[fz]),
+              file=sys.stderr, flush=True)
+    # the small groups solved whole in one workgroup per grid (option batch_persistent): up to 32^3,
+    # and up to 64^3
+    tpers = {}
+    for pers in (32, 64):
+        ctx.set_option("batch_persistent", pers)
+        try:
+            batched()
+            tpers[pers] = batched()
+        finally:
+            ctx.set_option("batch_persistent", 0)
+        print("bench: batched 100 realisations, one workgroup per grid up to %d^3: %.3f s" % (pers, tpers[pers][0]),
+              file=sys.stderr, flush=True)
     looped()
     tl = looped()
     print("bench: looped 100 realisations %.3f s" % tl, file=sys.stderr, flush=True)
@@ -262,6 +285,12 @@ def batched_leg(ctx, torch, nu, nu_top, nu_coarse):
     return {"workload": "40x32^3 + 40x64^3 + 20x128^3 lognormal, sigma U[0.5,2], seeds 0..99, to 1e-8 (FP64 PCG, "
                         "the main solve's preconditioner)",
             "batched_realizations_per_s": round(len(sizes) / tb, 2), "looped_realizations_per_s": round(len(sizes) / tl, 2),
+            "batched_coarse_levels_fused_realizations_per_s": {str(k): round(len(sizes) / v, 2)
+                                                               for k, v in sorted(tfused.items())},
+            "batched_one_workgroup_per_grid_upto_32_realizations_per_s": round(len(sizes) / tpers[32][0], 2),
+            "batched_one_workgroup_per_grid_upto_64_realizations_per_s": round(len(sizes) / tpers[64][0], 2),
+            "pcg_iterations_min_max_one_workgroup_per_grid": [min(s.cycles for s in tpers[64][1]),
+                                                              max(s.cycles for s in tpers[64][1])],
             "batched_speedup": round(tl / tb, 3), "pcg_iterations_min_max": [min(its), max(its)]}
 
 

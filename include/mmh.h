@@ -145,7 +145,8 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 (divergence guard, 32 + 32 sweeps on 4^3) and 2 runs the Krylov solve, both until
  *                 max|rhs - L u| / max|rhs| <= 10^-"mesh_tol" (MMH_ENOCONV if not reached)
  *   "mesh_tol"    8 (default): the tolerance exponent of "mesh_solve" 1 and 2
- *   "batch_graph", "batch_autotune", "batch_streams": see mmh_solve_batch_d */
+ *   "batch_graph", "batch_autotune", "batch_streams", "batch_coarse_fused", "batch_persistent":
+ *                 see mmh_solve_batch_d */
 int mmh_set_option(mmh_ctx* ctx, const char* key, int value);
 /* Current value of an option above, or the read-only "mesh_cycles": the cycles (mesh_solve 1) or
  * iterations (mesh_solve 2) of the last mesh solve inside calcdefp.  No reference counterpart. */
@@ -209,7 +210,12 @@ typedef struct {
  * Krylov form (opts: krylov = 1, schedule = 1, nu_pre = nu_post, mixed = 0); every item's result
  * is bit-identical to its mmh_solve_d.  The groups iterate side by side, each on a stream of its
  * own.  Options "batch_graph", "batch_autotune" and "batch_streams" (mmh_set_option) switch graph
- * replay, tuning and the group streams off (results unchanged). */
+ * replay, tuning and the group streams off (results unchanged).  "batch_coarse_fused" (4 by
+ * default, 8, 16, 32 or 0) runs the preconditioner's levels up to that size with one workgroup per
+ * grid, one launch for all of them (results unchanged).  "batch_persistent" (0 by default, 8 to 64)
+ * solves groups of n up to that size whole in one workgroup per grid, one launch for the entire
+ * PCG; its sums run in another order, so results agree with the default to rounding, not bit for
+ * bit, and it is slower than the default on MI355X (one CU then carries each grid's fine levels). */
 int mmh_solve_batch_d(mmh_ctx*, int count, const int* n, double* const* u, double* const* rhs,
                       double* const* defpot, double* const* ubig, int nu, const mmh_solve_opts* opts,
                       mmh_batch_stats* st);

@@ -29,6 +29,7 @@ struct BatchGroup {
   double* ssum = nullptr;  // set-up sums: det sums, rhs sums, max|rhs| (B each)
   double *sums, *rz, *pq, *rmax, *rden, *rznew, *zro, *alpha, *beta, *relres;
   int *active, *iters;
+  double* pscal = nullptr;  // one-workgroup-per-grid cycles: their level scalars (3 CLEV_MAX per grid)
   std::string tag;
 };
 
@@ -108,7 +109,43 @@ void batch_gal_sweeps(mmh_ctx* c, BatchGroup& g, double* phi, const double* rhs,
 // Symmetric V(nu, nu) cycle of the batch from level l, the arithmetic of vcycle<double> with the
 // Krylov schedule (no guard, reverse colour order on the way up, mean removal on every level).
 // act: grids y with act[y] = 0 are left alone (null: every grid).
+// Levels l.. of the group as a stack table for the one-workgroup-per-grid kernels (level l's u and
+// rhs are u0 and rhs0 when l = 0).
+StackLevels stack_levels(const BatchGroup& g, int l, double* u0, double* rhs0, const Sched& S) {
+  StackLevels L{};
+  L.nl = g.nlev - l;
+  for (int q = 0; q < L.nl; ++q) {
+    const int k = l + q;
+    L.u[q] = k == 0 ? u0 : g.u[k];
+    L.rhs[q] = k == 0 ? rhs0 : g.rhs[k];
+    L.r[q] = g.r[k];
+    L.G[q] = g.G[k];
+    L.A[q] = g.A[k];
+    L.m[q] = g.m[k];
+    L.sw[q] = S.pre(g.m[k]);
+    L.dx[q] = g.dx[k];
+  }
+  L.coarse = S.coarse();
+  L.symmetric = S.symmetric ? 1 : 0;
+  L.scal = g.pscal;
+  return L;
+}
+
+// Whether levels l.. run as one workgroup per grid (k_coarse_stack): levels up to the size set by
+// option "batch_coarse_fused", with a schedule that sweeps as often up as down on each of them.
+bool batch_coarse_applies(const mmh_ctx* c, const BatchGroup& g, int l, const Sched& S) {
+  if (!c->batch_coarse_fused || g.m[l] > std::min(c->batch_coarse_fused, CTOP_MAX) || g.nlev - l > CLEV_MAX) return false;
+  for (int k = l; k < g.nlev; ++k)
+    if (S.pre(g.m[k]) != S.post(g.m[k])) return false;
+  return true;
+}
+
 void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, const Sched& S, const int* act = nullptr) {
+  if (batch_coarse_applies(c, g, l, S)) {
+    k_coarse_stack<<<g.B, CW, 0, c->stream>>>(stack_levels(g, l, u0, rhs0, S), act);
+    LAUNCHCHK();
+    return;
+  }
   const int m = g.m[l];
   const i64 m3 = cube(m);
   double* u = l == 0 ? u0 : g.u[l];
@@ -249,6 +286,7 @@ void batch_alloc(mmh_ctx* c, BatchGroup& g) {
   }
   const i64 n3 = cube(g.n);
   g.x = wsT<double>(c, t + "x", B * n3);
+  g.pscal = wsT<double>(c, t + "pscal", B * 3 * CLEV_MAX);
   g.b = wsT<double>(c, t + "b", B * n3);
   g.cr = wsT<double>(c, t + "cr", B * n3);
   g.ro = wsT<double>(c, t + "ro", B * n3);
@@ -385,6 +423,7 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
     int it = 0;
     bool done = false;
     std::unique_ptr<Replay> graph;
+    bool persistent = false;  // the whole PCG of each grid in one workgroup (k_pcg_stack)
   };
   struct Streams {  // group streams and pinned flags, released on every exit path
     mmh_ctx* c;
@@ -424,12 +463,18 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
       k_zero_inactive<<<dim3(nblk(n3), g.B), 256, 0, c->stream>>>(g.x, g.active, n3);
       LAUNCHCHK();
       fill<double>(c, g.relres, 0.0, g.B);
-      if (c->batch_autotune) batch_autotune(c, g);
-      // r = b - L x, z = M r, rz = <r, z>, p = z
-      op_resid<double>(c, g.cr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, false, g.B, 1);
-      batch_precond(c, g, S);
-      batch_dot2(c, g, g.z, g.cr, g.cr, g.rz, g.zro);  // zro: set again before its first use
-      HIPCHK(hipMemcpyAsync(g.p, g.z, sizeof(double) * g.B * n3, hipMemcpyDeviceToDevice, c->stream));
+      // the whole solve in one workgroup per grid: grids up to 64^3 (block sums fit the kernel's
+      // LDS) and a symmetric schedule (the one-workgroup cycle sweeps as often up as down)
+      R.persistent = g.n <= c->batch_persistent && g.n <= 64 && g.nlev <= CLEV_MAX;
+      for (int l = 0; l < g.nlev; ++l) R.persistent = R.persistent && S.pre(g.m[l]) == S.post(g.m[l]);
+      if (!R.persistent) {
+        if (c->batch_autotune) batch_autotune(c, g);
+        // r = b - L x, z = M r, rz = <r, z>, p = z
+        op_resid<double>(c, g.cr, g.bm, g.x, g.b, g.G[0], g.n, 1.0, false, g.B, 1);
+        batch_precond(c, g, S);
+        batch_dot2(c, g, g.z, g.cr, g.cr, g.rz, g.zro);  // zro: set again before its first use
+        HIPCHK(hipMemcpyAsync(g.p, g.z, sizeof(double) * g.B * n3, hipMemcpyDeviceToDevice, c->stream));
+      }
       hipEvent_t e = nullptr;
       HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       streams.ev.push_back(e);
@@ -467,6 +512,21 @@ void solve_batch_dev(mmh_ctx* c, int count, const int* ns, double* const* u, dou
       if (R.done) continue;
       c->stream = R.s;
       S.ng = R.g.n;
+      if (R.persistent) {
+        // the whole solve of every grid, one workgroup each
+        BatchGroup& g = R.g;
+        PcgStack P{};
+        P.n = g.n; P.B = g.B; P.max_it = o->max_cycles; P.tol = o->tol;
+        P.L = stack_levels(g, 0, g.z, g.cr, S);
+        P.x = g.x; P.b = g.b; P.cr = g.cr; P.ro = g.ro; P.p = g.p; P.q = g.q; P.z = g.z;
+        P.rden = g.rden; P.active = g.active; P.iters = g.iters; P.relres = g.relres;
+        k_pcg_stack<<<g.B, CW, 0, c->stream>>>(P);
+        LAUNCHCHK();
+        R.it = o->max_cycles;
+        copy_flags(R);
+        pending = true;
+        continue;
+      }
       // one iteration captured as a graph (batch_alloc sized every buffer it uses) and replayed
       if (c->batch_graph) {
         if (!R.graph) {

@@ -186,10 +186,10 @@ __device__ void cw_bottom(T* u, const T* rhs, const T* G, const T* A, T dx, int 
   cw_zerosum<T>(u, 4, scal, guard, mode, blk, blk2, s);
 }
 
-template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCycle<T> C) {
-  __shared__ double s[CW / RB][RB];
-  __shared__ double blk[64], blk2[64];  // block values of a level's sums (32^3: 16 blocks)
-  __shared__ double wm[CW / 64];
+// The cycle, run by the whole workgroup (k_coarse_cycle, and the preconditioner of the
+// one-workgroup-per-grid Krylov solve); blk/blk2 hold a level's block sums (n^3 / 2048 of them).
+template <class T>
+__device__ void cw_cycle(const CoarseCycle<T>& C, double (*s)[RB], double* blk, double* blk2, double* wm) {
   // down: pre-smoothing, residual (and r1), restriction into the next level's rhs, zero next u
   for (int l = 0; l + 1 < C.nl; ++l) {
     const int n = C.n[l];
@@ -250,6 +250,173 @@ template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCy
     }
     const int mode = (C.ref && !(C.top && l == 0)) ? 1 : 0;
     cw_zerosum<T>(C.u[l], n, C.scal[l], C.guard[l], mode, blk, blk2, s);
+  }
+}
+
+template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCycle<T> C) {
+  __shared__ double s[CW / RB][RB];
+  __shared__ double blk[64], blk2[64];  // block values of a level's sums (32^3: 16 blocks)
+  __shared__ double wm[CW / 64];
+  cw_cycle<T>(C, s, blk, blk2, wm);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The whole flexible-PCG solve of one grid in one workgroup (batched solve, grids up to 64^3):
+// workgroup g solves grid g of a stack of B equal-size grids to its own tolerance, with the
+// preconditioner of the multi-launch batched solve (one symmetric V-cycle, cw_cycle, from the
+// finest level down) and the same recurrence (Polak-Ribiere beta, FP64 throughout).  No grid waits
+// for another: the launch replaces the batched iteration loop, its launches and its host round
+// trips.  Sums are blocked like reduce()'s but differ in order from the multi-launch solve's, so
+// the iterates agree with it to rounding, not bit for bit.
+
+// sum of a[p] * b[p] over n3 points by the workgroup (every thread gets it): thread partials in
+// point order, then the workgroup's halving tree.
+__device__ double cw_dot(const double* a, const double* b, i64 n3, double (*s)[RB]) {
+  double acc = 0.0;
+  for (i64 p = threadIdx.x; p < n3; p += CW) acc = acc + a[p] * b[p];
+  double* f = &s[0][0];
+  f[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = CW / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) f[threadIdx.x] = f[threadIdx.x] + f[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double r = f[0];
+  __syncthreads();
+  return r;
+}
+
+// The levels of a stack of B equal-size grids (batched solve), grid y of level l at
+// u[l] + y m_l^3 (G: 6 m_l^3 per grid, A: 27 m_l^3), as the batched launches store them.
+struct StackLevels {
+  int nl, coarse, symmetric;
+  double* u[CLEV_MAX];
+  double* rhs[CLEV_MAX];
+  double* r[CLEV_MAX];
+  const double* G[CLEV_MAX];
+  const double* A[CLEV_MAX];
+  int m[CLEV_MAX], sw[CLEV_MAX];
+  double dx[CLEV_MAX];
+  double* scal;  // 3 CLEV_MAX per grid: the cycle's per-level scalars
+};
+
+// Thread 0 writes grid g's cycle table (Krylov schedule: no guard, mean removal on every level).
+__device__ void stack_table(CoarseCycle<double>& C, const StackLevels& L, int g) {
+  if (threadIdx.x != 0) return;
+  C.nl = L.nl;
+  for (int l = 0; l < L.nl; ++l) {
+    const i64 m3 = cube_d(L.m[l]);
+    C.u[l] = L.u[l] + (i64)g * m3;
+    C.rhs[l] = L.rhs[l] + (i64)g * m3;
+    C.r[l] = L.r[l] + (i64)g * m3;
+    C.G[l] = L.G[l] ? L.G[l] + (i64)g * 6 * m3 : nullptr;
+    C.A[l] = L.A[l] ? L.A[l] + (i64)g * 27 * m3 : nullptr;
+    C.scal[l] = L.scal + (i64)g * 3 * CLEV_MAX + 3 * l;
+    C.guard[l] = nullptr;
+    C.dx[l] = L.dx[l];
+    C.n[l] = L.m[l];
+    C.sw[l] = L.sw[l];
+  }
+  C.coarse = L.coarse;
+  C.ref = 0;
+  C.top = 0;
+  C.symmetric = L.symmetric;
+  C.mixed = 0;
+}
+
+// The coarse end of the batched preconditioning cycle: workgroup g runs grid g's levels (from the
+// table's level 0 down to 4^3 and back) as k_coarse_cycle runs a single grid's, bit-identical to
+// the batched launches; grids with act[g] = 0 (converged) are skipped.
+__global__ __launch_bounds__(CW) void k_coarse_stack(StackLevels L, const int* __restrict__ act) {
+  __shared__ double s[CW / RB][RB];
+  __shared__ double blk[64], blk2[64];
+  __shared__ double wm[CW / 64];
+  // the level table once per workgroup in LDS (in registers it would take a scratch copy per lane)
+  __shared__ CoarseCycle<double> C;
+  if (act && !act[blockIdx.x]) return;
+  stack_table(C, L, blockIdx.x);
+  __syncthreads();
+  cw_cycle<double>(C, s, blk, blk2, wm);
+}
+
+struct PcgStack {
+  int n, B, max_it;
+  double tol;
+  StackLevels L;                        // level 0: z (u) and the residual (rhs) of the finest level
+  double *x, *b, *cr, *ro, *p, *q, *z;  // finest-level vectors
+  const double* rden;
+  int *active, *iters;
+  double* relres;
+};
+
+__global__ __launch_bounds__(CW) void k_pcg_stack(PcgStack P) {
+  __shared__ double s[CW / RB][RB];
+  __shared__ double blk[256], blk2[256];  // up to 64^3: 128 blocks
+  __shared__ double wm[CW / 64];
+  __shared__ CoarseCycle<double> C;
+  const int g = blockIdx.x;
+  const int n = P.n;
+  const i64 n3 = cube_d(n);
+  const i64 o = (i64)g * n3;
+  double *x = P.x + o, *b = P.b + o, *r = P.cr + o, *ro = P.ro + o, *p = P.p + o, *q = P.q + o, *z = P.z + o;
+  stack_table(C, P.L, g);
+  for (i64 t = threadIdx.x; t < n3; t += CW) {
+    x[t] = 0.0;
+    r[t] = b[t];
+  }
+  __syncthreads();
+  if (!P.active[g]) {  // converged from the start (the projected right-hand side vanishes)
+    if (threadIdx.x == 0) P.iters[g] = 0;
+    return;
+  }
+  auto precond = [&] {
+    for (i64 t = threadIdx.x; t < n3; t += CW) z[t] = 0.0;
+    __syncthreads();
+    cw_cycle<double>(C, s, blk, blk2, wm);
+  };
+  precond();
+  double rz = cw_dot(z, r, n3, s);
+  for (i64 t = threadIdx.x; t < n3; t += CW) p[t] = z[t];
+  __syncthreads();
+  const double dx2 = P.L.dx[0] * P.L.dx[0];
+  for (int it = 0; it < P.max_it; ++it) {
+    // q = L p (the 19-point operator of the finest level)
+    for (i64 t = threadIdx.x; t < n3; t += CW) {
+      int i, j, k;
+      point_of<1>(t, n, i, j, k);
+      double t1, t2, td;
+      stencil19<double, 1>(p, C.G[0], n3, n, i, j, k, t1, t2, td);
+      q[t] = (t1 + t2 + td * p[t]) / dx2;
+    }
+    __syncthreads();
+    const double pq = cw_dot(p, q, n3, s);
+    const double alpha = rz / pq;
+    double rmax = 0.0;
+    for (i64 t = threadIdx.x; t < n3; t += CW) {
+      x[t] = x[t] + alpha * p[t];
+      ro[t] = r[t];
+      r[t] = r[t] - alpha * q[t];
+      rmax = dmax(rmax, fabs(r[t]));
+    }
+    __syncthreads();
+    rmax = cw_max(rmax, wm);
+    const double rr = rmax / P.rden[g];
+    if (threadIdx.x == 0) {
+      P.relres[g] = rr;
+      P.iters[g] = it + 1;
+    }
+    if (rr <= P.tol) {
+      if (threadIdx.x == 0) P.active[g] = 0;
+      break;
+    }
+    if (!(rr == rr)) break;  // breakdown (p.Lp = 0): the grid stays active, reported as not converged
+    precond();
+    const double rzn = cw_dot(z, r, n3, s);
+    const double zro = cw_dot(z, ro, n3, s);
+    const double beta = (rzn - zro) / rz;
+    for (i64 t = threadIdx.x; t < n3; t += CW) p[t] = z[t] + beta * p[t];
+    __syncthreads();
+    rz = rzn;
   }
 }
 

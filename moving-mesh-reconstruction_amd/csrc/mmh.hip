@@ -60,6 +60,8 @@ struct mmh_ctx {
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
   int batch_autotune = 1;  // batched solve: pick the colour kernel's launch shape by timing
   int batch_streams = 1;   // batched solve: size groups iterate side by side on streams of their own
+  int batch_coarse_fused = 4;  // batched solve: levels <= this size one workgroup per grid (option "batch_coarse_fused", 0 off)
+  int batch_persistent = 0;  // batched solve: groups of n <= this solve in one workgroup per grid (option "batch_persistent")
   int precond_bf16 = 1;    // mixed Krylov solve: 16-bit coefficients in the FP32 preconditioner (option "precond_bf16")
   int mesh_solve = 0;      // calcdefp's mesh solve: 0 one reference V-cycle, 1 V(2,2) cycles to 10^-mesh_tol,
                            // 2 the Krylov solve to 10^-mesh_tol (option "mesh_solve")
@@ -1984,6 +1986,14 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->wave_min_n = value;
     } else if (k == "batch_graph") {
       c->batch_graph = value ? 1 : 0;
+    } else if (k == "batch_coarse_fused") {
+      require(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
+              "set_option: batch_coarse_fused is 0 (off) or the largest level size run one workgroup per grid, 4 to 32");
+      c->batch_coarse_fused = value;
+    } else if (k == "batch_persistent") {
+      require(value == 0 || value == 8 || value == 16 || value == 32 || value == 64,
+              "set_option: batch_persistent is 0 (off) or the largest grid size solved that way, 8 to 64");
+      c->batch_persistent = value;
     } else if (k == "precond_bf16") {
       c->precond_bf16 = value ? 1 : 0;
     } else if (k == "mesh_solve") {
@@ -2012,7 +2022,8 @@ int mmh_get_option(mmh_ctx* c, const char* key, int* value) {
         {"batch_streams", c->batch_streams}, {"coarse_fused_ref", c->coarse_fused_ref}, {"pcg_graph", c->pcg_graph},
         {"mixed_delta", c->mixed_delta}, {"wave", c->wave}, {"wave_p", c->wave_p}, {"wave_min_n", c->wave_min_n},
         {"batch_graph", c->batch_graph}, {"batch_autotune", c->batch_autotune}, {"mesh_solve", c->mesh_solve},
-        {"mesh_tol", c->mesh_tol}, {"mesh_cycles", c->mesh_cycles}, {"precond_bf16", c->precond_bf16}};
+        {"mesh_tol", c->mesh_tol}, {"mesh_cycles", c->mesh_cycles}, {"precond_bf16", c->precond_bf16},
+        {"batch_persistent", c->batch_persistent}, {"batch_coarse_fused", c->batch_coarse_fused}};
     auto it = v.find(k);
     if (it == v.end()) throw MmhError(MMH_EINVAL, std::string("get_option: unknown key ") + key);
     *value = it->second;

@@ -229,3 +229,56 @@ def test_batch_group_streams_equal_one_stream(ctx, probs):
     assert s1 == s0
     for a, b in zip(x1, x0):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("pers,nu_top", [(32, 0), (64, 1)])
+def test_batch_one_workgroup_per_grid(ctx, probs, pers, nu_top):
+    """Groups of n <= batch_persistent solved whole in one workgroup per grid (k_pcg_stack) reach
+    the tolerance and agree with the multi-launch batched solve to 1e-6 of the solution (the two
+    differ in summation order and in the first sweep from zero); the other groups are unchanged;
+    the true residual of every realisation is within the tolerance."""
+    print("case batch persistent <= %d nu_top %d" % (pers, nu_top), flush=True)
+    ctx.set_option("nu_top", nu_top)
+    try:
+        xs0, st0, ok0 = run_batch(ctx, probs)
+        ctx.set_option("batch_persistent", pers)
+        xs1, st1, ok1 = run_batch(ctx, probs)
+    finally:
+        ctx.set_option("batch_persistent", 0)
+        ctx.set_option("nu_top", 0)
+    assert ok0 and ok1
+    for x0, x1, s0, s1, (u, r, d), case in zip(xs0, xs1, st0, st1, probs, CASES):
+        n = case[0]
+        assert s1.converged and s1.relres <= SOLVE["tol"] and np.isfinite(x1).all()
+        if n > pers:
+            assert np.array_equal(x0, x1) and s0.cycles == s1.cycles
+        else:
+            assert abs(s1.cycles - s0.cycles) <= 2, (case, s0.cycles, s1.cycles)
+            assert np.abs(x1 - x0).max() <= 1e-6 * np.abs(x0).max(), case
+        # true residual of the returned solution against the projected right-hand side
+        b = r.clone()
+        ctx.solve(torch.zeros_like(r), b, d, u, tol=1e-30, max_cycles=1, schedule=1, nu_pre=1, nu_post=1,
+                  nu_coarse=1, nu=5)
+        res = torch.zeros_like(r)
+        ctx.residual(res, torch.from_numpy(x1).cuda(), b, d, u)
+        assert float(res.abs().max()) <= 1.05 * SOLVE["tol"] * float(b.abs().max()), case
+
+
+@pytest.mark.parametrize("fused,galerkin", [(4, 1), (8, 1), (16, 0), (32, 1)])
+def test_batch_coarse_levels_one_workgroup_per_grid(ctx, probs, fused, galerkin):
+    """The preconditioner's levels up to `fused` run one workgroup per grid (k_coarse_stack): every
+    realisation bit-identical to the batched launches of those levels (batch_coarse_fused 0)."""
+    print("case batch_coarse_fused %d galerkin %d" % (fused, galerkin), flush=True)
+    ctx.set_option("galerkin", galerkin)
+    try:
+        ctx.set_option("batch_coarse_fused", 0)
+        x0, s0, ok0 = run_batch(ctx, probs)
+        ctx.set_option("batch_coarse_fused", fused)
+        x1, s1, ok1 = run_batch(ctx, probs)
+    finally:
+        ctx.set_option("batch_coarse_fused", 4)
+        ctx.set_option("galerkin", 1)
+    assert ok0 and ok1
+    for a, b, t0, t1, case in zip(x0, x1, s0, s1, CASES):
+        assert t0.cycles == t1.cycles and t0.relres == t1.relres, case
+        assert np.array_equal(a, b), case
