@@ -85,9 +85,10 @@ def cpu_baseline(n, rho_host, rhs_host, cycles=2):
                           cycles, n, threads, [round(x, 3) for x in out["t"]])}
 
 
-def traffic_from_profile(n):
-    """HBM bytes per k_colour_col<double> launch from the committed counter summary, or None."""
-    path = os.path.join(REPO, "profiles", "k_colour_col_f64_%d_hbm_traffic.txt" % n)
+def traffic_from_profile(n, kind="f64"):
+    """HBM bytes per k_colour_col launch (kind "f64": FP64 values and coefficients; "bf16": FP32 values,
+    bfloat16 coefficients) from the committed counter summary, or None."""
+    path = os.path.join(REPO, "profiles", "k_colour_col_%s_%d_hbm_traffic.txt" % (kind, n))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -96,21 +97,26 @@ def traffic_from_profile(n):
     return None
 
 
-def smoother_only(ctx, torch, defn, u, rhs0, n, sweeps):
-    """k_colour_col<double> launches (8 per sweep) at n^3, then a plain 1 GiB device-to-device copy whose
-    FETCH_SIZE / WRITE_SIZE give the counters' scale against known bytes."""
+def smoother_only(ctx, torch, defn, u, rhs0, n, sweeps, coef):
+    """k_colour_col launches (8 per sweep) at n^3, FP64 (coef "f64") or FP32 with bfloat16 coefficients
+    (coef "bf16"), then a plain 1 GiB device-to-device copy whose FETCH_SIZE / WRITE_SIZE give the
+    counters' scale against known bytes."""
     n3 = n ** 3
     G = torch.zeros(6 * n3, dtype=torch.float64, device="cuda")
     ctx.metric(G, defn, u, 1.0, 0, nu=1)
-    phi = torch.zeros(n3, dtype=torch.float64, device="cuda")
-    ms = ctx.time_sweeps(phi, rhs0, G, 1.0, sweeps)
+    if coef == "bf16":
+        phi = torch.zeros(n3, dtype=torch.float32, device="cuda")
+        ms = ctx.time_sweeps_bf16(phi, rhs0.float(), G, 1.0, sweeps)
+    else:
+        phi = torch.zeros(n3, dtype=torch.float64, device="cuda")
+        ms = ctx.time_sweeps(phi, rhs0, G, 1.0, sweeps)
     del G, phi
     x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
-    print("smoother-only: %d^3 FP64 %.4f ms/sweep (%d sweeps); copy 1 GiB x3 done" % (n, ms, sweeps), flush=True)
+    print("smoother-only: %d^3 %s %.4f ms/sweep (%d sweeps); copy 1 GiB x3 done" % (n, coef, ms, sweeps), flush=True)
 
 
 def krylov_probe(ctx, torch, defn, u, rhs0, n, iters=2):
@@ -316,6 +322,8 @@ def main():
                     help="only two PCG iterations per precision plus the 1 GiB copy (counter runs)")
     ap.add_argument("--smoother-only", action="store_true",
                     help="only the smoother timing plus a 1 GiB device copy (the counter-calibration run)")
+    ap.add_argument("--coef", choices=("f64", "bf16"), default="f64",
+                    help="--smoother-only: FP64 sweeps, or FP32 sweeps with bfloat16 coefficients")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (--gpus %d)" % args.gpus,
@@ -346,7 +354,7 @@ def main():
     torch.cuda.empty_cache()
     print("bench: setup done", file=sys.stderr, flush=True)
     if args.smoother_only:
-        smoother_only(ctx, torch, defn, u, rhs0, n, args.sweeps)
+        smoother_only(ctx, torch, defn, u, rhs0, n, args.sweeps, args.coef)
         return
     if args.krylov_probe:
         krylov_probe(ctx, torch, defn, u, rhs0, n)
@@ -416,19 +424,43 @@ def main():
     # host copies for the CPU leg (the reference's precision is FP32)
     rho_host = rho.cpu().numpy().astype("float32") if not args.no_cpu else None
     rhs_host = rhs0.cpu().numpy().astype("float32") if not args.no_cpu else None
-    # roofline of the dominant kernel (k_colour_col: one colour class per launch), FP64 and FP32
+    # roofline of the dominant kernel of the timed (mixed) solve: the finest level's smoother,
+    # k_colour_col with FP32 values and bfloat16 coefficients, one colour class per launch; the
+    # FP64 smoother of the FP64 solve beside it.  Sweeps timed with HIP events on the library's
+    # stream.  Algorithmic bytes: 5 words per point update (phi read and written, rhs, def, rho),
+    # the metric counted as recomputed from def and rho; the kernel reads a stored metric instead,
+    # which the counter traffic (committed rocprofv3 --pmc summary) shows.
     G = torch.zeros(6 * n3, **f64)
     ctx.metric(G, defn, u, 1.0, 0, nu=1)
     phi = torch.zeros(n3, **f64)
     ctx.time_sweeps(phi, rhs0, G, 1.0, 2)
     ms64 = ctx.time_sweeps(phi, rhs0, G, 1.0, args.sweeps)
-    G32, phi32, rhs32 = G.float(), phi.float(), rhs0.float()
+    phi32, rhs32 = phi.float(), rhs0.float()
+    ctx.time_sweeps_bf16(phi32, rhs32, G, 1.0, 2)
+    msb = ctx.time_sweeps_bf16(phi32, rhs32, G, 1.0, args.sweeps)
+    G32 = G.float()
     del G
     ctx.time_sweeps(phi32, rhs32, G32, 1.0, 2)
     ms32 = ctx.time_sweeps(phi32, rhs32, G32, 1.0, args.sweeps)
-    launch_ms = ms64 / 8.0
-    alg_bytes_launch = B_ALG_WORDS * 8 * n3 / 8.0
-    achieved = alg_bytes_launch / (launch_ms * 1e-3) / 1e9
+    del G32, phi32, rhs32, phi
+
+    def roof(ms_sweep, word, kind, kernel):
+        launch_ms = ms_sweep / 8.0
+        alg = B_ALG_WORDS * word * n3 / 8.0
+        ach = alg / (launch_ms * 1e-3) / 1e9
+        tr = traffic_from_profile(n, kind)
+        out = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+               "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": tr, "kernel": kernel,
+               "alg_bytes_per_launch": alg, "avg_launch_ms": round(launch_ms, 5)}
+        if tr:
+            out["traffic_over_alg"] = round(tr / alg, 2)
+            out["traffic_GBs"] = round(tr / (launch_ms * 1e-3) / 1e9, 1)
+            out["traffic_frac"] = round(out["traffic_GBs"] / PEAK_HBM_GBS, 4)
+        return out
+    roof_b = roof(msb, 4, "bf16", "k_colour_col<float,bf16> (finest-level smoother of the mixed solve: FP32 "
+                                  "values, bfloat16 metric; one of 8 colour classes, %d^3, 8 points per thread "
+                                  "along k)" % n)
+    roof_64 = roof(ms64, 8, "f64", "k_colour_col<double> (the FP64 solve's smoother, %d^3)" % n)
 
     # The reference prints per V-cycle through the Fortran runtime, which flushes at process exit:
     # from here on fd 1 goes to /dev/null and the result line is written to a saved descriptor.
@@ -467,14 +499,12 @@ def main():
                    "mixed_rel_err_vs_fp64": err, "final_relres_fp64": r64, "final_relres_mixed": rmx,
                    "reference_vcycle_ms_fp64": ref_ms, "reference_relres_after_%d_cycles" % args.ref_cycles: ref_relres,
                    "sweep_ms_fp64": round(ms64, 4), "sweep_ms_fp32": round(ms32, 4),
+                   "sweep_ms_fp32_bf16": round(msb, 4),
                    "smoother_alg_GBs_fp32": round(B_ALG_WORDS * 4 * n3 / (ms32 * 1e-3) / 1e9, 1),
+                   "roofline_fp64_smoother": roof_64,
                    "curved_mesh": curved,
                    "batched_realizations": batch},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profile(n),
-                     "kernel": "k_colour_col<double> (one of 8 colour classes of a Gauss-Seidel sweep, %d^3, "
-                               "8 points per thread along k)" % n,
-                     "alg_bytes_per_launch": alg_bytes_launch, "avg_launch_ms": round(launch_ms, 5)},
+        "roofline": roof_b,
         "cpu_baseline": cpu,
     }
     result_out.write(json.dumps(line) + "\n")

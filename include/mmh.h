@@ -347,6 +347,9 @@ int mmh_metric_d(mmh_ctx*, double* G, const double* def, const double* u, int nu
 int mmh_metric_f(mmh_ctx*, float* G, const float* def, const float* u, int nu, int n, float dx, int lsquare);
 int mmh_time_sweeps_d(mmh_ctx*, double* phi, const double* rhs, const double* G, int n, double dx, int nsweeps, float* ms);
 int mmh_time_sweeps_f(mmh_ctx*, float* phi, const float* rhs, const float* G, int n, float dx, int nsweeps, float* ms);
+/* The same for FP32 phi with the coefficients rounded to bfloat16 (the finest-level smoother of
+ * the mixed Krylov solve, option "precond_bf16"); G is the FP64 metric (6 N^3). */
+int mmh_time_sweeps_fb(mmh_ctx*, float* phi, const float* rhs, const double* G, int n, float dx, int nsweeps, float* ms);
 
 /* ---- host-pointer entry points (used by the Fortran drop-ins) ------------------------------- */
 int mmh_relax_host_f(mmh_ctx*, float* arr, float* resid, const float* rhs, const float* deformpot, const float* u,

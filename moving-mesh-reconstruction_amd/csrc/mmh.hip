@@ -1862,6 +1862,37 @@ template <class T> void time_sweeps(mmh_ctx* c, T* phi, const T* rhs, const T* G
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
+// FP32 sweeps reading 16-bit (bfloat16) coefficients, the finest-level smoother of the mixed
+// Krylov solve: G (6 n^3 FP64, column-major) is rounded to bfloat16 as that solve rounds it.
+void time_sweeps_fb(mmh_ctx* c, float* phi, const float* rhs, const double* G, int n, float dx, int ns, float* ms) {
+  const i64 n3 = cube(n);
+  float* ps = wsT<float>(c, "tsb_p", n3);
+  float* rs = wsT<float>(c, "tsb_r", n3);
+  bf16* gb = wsT<bf16>(c, "tsb_gb", 6 * n3);
+  {
+    double* gd = wsT<double>(c, "tsb_gd", 6 * n3);
+    relayout(c, gd, G, n, 6, 1);
+    convert_bf16(c, gb, gd, 6 * n3);
+    ws_free(c, "tsb_gd:8");
+  }
+  relayout(c, ps, phi, n, 1, 1);
+  relayout(c, rs, rhs, n, 1, 1);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  sweeps<float, bf16>(c, ps, rs, gb, n, dx, ns, 1);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / std::max(1, ns);
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  relayout(c, phi, ps, n, 1, 0);
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 // Host-pointer wrapper: copies inputs to context buffers, runs f, copies outputs back.
 struct HostBuf {
   mmh_ctx* c;
@@ -2118,6 +2149,13 @@ mmh_ctx* mmh_default_ctx(void) {
   }
 
 DEF_DEV(double, d)
+
+int mmh_time_sweeps_fb(mmh_ctx* c, float* phi, const float* rhs, const double* G, int n, float dx, int ns, float* ms) {
+  return guarded(c, [&] {
+    require(is_pow2(n) && n >= 4 && ns >= 1 && phi && rhs && G && ms, "time_sweeps_fb: bad arguments");
+    time_sweeps_fb(c, phi, rhs, G, n, dx, ns, ms);
+  });
+}
 
 /* multigrid without allocation: the V-cycle of mmh_multigrid_d with every workspace carved from
  * the device's static scratch array instead of the context's cached buffers. */

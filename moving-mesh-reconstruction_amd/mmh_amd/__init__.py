@@ -334,6 +334,13 @@ class Context:
         self.call("mmh_time_sweeps_" + s, _ptr(phi), _ptr(rhs), _ptr(G), _cube_side(phi), T(dx), nsweeps, C.byref(ms))
         return ms.value
 
+    def time_sweeps_bf16(self, phi, rhs, G, dx=1.0, nsweeps=10):
+        """FP32 sweeps (phi, rhs float32) with G (FP64, 6 n^3) rounded to bfloat16; ms per sweep."""
+        ms = C.c_float(0)
+        self.call("mmh_time_sweeps_fb", _ptr(phi), _ptr(rhs), _ptr(G), _cube_side(phi), C.c_float(dx), nsweeps,
+                  C.byref(ms))
+        return ms.value
+
     def rlft3(self, data, speq, isign):
         self.call("mmh_rlft3_d", _ptr(data), _ptr(speq), _cube_side(data), isign)
 
