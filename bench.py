@@ -322,6 +322,8 @@ def main():
                     help="only two PCG iterations per precision plus the 1 GiB copy (counter runs)")
     ap.add_argument("--smoother-only", action="store_true",
                     help="only the smoother timing plus a 1 GiB device copy (the counter-calibration run)")
+    ap.add_argument("--solves-only", action="store_true",
+                    help="only the warm-up and timed FP64 and mixed solves (kernel-trace runs)")
     ap.add_argument("--coef", choices=("f64", "bf16"), default="f64",
                     help="--smoother-only: FP64 sweeps, or FP32 sweeps with bfloat16 coefficients")
     args = ap.parse_args()
@@ -392,6 +394,10 @@ def main():
     tmx, cmx, rmx, phimx = results[True]
     err = float((phimx - phi64).abs().max() / phi64.abs().max())
     del phi64, phimx
+    if args.solves_only:
+        print("solves-only: %d^3 fp64 %.4f s (%d iterations), mixed %.4f s (%d iterations), error %.2e" % (
+            n, t64, c64, tmx, cmx, err), flush=True)
+        return
 
     # the plain reference algorithm: FP64 V-cycles with the reference sweep schedule
     ref_ms, ref_relres = None, None
