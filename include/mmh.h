@@ -254,7 +254,7 @@ int mmh_particle_density_f(mmh_ctx*, const float* xp, long long np, int ng, floa
  * divides by an FP32 SUM).  Device arrays, Fortran order. */
 int mmh_coarsen_f(mmh_ctx*, const float* den, int n, int levels, float* out);
 
-/* Column density through the deformed mesh (column_density.f90:14-66): den is scaled to mean 1
+/* Column density through the deformed mesh (column_density.f90:14-61): den is scaled to mean 1
  * (FP64 total), and proj (ng^2) is den integrated along z over the deformed column of the cells
  * ng/2 .. ng/2 + nplot - 1, the column's ends moved by the mesh's z displacement from def.
  * Device arrays, Fortran order. */

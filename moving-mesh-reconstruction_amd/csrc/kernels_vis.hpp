@@ -26,7 +26,7 @@ __global__ void k_scale_mean1(float* __restrict__ x, const double* __restrict__ 
   x[p] = (float)((double)x[p] / sum[0] * (double)n3);
 }
 
-// Column density through the deformed mesh (column_density.f90:35-66): the z displacement of the
+// Column density through the deformed mesh (column_density.f90:26-57): the z displacement of the
 // cell faces, dspz = (4-point mean of def on plane k+1) - (same on plane k), sets the deformed
 // column [kk1, kk2] of the cells k1..k1+nplot-1 at (i, j); proj is the density integrated over it,
 // partial end cells weighted by their covered fraction, divided by nplot.  Fortran indices

@@ -1213,7 +1213,7 @@ static void particle_density(const float* xp_in, long np, int ng, float nf, floa
 
 // ------------------------------------------------------------------------------------------------
 // Downsampling (from1024to512.f90:17-23, 54-66) and the column-density projection
-// (column_density.f90:35-66); mean-1 scaling with a sequential FP64 total.
+// (column_density.f90:26-57); mean-1 scaling with a sequential FP64 total.
 static void scale_mean1(float* x, long n3) {
   double s = 0.0;
   for (long p = 0; p < n3; ++p) s += (double)x[p];
