@@ -601,3 +601,26 @@ def test_mixed_bf16_coefficients(ctx, n):
     G = R.or_metric(d, u, 5, n, 1.0)
     _, rinf = R.or_residual(x16, b16, G, n, 1.0)
     assert rinf <= 1.05e-9 * np.abs(b16).max()
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_bf16_coefficient_sweeps_equal_fp32_sweeps(ctx, n):
+    """The FP32 Gauss-Seidel sweeps that read 16-bit coefficients (mmh_time_sweeps_fb, the finest-level
+    smoother of the mixed solve; the column kernel at 256^3) give bit for bit the FP32 sweeps
+    (mmh_time_sweeps_f) run on the same metric rounded to bfloat16 by torch and widened back: the
+    16-bit path changes where the coefficients are read from, not the arithmetic."""
+    print("case bf16 sweeps n=%d" % n, flush=True)
+    rho = field(n, 2.0, 3, np.float64)
+    u = aos(rho)
+    d = smooth_def(n, 2.0, 4, np.float64)
+    Gd = dev(np.zeros(6 * n ** 3))
+    ctx.metric(Gd, dev(d), dev(u), 1.0, 0, nu=5)
+    rhs = dev((1 - rho).astype(np.float32))
+    pa = dev(np.zeros(n ** 3, np.float32))
+    ctx.time_sweeps_bf16(pa, rhs, Gd, 1.0, 3)
+    Gr = Gd.float().to(torch.bfloat16).float().contiguous()
+    pb = dev(np.zeros(n ** 3, np.float32))
+    ctx.time_sweeps(pb, rhs, Gr, 1.0, 3)
+    a, b = host(pa), host(pb)
+    assert np.abs(a).max() > 0
+    assert np.array_equal(a, b)
