@@ -75,6 +75,9 @@ typedef struct {
    * fields and so is the V-cycle preconditioner, so both must stay negative; a count > 0 means the
    * discrete operator (or the preconditioner) lost definiteness, where CG cannot converge. */
   int nonneg_pLp, nonneg_zr;
+  /* Krylov solve: 1 if its preconditioner used the Galerkin coarse operators (option "galerkin"),
+   * 0 if it fell back to the rebuilt ones because they did not fit in device memory. */
+  int galerkin;
 } mmh_solve_stats;
 
 /* Per outer iteration: the diagnostics the reference prints in its iteration loop

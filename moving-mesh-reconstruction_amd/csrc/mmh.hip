@@ -361,7 +361,8 @@ void build_hier(mmh_ctx* c, Hier<T>& H, const std::string& tag, T* u, T* rhs, co
 // that again in FP32 for the mixed preconditioner).  They are built only where they fit, with the
 // rest of the Krylov solve's workspace (its vectors, the FP32 hierarchy) and 4 GiB to spare, beside
 // the memory already held; otherwise the rebuilt coarse operators are used.
-bool galerkin_fits(mmh_ctx* c, const Hier<double>& H, const std::string& tag, bool mixed) {
+// The FP32 copies counted are those build_fp32 makes (bf: 16-bit coefficients on levels >= 32^3).
+bool galerkin_fits(mmh_ctx* c, const Hier<double>& H, const std::string& tag, bool mixed, bool bf) {
   size_t need = 0;
   const size_t n3 = (size_t)cube(H.n[0]);
   for (const char* k : {"cg_r:8", "cg_ro:8", "cg_p:8", "cg_q:8", "det:8"}) need += ws_need(c, k, n3 * sizeof(double));
@@ -371,9 +372,12 @@ bool galerkin_fits(mmh_ctx* c, const Hier<double>& H, const std::string& tag, bo
     const std::string t = "sf" + std::to_string(l);
     if (l > 0) need += ws_need(c, tag + std::to_string(l) + "A:8", 27 * m3 * sizeof(double));
     if (!mixed) continue;
-    if (l > 0) need += ws_need(c, t + "A:4", 27 * m3 * sizeof(float));
+    const bool b16 = bf && H.n[l] >= 32;
+    if (l > 0) need += b16 ? ws_need(c, t + "Ab", 27 * m3 * sizeof(bf16)) : ws_need(c, t + "A:4", 27 * m3 * sizeof(float));
+    if (b16 && l == 0) need += ws_need(c, t + "Gb", 6 * m3 * sizeof(bf16));
+    else if (!b16) need += ws_need(c, t + "G:4", 6 * m3 * sizeof(float));
     need += ws_need(c, t + "u:4", m3 * sizeof(float)) + ws_need(c, t + "rhs:4", m3 * sizeof(float)) +
-            ws_need(c, t + "r:4", m3 * sizeof(float)) + ws_need(c, t + "G:4", 6 * m3 * sizeof(float));
+            ws_need(c, t + "r:4", m3 * sizeof(float));
   }
   size_t freeb = 0, total = 0;
   HIPCHK(hipMemGetInfo(&freeb, &total));
@@ -1534,7 +1538,12 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
   double* rs = wsT<double>(c, "sd_rs", n3);
   Hier<double> H;
   build_hier(c, H, "sd", us, rs, defpot, ubig, nu, 1.0, n, 0);
-  if (o->krylov && c->galerkin && !S.delta && galerkin_fits(c, H, "sd", o->mixed != 0)) {
+  // The mixed Krylov solve keeps no FP64 preconditioned residual: an FP64 solve's buffer for it
+  // (N^3 words, 8 GiB at 1024^3) goes before the Galerkin operators are sized.
+  if (o->krylov && o->mixed) ws_free(c, "cg_z:8");
+  const bool gal = o->krylov && c->galerkin && !S.delta &&
+                   galerkin_fits(c, H, "sd", o->mixed != 0, o->mixed == 1 && c->precond_bf16);
+  if (gal) {
     try {
       build_galerkin(c, H, "sd");
     } catch (const MmhError& e) {
@@ -1558,6 +1567,7 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
     relayout(c, u, us, n, 1, 0);
     HIPCHK(hipStreamSynchronize(c->stream));
   };
+  if (st) st->galerkin = H.A[1] ? 1 : 0;
   if (rn == 0.0) {
     // L u = 0 has the mean-free solution 0
     fill<double>(c, u, 0.0, n3);
@@ -1569,13 +1579,15 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
     if (o->mixed) {
       build_fp32(c, F, H, o->mixed == 1 && c->precond_bf16);
       // The FP32 cycle reads its own copies; only the FP64 4^3 level's operator is read in FP64.
-      // The FP64 operators of the other levels stay cached for the next solve unless the Krylov
-      // vectors would not fit beside them (1024^3).
+      // The FP64 operators of the other levels stay cached for the next solve, which rounds its
+      // copies from them again: freeing them would leave it to rebuild tens of GiB at 1024^3, where
+      // the memory may then be short and the solve fall back to the rebuilt operators.  They go
+      // only if the Krylov vectors, still to be allocated, would not fit beside them.
       size_t freeb = 0, total = 0;
       HIPCHK(hipMemGetInfo(&freeb, &total));
       size_t vec = 0;
       for (const char* k : {"cg_r:8", "cg_ro:8", "cg_p:8", "cg_q:8"}) vec += ws_need(c, k, (size_t)n3 * sizeof(double));
-      if (freeb < vec + ((size_t)8 << 30))
+      if (vec > 0 && freeb < vec + ((size_t)1 << 30))
         for (int l = 1; l + 1 < H.nlev; ++l)
           if (H.A[l]) {
             ws_free(c, "sd" + std::to_string(l) + "A:8");

@@ -66,7 +66,7 @@ class SolveOpts(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("cycles", C.c_int), ("relres", C.c_double * 256), ("rhs_norm", C.c_double),
-                ("nonneg_pLp", C.c_int), ("nonneg_zr", C.c_int)]
+                ("nonneg_pLp", C.c_int), ("nonneg_zr", C.c_int), ("galerkin", C.c_int)]
 
 
 _lib = None

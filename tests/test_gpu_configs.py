@@ -103,6 +103,33 @@ def test_config3_mesh_solve_to_tolerance(ctx, n):
         assert rel <= 1e-8
 
 
+def test_repeated_solves_keep_galerkin_levels_1024(ctx):
+    """At 1024^3 device memory is tight: two FP64 solves and then two mixed solves in a row, with the
+    workspace and the results kept between them as the bench keeps them, must all use the Galerkin coarse operators
+    (stats.galerkin) and the two mixed solves must take the same iterations.  (The second mixed
+    solve used to find the FP64 operators freed and too little memory to rebuild them beside an
+    FP64 solve's leftover buffer, and fell back to the rebuilt operators: 71 iterations, not 27.)"""
+    n = 1024
+    print("case repeated solves n=%d" % n, flush=True)
+    ctx.release_workspace()
+    torch.cuda.empty_cache()
+    rho, d, rhs0 = workload(ctx, n, 2.0, 42)
+    runs, keep = [], []
+    for mixed in (False, False, True, True):
+        phi, rhs = torch.zeros(n ** 3, **F64), rhs0.clone()
+        st, ok = bench_solve(ctx, phi, rhs, d, rho, mixed)
+        print("  %s: %d iterations, galerkin %d, relres %.3e" % ("mixed" if mixed else "fp64", st.cycles, st.galerkin,
+                                                                st.relres[st.cycles - 1]), flush=True)
+        assert ok and st.galerkin == 1
+        runs.append(st.cycles)
+        keep.append(phi)  # results held as a caller holds them (the bench keeps both precisions')
+        del rhs
+    del keep
+    ctx.release_workspace()
+    torch.cuda.empty_cache()
+    assert runs[2] == runs[3] and runs[2] <= 30
+
+
 def test_config4_batched_set_against_oracle_operator(ctx):
     print("case config-4 batched set", flush=True)
     sizes = [32] * 40 + [64] * 40 + [128] * 20
