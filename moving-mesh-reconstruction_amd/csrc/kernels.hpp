@@ -738,6 +738,9 @@ __global__ void k_zero_inactive(double* __restrict__ x, const int* __restrict__ 
 // Per-grid CG scalars, one thread per grid.  step 0: alpha = rz / pq for active grids, else 0.
 // step 1: relres = rmax / rden and the iteration count advance for active grids; a grid whose
 // relres <= tol becomes inactive.  step 2: beta = (rz_new - zro) / rz, rz = rz_new (0 for inactive grids).
+// A breakdown (alpha not finite: p.Lp = 0, or a preconditioner that produced no finite z) freezes
+// the grid where it is: it stays active, so it is reported as not converged, but every kernel of
+// the iteration skips it (act0 = 0) and its x and r keep their last finite values.
 __global__ void k_cg_scalars(int step, int B, int it, double tol, double* __restrict__ alpha, double* __restrict__ beta,
                              double* __restrict__ rz, const double* __restrict__ pq, const double* __restrict__ rmax,
                              const double* __restrict__ rden, const double* __restrict__ rznew,
@@ -746,10 +749,12 @@ __global__ void k_cg_scalars(int step, int B, int it, double tol, double* __rest
   const int y = blockIdx.x * blockDim.x + threadIdx.x;
   if (y >= B) return;
   if (step == 0) {
-    alpha[y] = active[y] ? rz[y] / pq[y] : 0.0;
-    act0[y] = active[y];  // the grids this iteration works on (a grid converging in it still updates x)
+    const double a = active[y] ? rz[y] / pq[y] : 0.0;
+    const bool ok = isfinite(a);
+    alpha[y] = ok ? a : 0.0;
+    act0[y] = active[y] && ok;  // the grids this iteration works on (a grid converging in it still updates x)
   } else if (step == 1) {
-    if (active[y]) {
+    if (act0[y]) {
       const double rr = rmax[y] / rden[y];
       relres[y] = rr;
       iters[y] = iters[y] + 1;

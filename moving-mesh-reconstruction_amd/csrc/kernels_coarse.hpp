@@ -391,6 +391,9 @@ __global__ __launch_bounds__(CW) void k_pcg_stack(PcgStack P) {
     __syncthreads();
     const double pq = cw_dot(p, q, n3, s);
     const double alpha = rz / pq;
+    // breakdown (p.Lp = 0, or no finite z): x and r keep their last finite values and the grid
+    // stays active, reported as not converged
+    if (!isfinite(alpha)) break;
     double rmax = 0.0;
     for (i64 t = threadIdx.x; t < n3; t += CW) {
       x[t] = x[t] + alpha * p[t];
@@ -409,7 +412,6 @@ __global__ __launch_bounds__(CW) void k_pcg_stack(PcgStack P) {
       if (threadIdx.x == 0) P.active[g] = 0;
       break;
     }
-    if (!(rr == rr)) break;  // breakdown (p.Lp = 0): the grid stays active, reported as not converged
     precond();
     const double rzn = cw_dot(z, r, n3, s);
     const double zro = cw_dot(z, ro, n3, s);

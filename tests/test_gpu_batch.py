@@ -301,3 +301,28 @@ def test_batch_coarse_levels_one_workgroup_per_grid(ctx, probs, fused, galerkin)
     for a, b, t0, t1, case in zip(x0, x1, s0, s1, CASES):
         assert t0.cycles == t1.cycles and t0.relres == t1.relres, case
         assert np.array_equal(a, b), case
+
+
+@pytest.mark.parametrize("pers", [0, 64])
+def test_batch_breakdown_item_reported_not_converged(ctx, probs, pers):
+    """A realisation whose operator vanishes (zero density: G = 0, so L p = 0 and the V-cycle divides
+    by a zero diagonal) breaks the CG recurrence at its first step.  The batch must report it as not
+    converged (MMH_ENOCONV for the call) with its iterate left finite, and still solve the realisation
+    beside it to the tolerance, with the batched launches and with one workgroup per grid."""
+    print("case batch breakdown item, batch_persistent %d" % pers, flush=True)
+    u, r, d = probs[0]  # 32^3
+    n3 = r.numel()
+    u0 = torch.zeros_like(u)
+    b0 = torch.from_numpy(np.random.default_rng(5).standard_normal(n3)).cuda()
+    items = [(torch.zeros_like(r), r.clone(), d, u), (torch.zeros_like(r), b0, torch.zeros_like(d), u0)]
+    ctx.set_option("batch_persistent", pers)
+    try:
+        st, ok = ctx.solve_batch(items, **SOLVE)
+    finally:
+        ctx.set_option("batch_persistent", 0)
+    torch.cuda.synchronize()
+    assert not ok
+    assert st[0].converged and st[0].relres <= SOLVE["tol"]
+    assert not st[1].converged
+    assert bool(torch.isfinite(items[1][0]).all())
+    assert np.isfinite(st[1].relres) and st[1].relres > SOLVE["tol"]
