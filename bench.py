@@ -400,7 +400,7 @@ def main():
         return
 
     # the plain reference algorithm: FP64 V-cycles with the reference sweep schedule
-    ref_ms, ref_relres = None, None
+    ref_ms, ref_relres, ref32_ms = None, None, None
     if args.ref_cycles > 0:
         # one untimed cycle first (first launches of the schedule's kernels, its workspaces)
         phi = torch.zeros(n3, **f64)
@@ -416,6 +416,26 @@ def main():
         ref_relres = st.relres[st.cycles - 1]
         del phi, rhs
         print("bench: reference V-cycle %.1f ms" % ref_ms, file=sys.stderr, flush=True)
+        # the same cycle as the Fortran drop-in runs it, beside the CPU leg: FP32, through multigrid's
+        # own entry (rgzerosum of a fresh rhs, one V-cycle continuing from phi), 5-component u
+        u32 = torch.zeros(n3, 5, dtype=torch.float32, device="cuda")
+        u32[:, 0] = rho.float()
+        u32 = u32.view(-1)
+        d32 = defn.float()
+        a32 = torch.zeros(n3, dtype=torch.float32, device="cuda")
+        ctx.multigrid(a32, rhs0.float(), d32, u32, dx=1.0, iopt=2, nu=5)  # untimed first call
+        a32.zero_()
+        ts = []
+        for _ in range(2):
+            r32 = rhs0.float()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.multigrid(a32, r32, d32, u32, dx=1.0, iopt=2, nu=5)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ref32_ms = sum(ts) / len(ts) * 1e3
+        del u32, d32, a32, r32
+        print("bench: reference V-cycle, FP32 drop-in entry %.1f ms" % ref32_ms, file=sys.stderr, flush=True)
 
     curved = None
     if not args.no_curved:
@@ -479,6 +499,9 @@ def main():
         if cpu and ref_ms:
             cpu["gpu_reference_vcycle_s"] = round(ref_ms / 1e3, 4)
             cpu["gpu_vs_cpu_reference_vcycle"] = round(cpu["value"] / (ref_ms / 1e3), 1)
+        if cpu and ref32_ms:
+            cpu["gpu_reference_vcycle_fp32_dropin_s"] = round(ref32_ms / 1e3, 4)
+            cpu["gpu_fp32_dropin_vs_cpu_reference_vcycle"] = round(cpu["value"] / (ref32_ms / 1e3), 1)
 
     line = {
         "metric": "s to 1e-8 rel residual at 512^3 (mixed vs FP64 GPU); smoother HBM GB/s",
@@ -504,6 +527,7 @@ def main():
                    "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
                    "mixed_rel_err_vs_fp64": err, "final_relres_fp64": r64, "final_relres_mixed": rmx,
                    "reference_vcycle_ms_fp64": ref_ms, "reference_relres_after_%d_cycles" % args.ref_cycles: ref_relres,
+                   "reference_vcycle_ms_fp32_dropin_entry": ref32_ms,
                    "sweep_ms_fp64": round(ms64, 4), "sweep_ms_fp32": round(ms32, 4),
                    "sweep_ms_fp32_bf16": round(msb, 4),
                    "smoother_alg_GBs_fp32": round(B_ALG_WORDS * 4 * n3 / (ms32 * 1e-3) / 1e9, 1),
