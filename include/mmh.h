@@ -127,6 +127,8 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 one workgroup (bit-identical)
  *   "coarse_fused_ref" 16 (default), 4, 8, 32 or 0: the same for the reference V-cycle (multigrid,
  *                 schedule 0), its residual norms and divergence guard included (bit-identical)
+ *   "coarse_lds"  1 (default) / 0: the one-workgroup cycle holds its smallest levels (u, rhs, G)
+ *                 in LDS while it runs (bit-identical)
  *   "colour_col"  8 (default), 1, 2, 4 or 16: Gauss-Seidel colour launches on levels
  *                 n >= "colour_col_min_n" (256 by default) update this many points per thread along
  *                 k, carrying the k+1 plane neighbours of one point to the next (1 = one point per

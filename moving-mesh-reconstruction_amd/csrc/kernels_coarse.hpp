@@ -38,6 +38,7 @@ template <class T> struct CoarseCycle {
   T dx[CLEV_MAX];
   int n[CLEV_MAX];
   int sw[CLEV_MAX];       // sweeps each way on the level (pre == post)
+  int stage[CLEV_MAX];    // k_coarse_cycle_lds: the level's u, rhs and G are held in LDS for the cycle
   int nl;
   int coarse;             // sweeps each way on 4^3
   int ref;                // reference schedule: residual norms and the divergence guard
@@ -258,6 +259,66 @@ template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCy
   __shared__ double blk[64], blk2[64];  // block values of a level's sums (32^3: 16 blocks)
   __shared__ double wm[CW / 64];
   cw_cycle<T>(C, s, blk, blk2, wm);
+}
+
+// The same cycle with its smallest levels held in LDS.  The reference schedule sweeps the coarse
+// levels hundreds of times (nu = 4 NG/N: 128, 256 and 32 + 32 sweeps on 16^3, 8^3 and 4^3 at
+// NG = 512), some 3600 barrier-separated colour phases per cycle, each of which waits on the
+// latency of its loads; from LDS that wait is a fraction of an L2 round trip.  The levels marked
+// in stage[] (chosen on the host by stage_levels: 19-point levels, smallest first, while their
+// u, rhs and G fit in CLDS_BYTES) are copied in, the cycle runs on the copies with the same
+// per-point functions (bit-identical results), and their u is copied back.  Residual arrays,
+// per-level scalars and guard flags stay in global memory.
+constexpr int CLDS_BYTES = 148 * 1024;
+
+template <class T> void stage_levels(CoarseCycle<T>& C) {
+  size_t off = 0;
+  for (int l = C.nl - 1; l >= 0; --l) {
+    const size_t b = 8 * (size_t)C.n[l] * C.n[l] * C.n[l] * sizeof(T);
+    C.stage[l] = (!C.A[l] && C.G[l] && off + b <= (size_t)CLDS_BYTES) ? 1 : 0;
+    if (C.stage[l]) off += b;
+  }
+}
+
+template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle_lds(CoarseCycle<T> Cg) {
+  __shared__ double s[CW / RB][RB];
+  __shared__ double blk[64], blk2[64];
+  __shared__ double wm[CW / 64];
+  __shared__ __attribute__((aligned(16))) unsigned char pool[CLDS_BYTES];
+  __shared__ CoarseCycle<T> C;  // the level table, with the staged levels pointing into pool
+  if (threadIdx.x == 0) {
+    C = Cg;
+    size_t off = 0;
+    for (int l = Cg.nl - 1; l >= 0; --l) {
+      if (!Cg.stage[l]) continue;
+      const i64 m3 = cube_d(Cg.n[l]);
+      T* b = reinterpret_cast<T*>(pool + off);
+      C.u[l] = b;
+      C.rhs[l] = b + m3;
+      C.G[l] = b + 2 * m3;
+      off += 8 * (size_t)m3 * sizeof(T);
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < Cg.nl; ++l) {
+    if (!Cg.stage[l]) continue;
+    const i64 m3 = cube_d(Cg.n[l]);
+    T* u = C.u[l];
+    T* rh = C.rhs[l];
+    T* g = const_cast<T*>(C.G[l]);
+    for (i64 t = threadIdx.x; t < m3; t += CW) {
+      u[t] = Cg.u[l][t];
+      rh[t] = Cg.rhs[l][t];
+    }
+    for (i64 t = threadIdx.x; t < 6 * m3; t += CW) g[t] = Cg.G[l][t];
+  }
+  __syncthreads();
+  cw_cycle<T>(C, s, blk, blk2, wm);
+  for (int l = 0; l < Cg.nl; ++l) {
+    if (!Cg.stage[l]) continue;
+    const i64 m3 = cube_d(Cg.n[l]);
+    for (i64 t = threadIdx.x; t < m3; t += CW) Cg.u[l][t] = C.u[l][t];
+  }
 }
 
 // ---------------------------------------------------------------------------------------------

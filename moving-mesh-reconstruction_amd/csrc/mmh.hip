@@ -51,6 +51,7 @@ struct mmh_ctx {
   int galerkin = 1;        // Krylov preconditioner: Galerkin coarse operators (option "galerkin")
   int nu_top = 0;          // Krylov preconditioner: sweeps on the finest level, 0 = nu_pre (option "nu_top")
   int coarse_fused = 4;    // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
+  int coarse_lds = 1;      // one-workgroup coarse cycle: its smallest levels held in LDS (option "coarse_lds")
   int coarse_fused_ref = 16;  // the reference V-cycle's levels <= this size in one workgroup (option "coarse_fused_ref")
   int zero_start = 1;      // first sweep from u = 0 reads only the colour classes already updated (option "zero_start")
   int colour_col = 8;      // colour kernels of levels n >= colour_col_min_n: points per thread along k (option "colour_col": 1 .. 16)
@@ -663,7 +664,12 @@ template <class T> void vcycle(mmh_ctx* c, Hier<T>& H, int l, int ng_top, const 
   double* sc = H.scal + 3 * l;
   if (coarse_cycle_applies(c, H, l, S)) {
     CoarseCycle<T> C = coarse_cycle_of(H, l, S, ng_top);
-    k_coarse_cycle<T><<<1, CW, 0, c->stream>>>(C);
+    if (c->coarse_lds) {
+      stage_levels(C);
+      k_coarse_cycle_lds<T><<<1, CW, 0, c->stream>>>(C);
+    } else {
+      k_coarse_cycle<T><<<1, CW, 0, c->stream>>>(C);
+    }
     LAUNCHCHK();
     return;
   }
@@ -2029,6 +2035,8 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->wave_min_n = value;
     } else if (k == "batch_graph") {
       c->batch_graph = value ? 1 : 0;
+    } else if (k == "coarse_lds") {
+      c->coarse_lds = value ? 1 : 0;
     } else if (k == "batch_coarse_fused") {
       require(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
               "set_option: batch_coarse_fused is 0 (off) or the largest level size run one workgroup per grid, 4 to 32");
@@ -2066,7 +2074,8 @@ int mmh_get_option(mmh_ctx* c, const char* key, int* value) {
         {"mixed_delta", c->mixed_delta}, {"wave", c->wave}, {"wave_p", c->wave_p}, {"wave_min_n", c->wave_min_n},
         {"batch_graph", c->batch_graph}, {"batch_autotune", c->batch_autotune}, {"mesh_solve", c->mesh_solve},
         {"mesh_tol", c->mesh_tol}, {"mesh_cycles", c->mesh_cycles}, {"precond_bf16", c->precond_bf16},
-        {"batch_persistent", c->batch_persistent}, {"batch_coarse_fused", c->batch_coarse_fused}};
+        {"batch_persistent", c->batch_persistent}, {"batch_coarse_fused", c->batch_coarse_fused},
+        {"coarse_lds", c->coarse_lds}};
     auto it = v.find(k);
     if (it == v.end()) throw MmhError(MMH_EINVAL, std::string("get_option: unknown key ") + key);
     *value = it->second;

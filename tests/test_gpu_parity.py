@@ -493,8 +493,9 @@ def test_krylov_solution_solves_the_reference_equation(ctx, n, mixed, galerkin):
 @pytest.mark.parametrize("n,ng", [(32, 32), (32, 64), (64, 64)])
 def test_reference_vcycle_fused_coarse_levels_equal_launches(ctx, dt, n, ng):
     """The reference V-cycle's coarse levels in one workgroup (option coarse_fused_ref: levels up
-    to 32, 16, 8 or 4) against the launch-by-launch cycle: the same iterate, per-level residual norms
-    r1 and relaxerr, and divergence-guard decisions (the finest level is guarded too when NG > N)."""
+    to 32, 16, 8 or 4; with the smallest of them held in LDS, option coarse_lds, and without)
+    against the launch-by-launch cycle: the same iterate, per-level residual norms r1 and relaxerr,
+    and divergence-guard decisions (the finest level is guarded too when NG > N)."""
     print("case fused reference V-cycle n=%d ng=%d %s" % (n, ng, dt.__name__), flush=True)
     rho = field(n, 2.0, 7, dt)
     u = aos(rho)
@@ -502,13 +503,15 @@ def test_reference_vcycle_fused_coarse_levels_equal_launches(ctx, dt, n, ng):
     rhs = (1 - rho).astype(dt)
     R.or_nsmooth(rhs, n)
     out = []
-    for cf in (32, 16, 8, 4, 0):
+    for cf, lds in ((32, 1), (32, 0), (16, 1), (16, 0), (8, 1), (4, 1), (0, 1)):
         ctx.set_option("coarse_fused_ref", cf)
+        ctx.set_option("coarse_lds", lds)
         try:
             ad, bd = dev(np.zeros(n ** 3, dt)), dev(rhs)
             st = ctx.multigrid(ad, bd, dev(d), dev(u), 1.0, 2, ng, nu=5)
         finally:
             ctx.set_option("coarse_fused_ref", 16)
+            ctx.set_option("coarse_lds", 1)
         out.append((host(ad), [(st.r1[l], st.relaxerr[l], st.guard[l]) for l in range(st.nlev)]))
     for a, s in out[:-1]:
         assert s == out[-1][1]
