@@ -3,7 +3,7 @@
 // Below 64^3 a V-cycle level is a few thousand points or fewer, and each of its ~40 launches
 // (8 colours per sweep, residual, restriction, fill, prolongation, sums, mean removal) takes a few
 // microseconds plus the dispatch gap: the 4^3..32^3 levels cost about as much as the 256^3 level.
-// Here one workgroup of 1024 threads runs the whole cycle from the top coarse level down to 4^3
+// Here one workgroup of CW threads runs the whole cycle from the top coarse level down to 4^3
 // and back, with a barrier between dependent phases.  The level arrays stay in global memory
 // (they fit in the L2).  Every phase applies the per-point functions of the multi-launch kernels
 // (colour_point, resid_point, restrict_point, prolong_cell) and the sums follow reduce()'s
@@ -18,7 +18,7 @@
 
 namespace mmh {
 
-constexpr int CW = 1024;       // threads of the coarse-cycle workgroup
+constexpr int CW = 512;        // threads of the coarse-cycle workgroup (1024 and 256 measured slower)
 constexpr int CLEV_MAX = 8;    // levels one coarse cycle may hold
 constexpr int CTOP_MAX = 32;   // largest level run this way (its sums have <= 16 blocks)
 
