@@ -215,8 +215,8 @@ typedef struct {
  * Krylov form (opts: krylov = 1, schedule = 1, nu_pre = nu_post, mixed = 0); every item's result
  * is bit-identical to its mmh_solve_d.  The groups iterate side by side, each on a stream of its
  * own.  Options "batch_graph", "batch_autotune" and "batch_streams" (mmh_set_option) switch graph
- * replay, tuning and the group streams off (results unchanged).  "batch_coarse_fused" (4 by
- * default, 8, 16, 32 or 0) runs the preconditioner's levels up to that size with one workgroup per
+ * replay, tuning and the group streams off (results unchanged).  "batch_coarse_fused" (16 by
+ * default, 4, 8, 32 or 0) runs the preconditioner's levels up to that size with one workgroup per
  * grid, one launch for all of them (results unchanged).  "batch_persistent" (0 by default, 8 to 64)
  * solves groups of n up to that size whole in one workgroup per grid, one launch for the entire
  * PCG; its sums run in another order, so results agree with the default to rounding, not bit for

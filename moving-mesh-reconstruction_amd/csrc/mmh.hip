@@ -61,7 +61,7 @@ struct mmh_ctx {
   int graphs_live = 0;     // captured graphs holding workspace pointers (workspace must not move)
   int batch_autotune = 1;  // batched solve: pick the colour kernel's launch shape by timing
   int batch_streams = 1;   // batched solve: size groups iterate side by side on streams of their own
-  int batch_coarse_fused = 4;  // batched solve: levels <= this size one workgroup per grid (option "batch_coarse_fused", 0 off)
+  int batch_coarse_fused = 16;  // batched solve: levels <= this size one workgroup per grid (option "batch_coarse_fused", 0 off)
   int batch_persistent = 0;  // batched solve: groups of n <= this solve in one workgroup per grid (option "batch_persistent")
   int precond_bf16 = 1;    // mixed Krylov solve: 16-bit coefficients in the FP32 preconditioner (option "precond_bf16")
   int mesh_solve = 0;      // calcdefp's mesh solve: 0 one reference V-cycle, 1 V(2,2) cycles to 10^-mesh_tol,

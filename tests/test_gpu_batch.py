@@ -295,7 +295,7 @@ def test_batch_coarse_levels_one_workgroup_per_grid(ctx, probs, fused, galerkin)
         ctx.set_option("batch_coarse_fused", fused)
         x1, s1, ok1 = run_batch(ctx, probs)
     finally:
-        ctx.set_option("batch_coarse_fused", 4)
+        ctx.set_option("batch_coarse_fused", 16)
         ctx.set_option("galerkin", 1)
     assert ok0 and ok1
     for a, b, t0, t1, case in zip(x0, x1, s0, s1, CASES):
