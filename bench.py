@@ -505,9 +505,9 @@ def main():
     ms32 = ctx.time_sweeps(phi32, rhs32, G32, 1.0, args.sweeps)
     del G32, phi32, rhs32, phi
 
-    def roof(ms_sweep, word, kind, kernel):
-        launch_ms = ms_sweep / 8.0
-        alg = B_ALG_WORDS * word * n3 / 8.0
+    def roof(ms_sweep, word, kind, kernel, launches=8):
+        launch_ms = ms_sweep / launches
+        alg = B_ALG_WORDS * word * n3 / launches
         ach = alg / (launch_ms * 1e-3) / 1e9
         tr = traffic_from_profile(n, kind)
         out = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -518,9 +518,13 @@ def main():
             out["traffic_GBs"] = round(tr / (launch_ms * 1e-3) / 1e9, 1)
             out["traffic_frac"] = round(out["traffic_GBs"] / PEAK_HBM_GBS, 4)
         return out
+    # the mixed solve's finest-level sweeps run slab by slab (option sweep_slabs, levels >= 512^3)
+    slabs = ctx.get_option("sweep_slabs")
+    slabs = slabs if slabs > 1 and n >= 512 and (n // 16) % slabs == 0 else 1
     roof_b = roof(msb, 4, "bf16", "k_colour_col<float,bf16> (finest-level smoother of the mixed solve: FP32 "
-                                  "values, bfloat16 metric; one of 8 colour classes, %d^3, 8 points per thread "
-                                  "along k)" % n)
+                                  "values, bfloat16 metric; one of 8 colour classes on one of %d k-slabs, %d^3, 8 "
+                                  "points per thread along k; %d launches per sweep)" % (slabs, n, 8 * slabs),
+                  launches=8 * slabs)
     roof_64 = roof(ms64, 8, "f64", "k_colour_col<double> (the FP64 solve's smoother, %d^3)" % n)
 
     # The reference prints per V-cycle through the Fortran runtime, which flushes at process exit:

@@ -129,6 +129,10 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 schedule 0), its residual norms and divergence guard included (bit-identical)
  *   "coarse_lds"  1 (default) / 0: the one-workgroup cycle holds its smallest levels (u, rhs, G)
  *                 in LDS while it runs (bit-identical)
+ *   "sweep_slabs" 8 (default), 2, 4, 16, 32 or 0: the mixed solve's finest-level sweeps (n >= 512)
+ *                 run all colours on one k-slab before the next (this many slabs; the reverse
+ *                 sweep in exactly reverse order).  Another Gauss-Seidel order: the preconditioner
+ *                 changes, the FP64 tolerance reached does not
  *   "colour_col"  8 (default), 1, 2, 4 or 16: Gauss-Seidel colour launches on levels
  *                 n >= "colour_col_min_n" (256 by default) update this many points per thread along
  *                 k, carrying the k+1 plane neighbours of one point to the next (1 = one point per
@@ -355,6 +359,7 @@ int mmh_time_sweeps_f(mmh_ctx*, float* phi, const float* rhs, const float* G, in
 /* The same for FP32 phi with the coefficients rounded to bfloat16 (the finest-level smoother of
  * the mixed Krylov solve, option "precond_bf16"); G is the FP64 metric (6 N^3). */
 int mmh_time_sweeps_fb(mmh_ctx*, float* phi, const float* rhs, const double* G, int n, float dx, int nsweeps, float* ms);
+/* (mmh_time_sweeps_fb sweeps in the slab order of option "sweep_slabs", as the mixed solve does.) */
 
 /* ---- host-pointer entry points (used by the Fortran drop-ins) ------------------------------- */
 int mmh_relax_host_f(mmh_ctx*, float* arr, float* resid, const float* rhs, const float* deformpot, const float* u,

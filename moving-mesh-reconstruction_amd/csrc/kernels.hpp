@@ -367,13 +367,13 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
 template <class T, bool DELTA, bool MASK, int S, class TG = T>
 __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T* __restrict__ rhs,
                                                     const TG* __restrict__ G, int n, T dx2, int kpar, int io, int jo,
-                                                    unsigned live = 0xFFu) {
+                                                    unsigned live = 0xFFu, int seg0 = 0, int nseg = 0) {
   static_assert(!(DELTA && MASK), "the masked stencil is the plain form");
   const int h = n >> 1, m = n - 1;
   const i64 hh = (i64)h * h, h3 = hh * h, n3 = 8 * h3;
   const i64 q = (i64)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (q >= hh * (h / S)) return;
-  const int qi = (int)(q % h), qj = (int)((q / h) % h), kk0 = (int)(q / hh) * S;
+  if (q >= hh * (nseg ? nseg : h / S)) return;
+  const int qi = (int)(q % h), qj = (int)((q / h) % h), kk0 = ((int)(q / hh) + seg0) * S;
   const int i = 2 * qi + io, j = 2 * qj + jo;
   const int xp = ((i + 1) & m) >> 1, xm = ((i - 1) & m) >> 1, yp = ((j + 1) & m) >> 1, ym = ((j - 1) & m) >> 1;
   // sub-lattice classes (i&1 | (j&1)<<1 | (k&1)<<2) of the point and of its neighbours

@@ -51,6 +51,8 @@ struct mmh_ctx {
   int galerkin = 1;        // Krylov preconditioner: Galerkin coarse operators (option "galerkin")
   int nu_top = 0;          // Krylov preconditioner: sweeps on the finest level, 0 = nu_pre (option "nu_top")
   int coarse_fused = 4;    // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
+  int sweep_slabs = 8;  // mixed preconditioner's finest level (n >= 512): colour sweeps slab by slab (option "sweep_slabs", 0 off)
+  int slab_now = 0;     // set while such sweeps are issued
   int coarse_lds = 1;      // one-workgroup coarse cycle: its smallest levels held in LDS (option "coarse_lds")
   int coarse_fused_ref = 16;  // the reference V-cycle's levels <= this size in one workgroup (option "coarse_fused_ref")
   int zero_start = 1;      // first sweep from u = 0 reads only the colour classes already updated (option "zero_start")
@@ -556,6 +558,27 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const TG* G, int n, T dx, int ns, 
     ns -= 2 * pairs;
   }
   const int S = (layout == 1 && n >= c->colour_col_min_n && h % c->colour_col == 0) ? c->colour_col : 1;
+  if (c->slab_now > 1 && S == 8 && !delta && n >= 512 && (h / S) % c->slab_now == 0) {
+    // Slab order (the mixed preconditioner's finest level): all 8 colours on one k-slab of the
+    // level, then the next slab; reversed, the exact reverse sequence, so the symmetric V-cycle stays
+    // symmetric.  A slab's class arrays and coefficients are a fraction of the level, and the
+    // colour launches after the first find part of them in the 256 MB Infinity Cache.  A different
+    // Gauss-Seidel order from the colour-by-colour sweep (this is a preconditioner, not the
+    // reference's relaxation); phi must hold its real values (0 for a sweep from zero).
+    const int slabs = c->slab_now, per = h / S / slabs;
+    const unsigned nbs = nblk((i64)h * h * per);
+    for (int s = 0; s < ns; ++s)
+      for (int bi = 0; bi < slabs; ++bi) {
+        const int b = reverse ? slabs - 1 - bi : bi;
+        for (int cc = 0; cc < 8; ++cc) {
+          const int col = reverse ? 7 - cc : cc;
+          k_colour_col<T, false, false, 8, TG><<<nbs, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, col >> 2, (col >> 1) & 1,
+                                                                        col & 1, 0xFFu, b * per, per);
+        }
+      }
+    LAUNCHCHK();
+    return;
+  }
   for (int s = 0; s < ns; ++s)
     for (int cc = 0, done = 0; cc < 8; ++cc) {
       const int col = reverse ? 7 - cc : cc;
@@ -1228,9 +1251,12 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   const bool zero = c->zero_start && (l > 0 || S.top_zero);
   if (Hf.Ab[l]) gal_relax<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Ab[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1], zero);
   else if (Hf.A[l]) gal_relax<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1], zero);
-  else if (Hf.Gb[l])
+  else if (Hf.Gb[l]) {
+    c->slab_now = (l == 0 && S.top_zero) ? c->sweep_slabs : 0;
     relax_dev<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Gb[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1],
-                           sc + 0, false, S.delta, zero);
+                           sc + 0, false, S.delta, zero && !c->slab_now);
+    c->slab_now = 0;
+  }
   else relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.pre(n), S.guard, Hf.rhs[l + 1], sc + 0, false,
                  S.delta, zero);
   fill<float>(c, Hf.u[l + 1], 0.f, cube(n / 2));
@@ -1239,9 +1265,12 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   LAUNCHCHK();
   if (Hf.Ab[l]) gal_sweeps<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Ab[l], n, S.post(n), S.symmetric);
   else if (Hf.A[l]) gal_sweeps<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], n, S.post(n), S.symmetric);
-  else if (Hf.Gb[l])
+  else if (Hf.Gb[l]) {
+    c->slab_now = (l == 0 && S.top_zero) ? c->sweep_slabs : 0;
     relax_dev<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Gb[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr,
                            sc + 1, S.symmetric, S.delta);
+    c->slab_now = 0;
+  }
   else relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1,
                  S.symmetric, S.delta);
   reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
@@ -1899,7 +1928,9 @@ void time_sweeps_fb(mmh_ctx* c, float* phi, const float* rhs, const double* G, i
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c->stream));
+  c->slab_now = c->sweep_slabs;  // as the mixed solve's finest level runs them
   sweeps<float, bf16>(c, ps, rs, gb, n, dx, ns, 1);
+  c->slab_now = 0;
   HIPCHK(hipEventRecord(e1, c->stream));
   HIPCHK(hipEventSynchronize(e1));
   float t = 0;
@@ -2035,6 +2066,10 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->wave_min_n = value;
     } else if (k == "batch_graph") {
       c->batch_graph = value ? 1 : 0;
+    } else if (k == "sweep_slabs") {
+      require(value == 0 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32,
+              "set_option: sweep_slabs is 0 (off) or 2, 4, 8, 16 or 32");
+      c->sweep_slabs = value;
     } else if (k == "coarse_lds") {
       c->coarse_lds = value ? 1 : 0;
     } else if (k == "batch_coarse_fused") {
@@ -2075,7 +2110,7 @@ int mmh_get_option(mmh_ctx* c, const char* key, int* value) {
         {"batch_graph", c->batch_graph}, {"batch_autotune", c->batch_autotune}, {"mesh_solve", c->mesh_solve},
         {"mesh_tol", c->mesh_tol}, {"mesh_cycles", c->mesh_cycles}, {"precond_bf16", c->precond_bf16},
         {"batch_persistent", c->batch_persistent}, {"batch_coarse_fused", c->batch_coarse_fused},
-        {"coarse_lds", c->coarse_lds}};
+        {"coarse_lds", c->coarse_lds}, {"sweep_slabs", c->sweep_slabs}};
     auto it = v.find(k);
     if (it == v.end()) throw MmhError(MMH_EINVAL, std::string("get_option: unknown key ") + key);
     *value = it->second;
