@@ -136,7 +136,15 @@ template <class T> __global__ void k_copy_strided(T* __restrict__ dst, const T* 
 // sub-lattice order -- the 8 parity classes c = i%2 + 2*(j%2) + 4*(k%2) stored one after another,
 // each (n/2)^3 in column-major order of (i/2, j/2, k/2).  A colour sweep touches one class and
 // reads six others, all at unit stride in this order.
-template <int L> __device__ __forceinline__ i64 lx(int i, int j, int k, int n) {
+// SM: the level has fewer than 2^31 points per array (the one-workgroup cycle's levels), so the
+// offset is formed in 32-bit arithmetic (the same offset, fewer instructions).
+template <int L, bool SM = false> __device__ __forceinline__ i64 lx(int i, int j, int k, int n) {
+  if constexpr (SM) {
+    if (L == 0) return (i64)(i + n * (j + n * k));
+    const int h = n >> 1;
+    const int c = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
+    return (i64)(c * (h * h * h) + (i >> 1) + h * ((j >> 1) + h * (k >> 1)));
+  }
   if (L == 0) return ix3(i, j, k, n);
   const int h = n >> 1;
   const int c = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
@@ -144,7 +152,21 @@ template <int L> __device__ __forceinline__ i64 lx(int i, int j, int k, int n) {
 }
 
 // Point of linear thread index p (0 <= p < n^3) such that consecutive p are consecutive in memory.
-template <int L> __device__ __forceinline__ void point_of(i64 p, int n, int& i, int& j, int& k) {
+template <int L, bool SM = false> __device__ __forceinline__ void point_of(i64 p, int n, int& i, int& j, int& k) {
+  if constexpr (SM) {  // n a power of two, n^3 < 2^31: shifts and masks
+    const int q = (int)p;
+    if (L == 0) {
+      const int lg = __builtin_ctz(n);
+      i = q & (n - 1); j = (q >> lg) & (n - 1); k = q >> (2 * lg);
+      return;
+    }
+    const int h = n >> 1, lg = __builtin_ctz(h), h3 = h * h * h;
+    const int c = q >> (3 * lg), r = q & (h3 - 1);
+    i = 2 * (r & (h - 1)) + (c & 1);
+    j = 2 * ((r >> lg) & (h - 1)) + ((c >> 1) & 1);
+    k = 2 * (r >> (2 * lg)) + ((c >> 2) & 1);
+    return;
+  }
   if (L == 0) {
     i = (int)(p % n); j = (int)((p / n) % n); k = (int)(p / ((i64)n * n));
     return;
@@ -254,7 +276,7 @@ __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __re
 // (j&1)) of phi may be non-zero; values of the other classes are taken as 0 without being loaded,
 // and so are the G2/G3/G5 factors of cross terms whose phi difference is then 0.  The result has
 // the value the unmasked stencil gives when those classes hold zeros (a zero may differ in sign).
-template <class T, int L, bool DELTA = false, bool MASK = false, class TG = T>
+template <class T, int L, bool DELTA = false, bool MASK = false, class TG = T, bool SM = false>
 __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const TG* __restrict__ G, i64 n3, int n, int i, int j,
                                           int k, T& t1, T& t2, T& tdiag, unsigned live = 0xFFu) {
   const TG* G1 = G;
@@ -265,8 +287,8 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const TG* _
   const TG* G6 = G + 5 * n3;
   const int m = n - 1;
   const int ip = (i + 1) & m, im = (i - 1) & m, jp = (j + 1) & m, jm = (j - 1) & m, kp = (k + 1) & m, km = (k - 1) & m;
-  const i64 c = lx<L>(i, j, k, n);
-#define X_(a, b, cc) lx<L>(a, b, cc, n)
+  const i64 c = lx<L, SM>(i, j, k, n);
+#define X_(a, b, cc) lx<L, SM>(a, b, cc, n)
   T b11p = (gl<T>(G1, X_(ip, j, k)) + gl<T>(G1, c)) / T(2);
   T b11m = (gl<T>(G1, X_(im, j, k)) + gl<T>(G1, c)) / T(2);
   T b22p = (gl<T>(G4, X_(i, jp, k)) + gl<T>(G4, c)) / T(2);
@@ -322,15 +344,21 @@ __device__ __forceinline__ void stencil19(const T* __restrict__ phi, const TG* _
 
 // Gauss-Seidel update of point q (0 <= q < (n/2)^3) of the colour class (kpar, io, jo), in place
 // (relaxplane, mmh/gauss1.f90:954-980).
-template <class T, int L, bool DELTA = false, bool MASK = false, class TG = T>
+template <class T, int L, bool DELTA = false, bool MASK = false, class TG = T, bool SM = false>
 __device__ __forceinline__ void colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ G,
                                              int n, T dx2, int kpar, int io, int jo, i64 q, unsigned live = 0xFFu) {
   const int h = n >> 1;
-  const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
+  int i, j, k;
+  if constexpr (SM) {
+    const int qq = (int)q, lg = __builtin_ctz(h);
+    i = 2 * (qq & (h - 1)) + io; j = 2 * ((qq >> lg) & (h - 1)) + jo; k = 2 * (qq >> (2 * lg)) + kpar;
+  } else {
+    i = 2 * (int)(q % h) + io; j = 2 * (int)((q / h) % h) + jo; k = 2 * (int)(q / ((i64)h * h)) + kpar;
+  }
   const i64 n3 = (i64)n * n * n;
   T t1, t2, tdiag;
-  stencil19<T, L, DELTA, MASK, TG>(phi, G, n3, n, i, j, k, t1, t2, tdiag, live);
-  const i64 c = lx<L>(i, j, k, n);
+  stencil19<T, L, DELTA, MASK, TG, SM>(phi, G, n3, n, i, j, k, t1, t2, tdiag, live);
+  const i64 c = lx<L, SM>(i, j, k, n);
   if (DELTA) phi[c] = phi[c] + (rhs[c] * dx2 - t1 - t2) / tdiag;
   else phi[c] = (rhs[c] * dx2 - t1 - t2) / tdiag;
 }
@@ -464,13 +492,13 @@ __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T
 }
 
 // Residual rhs - L[phi] at point (i, j, k) (calcresid, mmh/gauss1.f90:683-709).
-template <class T, int L, bool DELTA = false, class TG = T>
+template <class T, int L, bool DELTA = false, class TG = T, bool SM = false>
 __device__ __forceinline__ T resid_point(const T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ G,
                                         int n, T di2, int i, int j, int k) {
   const i64 n3 = (i64)n * n * n;
-  const i64 p = lx<L>(i, j, k, n);
+  const i64 p = lx<L, SM>(i, j, k, n);
   T t1, t2, tdiag;
-  stencil19<T, L, DELTA, false, TG>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
+  stencil19<T, L, DELTA, false, TG, SM>(phi, G, n3, n, i, j, k, t1, t2, tdiag);
   return DELTA ? rhs[p] - (t1 + t2) * di2 : rhs[p] - (t1 + t2 + tdiag * phi[p]) * di2;
 }
 
@@ -775,17 +803,17 @@ __global__ void k_cg_scalars(int step, int B, int it, double tol, double* __rest
 // share from fine plane 2ck-1; plane ck0 accumulates main then share, every other plane share
 // then main (the order in which r1lax's plane pipeline reaches them).  r and rc in layout L.
 // restrict_point computes coarse point q (column-major order of the coarse grid).
-template <class T, int L>
+template <class T, int L, bool SM = false>
 __device__ __forceinline__ void restrict_point(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0, i64 q) {
   const int h = n >> 1;
   // coarse points in column-major order: in layout 1 the 27 fine reads of consecutive threads are
   // then consecutive within each fine class array
   int ci, cj, ck;
-  point_of<0>(q, h, ci, cj, ck);
+  point_of<0, SM>(q, h, ci, cj, ck);
   const int kb = 2 * ck, kbp = 2 * ck + 1, ks = wrapi(2 * ck - 1, n);
   const int ib = 2 * ci, ibp = wrapi(ib + 1, n), ibm = wrapi(ib - 1, n);
   const int jb = 2 * cj, jbp = wrapi(jb + 1, n), jbm = wrapi(jb - 1, n);
-#define R_(a, b, c) r[lx<L>(a, b, c, n)]
+#define R_(a, b, c) r[lx<L, SM>(a, b, c, n)]
   T acc = 0;
   if (ck == ck0) {
     acc = acc + R_(ib, jb, kb) / T(8)
@@ -805,7 +833,7 @@ __device__ __forceinline__ void restrict_point(T* __restrict__ rc, const T* __re
       + (R_(ibp, jbp, kbp) + R_(ibp, jbm, kbp) + R_(ibm, jbp, kbp) + R_(ibm, jbm, kbp)) / T(64);
   }
 #undef R_
-  rc[lx<L>(ci, cj, ck, h)] = acc;
+  rc[lx<L, SM>(ci, cj, ck, h)] = acc;
 }
 
 template <class T, int L>
@@ -864,23 +892,24 @@ __global__ __launch_bounds__(256) void k_smooth27(T* __restrict__ out, const T* 
 // Trilinear prolongation of fine 2x2x2 cell pp (column-major cell order) in sub-lattice layout:
 // its 8 points interpolate from the coarse block {ci, ci+1} x {cj, cj+1} x {ck, ck+1}; each
 // point's 8-term sum keeps the order below.  add: u += s, else u = s.
-template <class T>
+template <class T, bool SM = false>
 __device__ __forceinline__ void prolong_cell(T* __restrict__ u, const T* __restrict__ uh, int n, int add, i64 pp) {
   const int h = n >> 1;
   int ci, cj, ck;
-  point_of<0>(pp, h, ci, cj, ck);
-  const int c1 = (ci + 1) % h, d1 = (cj + 1) % h, e1 = (ck + 1) % h;
+  point_of<0, SM>(pp, h, ci, cj, ck);
+  const int c1 = SM ? ((ci + 1) & (h - 1)) : (ci + 1) % h, d1 = SM ? ((cj + 1) & (h - 1)) : (cj + 1) % h,
+            e1 = SM ? ((ck + 1) & (h - 1)) : (ck + 1) % h;
   T v[2][2][2];
-  v[0][0][0] = uh[lx<1>(ci, cj, ck, h)]; v[0][0][1] = uh[lx<1>(ci, cj, e1, h)];
-  v[0][1][0] = uh[lx<1>(ci, d1, ck, h)]; v[0][1][1] = uh[lx<1>(ci, d1, e1, h)];
-  v[1][0][0] = uh[lx<1>(c1, cj, ck, h)]; v[1][0][1] = uh[lx<1>(c1, cj, e1, h)];
-  v[1][1][0] = uh[lx<1>(c1, d1, ck, h)]; v[1][1][1] = uh[lx<1>(c1, d1, e1, h)];
+  v[0][0][0] = uh[lx<1, SM>(ci, cj, ck, h)]; v[0][0][1] = uh[lx<1, SM>(ci, cj, e1, h)];
+  v[0][1][0] = uh[lx<1, SM>(ci, d1, ck, h)]; v[0][1][1] = uh[lx<1, SM>(ci, d1, e1, h)];
+  v[1][0][0] = uh[lx<1, SM>(c1, cj, ck, h)]; v[1][0][1] = uh[lx<1, SM>(c1, cj, e1, h)];
+  v[1][1][0] = uh[lx<1, SM>(c1, d1, ck, h)]; v[1][1][1] = uh[lx<1, SM>(c1, d1, e1, h)];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int a = c & 1, b = (c >> 1) & 1, e = c >> 2;  // i = 2ci+a: low index ci, high index ci+a
     const T s = (v[0][0][0] + v[0][0][e] + v[0][b][0] + v[0][b][e]
                  + v[a][b][0] + v[a][b][e] + v[a][0][0] + v[a][0][e]) / T(8);
-    const i64 p = lx<1>(2 * ci + a, 2 * cj + b, 2 * ck + e, n);
+    const i64 p = lx<1, SM>(2 * ci + a, 2 * cj + b, 2 * ck + e, n);
     u[p] = add ? u[p] + s : s;
   }
 }

@@ -117,7 +117,8 @@ __device__ void cw_sweeps(T* u, const T* rhs, const T* G, const T* A, int n, T d
       if (A)
         for (i64 q = threadIdx.x; q < h3; q += CW) gal_colour_point<T>(u, rhs, A, n, kp, io, jo, q);
       else
-        for (i64 q = threadIdx.x; q < h3; q += CW) colour_point<T, 1>(u, rhs, G, n, dx2, kp, io, jo, q);
+        for (i64 q = threadIdx.x; q < h3; q += CW)
+          colour_point<T, 1, false, false, T, true>(u, rhs, G, n, dx2, kp, io, jo, q);
       __syncthreads();
     }
 }
@@ -141,8 +142,9 @@ __device__ double cw_resid_norm(const T* u, const T* rhs, const T* G, const T* A
   double a = 0.0;
   for (i64 p = threadIdx.x; p < n3; p += CW) {
     int i, j, k;
-    point_of<1>(p, n, i, j, k);
-    const T rv = A ? gal_resid_point<T>(u, rhs, A, n, i, j, k) : resid_point<T, 1>(u, rhs, G, n, di2, i, j, k);
+    point_of<1, true>(p, n, i, j, k);
+    const T rv =
+        A ? gal_resid_point<T>(u, rhs, A, n, i, j, k) : resid_point<T, 1, false, T, true>(u, rhs, G, n, di2, i, j, k);
     a = dmax(a, (double)fabs(rv));
   }
   return cw_max(a, wm);
@@ -200,9 +202,9 @@ __device__ void cw_cycle(const CoarseCycle<T>& C, double (*s)[RB], double* blk, 
     double rmax = 0.0;
     for (i64 p = threadIdx.x; p < n3; p += CW) {
       int i, j, k;
-      point_of<1>(p, n, i, j, k);
+      point_of<1, true>(p, n, i, j, k);
       const T rv = C.A[l] ? gal_resid_point<T>(C.u[l], C.rhs[l], C.A[l], n, i, j, k)
-                          : resid_point<T, 1>(C.u[l], C.rhs[l], C.G[l], n, di2, i, j, k);
+                          : resid_point<T, 1, false, T, true>(C.u[l], C.rhs[l], C.G[l], n, di2, i, j, k);
       C.r[l][p] = rv;
       rmax = dmax(rmax, (double)fabs(rv));
     }
@@ -215,7 +217,7 @@ __device__ void cw_cycle(const CoarseCycle<T>& C, double (*s)[RB], double* blk, 
     const int nrmax = min(4, max(1, n / 4));
     const int nrl = pre > nrmax ? (pre % nrmax ? pre % nrmax : nrmax) : pre;
     for (i64 q = threadIdx.x; q < h3; q += CW) {
-      restrict_point<T, 1>(C.rhs[l + 1], C.r[l], n, nrl % (n / 2), q);
+      restrict_point<T, 1, true>(C.rhs[l + 1], C.r[l], n, nrl % (n / 2), q);
       C.u[l + 1][q] = T(0);
     }
     __syncthreads();
@@ -241,7 +243,7 @@ __device__ void cw_cycle(const CoarseCycle<T>& C, double (*s)[RB], double* blk, 
   for (int l = C.nl - 2; l >= 0; --l) {
     const int n = C.n[l];
     const i64 h3 = cube_d(n / 2);
-    for (i64 pp = threadIdx.x; pp < h3; pp += CW) prolong_cell<T>(C.u[l], C.u[l + 1], n, 1, pp);
+    for (i64 pp = threadIdx.x; pp < h3; pp += CW) prolong_cell<T, true>(C.u[l], C.u[l + 1], n, 1, pp);
     __syncthreads();
     cw_sweeps<T>(C.u[l], C.rhs[l], C.G[l], C.A[l], n, C.dx[l], C.sw[l], C.symmetric != 0);
     if (C.ref) {
@@ -444,9 +446,9 @@ __global__ __launch_bounds__(CW) void k_pcg_stack(PcgStack P) {
     // q = L p (the 19-point operator of the finest level)
     for (i64 t = threadIdx.x; t < n3; t += CW) {
       int i, j, k;
-      point_of<1>(t, n, i, j, k);
+      point_of<1, true>(t, n, i, j, k);
       double t1, t2, td;
-      stencil19<double, 1>(p, C.G[0], n3, n, i, j, k, t1, t2, td);
+      stencil19<double, 1, false, false, double, true>(p, C.G[0], n3, n, i, j, k, t1, t2, td);
       q[t] = (t1 + t2 + td * p[t]) / dx2;
     }
     __syncthreads();
