@@ -393,7 +393,10 @@ def test_mixed_preconditioner_1024(ctx):
     rebuilt coarse operators (option galerkin = 0): at 1024^3 the Galerkin ones fit beside the
     FP64 run's workspace for one precision only, and the comparison needs the same preconditioner.
     The mixed run is repeated with the 16-bit coefficient copies (option precond_bf16), whose
-    conversion is split the same way, with the looser bounds of their 8-bit significands."""
+    conversion is split the same way, with the looser bounds of their 8-bit significands.  These
+    three runs sweep the finest level colour by colour (option sweep_slabs = 0), so only the
+    precision differs; a fourth, 16-bit with the default slab order (another symmetric
+    Gauss-Seidel order), must stay within 1e-2 of the third."""
     n = 1024
     print("case mixed preconditioner n=%d" % n, flush=True)
     f64 = dict(dtype=torch.float64, device="cuda:0")
@@ -404,8 +407,9 @@ def test_mixed_preconditioner_1024(ctx):
     out = []
     ctx.set_option("galerkin", 0)
     try:
-        for mixed, bf in ((0, 0), (1, 0), (1, 1)):
+        for mixed, bf, slabs in ((0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 8)):
             ctx.set_option("precond_bf16", bf)
+            ctx.set_option("sweep_slabs", slabs)
             phi, rhs = torch.zeros(n ** 3, **f64), rhs0.clone()
             st, ok = ctx.solve(phi, rhs, d, rho, tol=1e-30, max_cycles=3, schedule=1, nu_pre=2, nu_post=2,
                                nu_coarse=8, mixed=mixed, nu=1, krylov=True)
@@ -415,13 +419,16 @@ def test_mixed_preconditioner_1024(ctx):
     finally:
         ctx.set_option("galerkin", 1)
         ctx.set_option("precond_bf16", 1)
-    (a, ra), (b, rb), (c, rc) = out
-    print("relres fp64 %s mixed %s mixed-bf16 %s" % (ra, rb, rc), flush=True)
+        ctx.set_option("sweep_slabs", 8)
+    (a, ra), (b, rb), (c, rc), (e, re) = out
+    print("relres fp64 %s mixed %s mixed-bf16 %s slab order %s" % (ra, rb, rc, re), flush=True)
     assert float((a - b).abs().max() / a.abs().max()) <= 1e-6
     assert all(abs(x - y) <= 1e-3 * abs(x) for x, y in zip(ra, rb))
     # 16-bit coefficients (8-bit significands) perturb the preconditioner by ~1e-3 relative
     assert float((a - c).abs().max() / a.abs().max()) <= 1e-4
     assert all(abs(x - y) <= 1e-2 * abs(x) for x, y in zip(ra, rc))
+    assert float((c - e).abs().max() / c.abs().max()) <= 1e-2
+    assert all(abs(x - y) <= 1e-2 * abs(x) for x, y in zip(rc, re))
 
 
 @pytest.mark.parametrize("mixed", [0, 1])
