@@ -123,7 +123,7 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *   "nu_top"      0 (default) or the sweeps each way on the finest level of the Krylov solve's
  *                 preconditioner (nu_pre/nu_post then apply below it); with Galerkin coarse levels
  *                 1 reaches the iteration count of 2 at 512^3
- *   "coarse_fused" 4 (default), 8, 16, 32 or 0: the preconditioner's levels up to this size run in
+ *   "coarse_fused" 16 (default), 4, 8, 32 or 0: the preconditioner's levels up to this size run in
  *                 one workgroup (bit-identical)
  *   "coarse_fused_ref" 16 (default), 4, 8, 32 or 0: the same for the reference V-cycle (multigrid,
  *                 schedule 0), its residual norms and divergence guard included (bit-identical)

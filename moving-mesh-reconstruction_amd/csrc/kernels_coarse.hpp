@@ -115,7 +115,8 @@ __device__ void cw_sweeps(T* u, const T* rhs, const T* G, const T* A, int n, T d
       const int col = rev ? 7 - cc : cc;
       const int kp = col >> 2, io = (col >> 1) & 1, jo = col & 1;
       if (A)
-        for (i64 q = threadIdx.x; q < h3; q += CW) gal_colour_point<T>(u, rhs, A, n, kp, io, jo, q);
+        for (i64 q = threadIdx.x; q < h3; q += CW)
+          gal_colour_point<T, false, T, true>(u, rhs, A, n, kp, io, jo, q);
       else
         for (i64 q = threadIdx.x; q < h3; q += CW)
           colour_point<T, 1, false, false, T, true>(u, rhs, G, n, dx2, kp, io, jo, q);
@@ -143,8 +144,8 @@ __device__ double cw_resid_norm(const T* u, const T* rhs, const T* G, const T* A
   for (i64 p = threadIdx.x; p < n3; p += CW) {
     int i, j, k;
     point_of<1, true>(p, n, i, j, k);
-    const T rv =
-        A ? gal_resid_point<T>(u, rhs, A, n, i, j, k) : resid_point<T, 1, false, T, true>(u, rhs, G, n, di2, i, j, k);
+    const T rv = A ? gal_resid_point<T, T, true>(u, rhs, A, n, i, j, k)
+                   : resid_point<T, 1, false, T, true>(u, rhs, G, n, di2, i, j, k);
     a = dmax(a, (double)fabs(rv));
   }
   return cw_max(a, wm);
@@ -203,7 +204,7 @@ __device__ void cw_cycle(const CoarseCycle<T>& C, double (*s)[RB], double* blk, 
     for (i64 p = threadIdx.x; p < n3; p += CW) {
       int i, j, k;
       point_of<1, true>(p, n, i, j, k);
-      const T rv = C.A[l] ? gal_resid_point<T>(C.u[l], C.rhs[l], C.A[l], n, i, j, k)
+      const T rv = C.A[l] ? gal_resid_point<T, T, true>(C.u[l], C.rhs[l], C.A[l], n, i, j, k)
                           : resid_point<T, 1, false, T, true>(C.u[l], C.rhs[l], C.G[l], n, di2, i, j, k);
       C.r[l][p] = rv;
       rmax = dmax(rmax, (double)fabs(rv));
@@ -267,8 +268,8 @@ template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle(CoarseCy
 // levels hundreds of times (nu = 4 NG/N: 128, 256 and 32 + 32 sweeps on 16^3, 8^3 and 4^3 at
 // NG = 512), some 3600 barrier-separated colour phases per cycle, each of which waits on the
 // latency of its loads; from LDS that wait is a fraction of an L2 round trip.  The levels marked
-// in stage[] (chosen on the host by stage_levels: 19-point levels, smallest first, while their
-// u, rhs and G fit in CLDS_BYTES) are copied in, the cycle runs on the copies with the same
+// in stage[] (chosen on the host by stage_levels: smallest first, while their u, rhs and G, or
+// Galerkin A, fit in CLDS_BYTES) are copied in, the cycle runs on the copies with the same
 // per-point functions (bit-identical results), and their u is copied back.  Residual arrays,
 // per-level scalars and guard flags stay in global memory.
 constexpr int CLDS_BYTES = 148 * 1024;
@@ -276,8 +277,8 @@ constexpr int CLDS_BYTES = 148 * 1024;
 template <class T> void stage_levels(CoarseCycle<T>& C) {
   size_t off = 0;
   for (int l = C.nl - 1; l >= 0; --l) {
-    const size_t b = 8 * (size_t)C.n[l] * C.n[l] * C.n[l] * sizeof(T);
-    C.stage[l] = (!C.A[l] && C.G[l] && off + b <= (size_t)CLDS_BYTES) ? 1 : 0;
+    const size_t b = (C.A[l] ? 29 : 8) * (size_t)C.n[l] * C.n[l] * C.n[l] * sizeof(T);
+    C.stage[l] = ((C.A[l] || C.G[l]) && off + b <= (size_t)CLDS_BYTES) ? 1 : 0;
     if (C.stage[l]) off += b;
   }
 }
@@ -297,8 +298,9 @@ template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle_lds(Coar
       T* b = reinterpret_cast<T*>(pool + off);
       C.u[l] = b;
       C.rhs[l] = b + m3;
-      C.G[l] = b + 2 * m3;
-      off += 8 * (size_t)m3 * sizeof(T);
+      if (Cg.A[l]) C.A[l] = b + 2 * m3;  // 27 coefficient fields
+      else C.G[l] = b + 2 * m3;
+      off += (Cg.A[l] ? 29 : 8) * (size_t)m3 * sizeof(T);
     }
   }
   __syncthreads();
@@ -307,12 +309,17 @@ template <class T> __global__ __launch_bounds__(CW) void k_coarse_cycle_lds(Coar
     const i64 m3 = cube_d(Cg.n[l]);
     T* u = C.u[l];
     T* rh = C.rhs[l];
-    T* g = const_cast<T*>(C.G[l]);
     for (i64 t = threadIdx.x; t < m3; t += CW) {
       u[t] = Cg.u[l][t];
       rh[t] = Cg.rhs[l][t];
     }
-    for (i64 t = threadIdx.x; t < 6 * m3; t += CW) g[t] = Cg.G[l][t];
+    if (Cg.A[l]) {
+      T* a = const_cast<T*>(C.A[l]);
+      for (i64 t = threadIdx.x; t < 27 * m3; t += CW) a[t] = Cg.A[l][t];
+    } else {
+      T* g = const_cast<T*>(C.G[l]);
+      for (i64 t = threadIdx.x; t < 6 * m3; t += CW) g[t] = Cg.G[l][t];
+    }
   }
   __syncthreads();
   cw_cycle<T>(C, s, blk, blk2, wm);

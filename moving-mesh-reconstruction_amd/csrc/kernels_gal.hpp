@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_rap27(double* __restrict__ Ac, const do
 // With MASK, neighbours in colour classes b (4 (k&1) + 2 (i&1) + (j&1)) whose bit of live is clear
 // hold zeros: their terms are left out, coefficient and value unread (the sum keeps its value; a
 // zero may differ in sign).
-template <class T, bool MASK = false, class TG = T>
+template <class T, bool MASK = false, class TG = T, bool SM = false>
 __device__ __forceinline__ T gal_offsum(const T* __restrict__ phi, const TG* __restrict__ A, int n, int i, int j, int k,
                                        i64 p, unsigned live = 0xFFu) {
   const int m = n - 1;
@@ -187,29 +187,35 @@ __device__ __forceinline__ T gal_offsum(const T* __restrict__ phi, const TG* __r
     if (e == 13) continue;
     const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
     if (MASK && !((live >> (cc ^ (4u * (ez != 0) + 2u * (ex != 0) + (ey != 0)))) & 1u)) continue;
-    s = s + gl<T>(A, (i64)e * n3 + p) * phi[lx<1>((i + ex) & m, (j + ey) & m, (k + ez) & m, n)];
+    s = s + gl<T>(A, (i64)e * n3 + p) * phi[lx<1, SM>((i + ex) & m, (j + ey) & m, (k + ez) & m, n)];
   }
   return s;
 }
 
 // Gauss-Seidel update of point q of colour class (kpar, io, jo) on a 27-point level (the 8
 // classes decouple the 27-point stencil as they do the 19-point one).
-template <class T, bool MASK = false, class TG = T>
+template <class T, bool MASK = false, class TG = T, bool SM = false>
 __device__ __forceinline__ void gal_colour_point(T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ A,
                                                  int n, int kpar, int io, int jo, i64 q, unsigned live = 0xFFu) {
   const int h = n >> 1;
-  const int i = 2 * (int)(q % h) + io, j = 2 * (int)((q / h) % h) + jo, k = 2 * (int)(q / ((i64)h * h)) + kpar;
-  const i64 p = lx<1>(i, j, k, n);
+  int i, j, k;
+  if constexpr (SM) {  // n^3 < 2^31, n a power of two: shifts and masks
+    const int qq = (int)q, lg = __builtin_ctz(h);
+    i = 2 * (qq & (h - 1)) + io; j = 2 * ((qq >> lg) & (h - 1)) + jo; k = 2 * (qq >> (2 * lg)) + kpar;
+  } else {
+    i = 2 * (int)(q % h) + io; j = 2 * (int)((q / h) % h) + jo; k = 2 * (int)(q / ((i64)h * h)) + kpar;
+  }
+  const i64 p = lx<1, SM>(i, j, k, n);
   const i64 n3 = (i64)n * n * n;
-  phi[p] = (rhs[p] - gal_offsum<T, MASK, TG>(phi, A, n, i, j, k, p, live)) / gl<T>(A, 13 * n3 + p);
+  phi[p] = (rhs[p] - gal_offsum<T, MASK, TG, SM>(phi, A, n, i, j, k, p, live)) / gl<T>(A, 13 * n3 + p);
 }
 
-template <class T, class TG = T>
+template <class T, class TG = T, bool SM = false>
 __device__ __forceinline__ T gal_resid_point(const T* __restrict__ phi, const T* __restrict__ rhs, const TG* __restrict__ A,
                                             int n, int i, int j, int k) {
-  const i64 p = lx<1>(i, j, k, n);
+  const i64 p = lx<1, SM>(i, j, k, n);
   const i64 n3 = (i64)n * n * n;
-  return rhs[p] - (gal_offsum<T, false, TG>(phi, A, n, i, j, k, p) + gl<T>(A, 13 * n3 + p) * phi[p]);
+  return rhs[p] - (gal_offsum<T, false, TG, SM>(phi, A, n, i, j, k, p) + gl<T>(A, 13 * n3 + p) * phi[p]);
 }
 
 // Batched launches: gridDim.y = B grids stored one after another (bstride = n^3 values; A holds

@@ -50,7 +50,7 @@ struct mmh_ctx {
   int march_tile = 0;      // tile shape of the k-marching kernels (option "march_tile": 0, 1, 2)
   int galerkin = 1;        // Krylov preconditioner: Galerkin coarse operators (option "galerkin")
   int nu_top = 0;          // Krylov preconditioner: sweeps on the finest level, 0 = nu_pre (option "nu_top")
-  int coarse_fused = 4;    // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
+  int coarse_fused = 16;   // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
   int sweep_slabs = 8;  // mixed preconditioner's finest level (n >= 512): colour sweeps slab by slab (option "sweep_slabs", 0 off)
   int slab_now = 0;     // set while such sweeps are issued
   int coarse_lds = 1;      // one-workgroup coarse cycle: its smallest levels held in LDS (option "coarse_lds")
@@ -643,9 +643,9 @@ void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const TG* A, T* r, int n, int n
 
 // The levels l.. of a hierarchy as one coarse cycle (k_coarse_cycle), when the schedule allows it:
 // the Krylov preconditioner's fixed symmetric schedule without divergence guard, levels up to the
-// size set by option "coarse_fused" (4 by default: with the rebuilt 19-point operators one
-// workgroup beats the launches up to 16^3, with the 27-point Galerkin operators only at 4^3; at
-// 32^3 a level's sweeps move megabytes through one CU and the launches win).
+// size set by option "coarse_fused" (16 by default: with 32-bit indexing in the workgroup and the
+// smallest levels in LDS, one workgroup beats the launches up to 16^3 with the 27-point Galerkin
+// operators too; at 32^3 a level's sweeps move megabytes through one CU and the launches win).
 // The reference schedule (kind 0: guard, nu = 4 NG/N) fuses its 19-point levels up to the size set
 // by option "coarse_fused_ref" (16 by default: the 16^3..4^3 levels run 128, 256 and 32+32 sweeps,
 // some 3000 launches per V-cycle otherwise; at 512^3 both take 88-89 ms per cycle, and fusing 32^3
@@ -1232,7 +1232,12 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
     C.dguard = Hd.guard + lb;
     C.ddx = Hd.dx[lb];
     C.dA = Hd.A[lb];
-    k_coarse_cycle<float><<<1, CW, 0, c->stream>>>(C);
+    if (c->coarse_lds) {
+      stage_levels(C);
+      k_coarse_cycle_lds<float><<<1, CW, 0, c->stream>>>(C);
+    } else {
+      k_coarse_cycle<float><<<1, CW, 0, c->stream>>>(C);
+    }
     LAUNCHCHK();
     return;
   }

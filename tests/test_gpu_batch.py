@@ -171,19 +171,21 @@ def test_single_solve_graph_replay_equals_eager(ctx, probs, mixed):
 @pytest.mark.parametrize("mixed", [0, 1])
 def test_single_solve_coarse_cycle_equals_launches(ctx, probs, mixed):
     """The preconditioner's coarse levels run in one workgroup (k_coarse_cycle, levels up to 32^3 by
-    option); every iterate must equal the launch-by-launch cycle's (the same per-point functions,
-    the same sum order)."""
+    option, with and without the smallest of them held in LDS); every iterate must equal the
+    launch-by-launch cycle's (the same per-point functions, the same sum order)."""
     print("case single-solve fused coarse cycle vs launches, mixed=%d" % mixed, flush=True)
     for (u, r, d), case in zip(probs, CASES):
         out = []
-        for fused in (32, 16, 4, 0):
+        for fused, lds in ((32, 1), (16, 1), (16, 0), (8, 1), (4, 1), (0, 1)):
             ctx.set_option("coarse_fused", fused)
+            ctx.set_option("coarse_lds", lds)
             try:
                 phi = torch.zeros_like(r)
                 st, ok = ctx.solve(phi, r.clone(), d, u, tol=1e-8, max_cycles=200, schedule=1, nu_pre=2, nu_post=2,
                                    nu_coarse=8, mixed=mixed, nu=5, krylov=True)
             finally:
-                ctx.set_option("coarse_fused", 4)
+                ctx.set_option("coarse_fused", 16)
+                ctx.set_option("coarse_lds", 1)
             assert ok, case
             out.append((phi.cpu().numpy(), [st.relres[i] for i in range(st.cycles)]))
         xl, rl = out[-1]
