@@ -167,7 +167,7 @@ void batch_vcycle(mmh_ctx* c, BatchGroup& g, int l, double* u0, double* rhs0, co
   const int pre = S.pre(m);
   if (g.A[l]) {
     batch_gal_sweeps(c, g, u, rh, l, pre, false, c->zero_start, act);
-    k_gal_resid<double><<<dim3(nblk(m3), g.B), 256, 0, c->stream>>>(g.r[l], u, rh, g.A[l], m, m3, act);
+    k_gal_resid<double><<<dim3(gal_resid_blocks(m), g.B), 256, 0, c->stream>>>(g.r[l], u, rh, g.A[l], m, m3, act);
     LAUNCHCHK();
   } else {
     batch_sweeps(c, g, u, rh, g.G[l], l, pre, false, shape, c->zero_start, act);

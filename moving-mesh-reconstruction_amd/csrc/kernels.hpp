@@ -392,93 +392,115 @@ __global__ __launch_bounds__(256) void k_colour(T* __restrict__ phi, const T* __
 // k-1 values of the next point, carried in registers instead of fetched again.  Each point's
 // arithmetic is stencil19's, term for term and in the same order (the build contracts no
 // multiply-add), so the results are bit-identical to k_colour's, MASK and DELTA forms included.
+// Occupancy the register allocator aims at: FP32 values (4-byte loads, low register need) up to 5
+// waves per SIMD, so that the scheduler keeps some 20-30 loads of a wave in flight rather than
+// waiting on each one to reach 8 waves; FP64 is faster at the compiler's own choice.
+template <class T> struct ColWaves { static constexpr int value = sizeof(T) == 4 ? 5 : 8; };
 template <class T, bool DELTA, bool MASK, int S, class TG = T>
-__global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T* __restrict__ rhs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, ColWaves<T>::value))) void k_colour_col(T* __restrict__ phi, const T* __restrict__ rhs,
                                                     const TG* __restrict__ G, int n, T dx2, int kpar, int io, int jo,
                                                     unsigned live = 0xFFu, int seg0 = 0, int nseg = 0) {
   static_assert(!(DELTA && MASK), "the masked stencil is the plain form");
-  const int h = n >> 1, m = n - 1;
-  const i64 hh = (i64)h * h, h3 = hh * h, n3 = 8 * h3;
-  const i64 q = (i64)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (q >= hh * (nseg ? nseg : h / S)) return;
-  const int qi = (int)(q % h), qj = (int)((q / h) % h), kk0 = ((int)(q / hh) + seg0) * S;
+  const int h = n >> 1, m = n - 1, hh = h * h;
+  const i64 h3 = (i64)hh * h, n3 = 8 * h3;
+  // Block b (XCD-remapped) holds 256 columns of one segment of S class planes, colour_col_blocks(n)
+  // blocks per segment: the planes are uniform over the block, so every array base below, plane
+  // offset included, is a scalar, and a lane adds only its 32-bit offset within the plane.
+  const unsigned gx = (unsigned)(hh + 255) >> 8;
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const int seg = (int)(b / gx);
+  const int r = (int)(b - (unsigned)seg * gx) * 256 + (int)threadIdx.x;
+  if (r >= hh || seg >= (nseg ? nseg : h / S)) return;
+  const int qi = r % h, qj = r / h, kk0 = (seg + seg0) * S;
   const int i = 2 * qi + io, j = 2 * qj + jo;
   const int xp = ((i + 1) & m) >> 1, xm = ((i - 1) & m) >> 1, yp = ((j + 1) & m) >> 1, ym = ((j - 1) & m) >> 1;
+  // offsets within a class plane: the column and its in-plane neighbours
+  const unsigned o0 = qi + h * qj, oxp = xp + h * qj, oxm = xm + h * qj, oyp = qi + h * yp, oym = qi + h * ym;
+  const unsigned opp = xp + h * yp, omp = xm + h * yp, opm = xp + h * ym, omm = xm + h * ym;
   // sub-lattice classes (i&1 | (j&1)<<1 | (k&1)<<2) of the point and of its neighbours
   const int c = io | (jo << 1) | (kpar << 2);
+  const i64 ox = (c ^ 1) * h3, oy = (c ^ 2) * h3, oz = (c ^ 4) * h3, oc = (i64)c * h3;
   const TG* G1 = G;
   const TG* G2 = G + n3;
   const TG* G3 = G + 2 * n3;
   const TG* G4 = G + 3 * n3;
   const TG* G5 = G + 4 * n3;
   const TG* G6 = G + 5 * n3;
-  const T* px = phi + (c ^ 1) * h3;
-  const T* py = phi + (c ^ 2) * h3;
-  const T* pz = phi + (c ^ 4) * h3;
+  const T* px = phi + ox;
+  const T* py = phi + oy;
+  const T* pz = phi + oz;
   const T* pxy = phi + (c ^ 3) * h3;
   const T* pyz = phi + (c ^ 6) * h3;
   const T* pxz = phi + (c ^ 5) * h3;
-  const i64 ox = (c ^ 1) * h3, oy = (c ^ 2) * h3, oz = (c ^ 4) * h3, oc = (i64)c * h3;
+  T* pc_ = phi + oc;
+  const T* rc_ = rhs + oc;
   // which neighbour classes may be non-zero (live is indexed by colour 4 kpar + 2 io + jo)
   const unsigned cc = 4u * kpar + 2u * io + jo;
   const bool fx = !MASK || ((live >> (cc ^ 2u)) & 1u), fy = !MASK || ((live >> (cc ^ 1u)) & 1u);
   const bool fz = !MASK || ((live >> (cc ^ 4u)) & 1u), fxy = !MASK || ((live >> (cc ^ 3u)) & 1u);
   const bool fyz = !MASK || ((live >> (cc ^ 5u)) & 1u), fxz = !MASK || ((live >> (cc ^ 6u)) & 1u);
-  auto P = [&](bool on, const T* a, i64 o) { return on ? a[o] : T(0); };
-  auto F = [&](bool on, const TG* g, i64 o) { return on ? gl<T>(g, o) : T(0); };
-  auto at = [&](int a, int b, int kq) { return a + (i64)h * (b + (i64)h * kq); };
+  // value at offset o of class plane kq of an array: the plane base is a scalar and the lane adds
+  // its offset in bytes as a 32-bit value (o * 8 < 2^32 for n <= 2048), the load's own
+  // scalar-base-plus-offset form
+  auto V = [&](const T* a, int kq, unsigned o) -> T {
+    return *(const T*)((const char*)(a + (i64)hh * kq) + (uint32_t)(o * (unsigned)sizeof(T)));
+  };
+  auto W = [&](const TG* g, int kq, unsigned o) -> T {
+    return gl<T>((const TG*)((const char*)(g + (i64)hh * kq) + (uint32_t)(o * (unsigned)sizeof(TG))), 0);
+  };
+  auto P = [&](bool on, const T* a, int kq, unsigned o) { return on ? V(a, kq, o) : T(0); };
+  auto F = [&](bool on, const TG* g, int kq, unsigned o) { return on ? W(g, kq, o) : T(0); };
   // the k-1 plane values of the first point
   const int km0 = (((2 * kk0 + kpar) - 1) & m) >> 1;
-  T g6m = gl<T>(G6, oz + at(qi, qj, km0)), pzm = P(fz, pz, at(qi, qj, km0));
-  T pyzpm = P(fyz, pyz, at(qi, yp, km0)), pyzmm = P(fyz, pyz, at(qi, ym, km0));
-  T g5m = F(fyz, G5 + oz, at(qi, qj, km0));
-  T pxzpm = P(fxz, pxz, at(xp, qj, km0)), pxzmm = P(fxz, pxz, at(xm, qj, km0));
-  T g3m = F(fxz, G3 + oz, at(qi, qj, km0));
+  T g6m = W(G6 + oz, km0, o0), pzm = P(fz, pz, km0, o0);
+  T pyzpm = P(fyz, pyz, km0, oyp), pyzmm = P(fyz, pyz, km0, oym);
+  T g5m = F(fyz, G5 + oz, km0, o0);
+  T pxzpm = P(fxz, pxz, km0, oxp), pxzmm = P(fxz, pxz, km0, oxm);
+  T g3m = F(fxz, G3 + oz, km0, o0);
   T res[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int kk = kk0 + s;
     const int kpl = (((2 * kk + kpar) + 1) & m) >> 1;
-    const i64 o = at(qi, qj, kk);
-    const T g6p = gl<T>(G6, oz + at(qi, qj, kpl)), pzp = P(fz, pz, at(qi, qj, kpl));
-    const T pyzpp = P(fyz, pyz, at(qi, yp, kpl)), pyzmp = P(fyz, pyz, at(qi, ym, kpl));
-    const T g5p = F(fyz, G5 + oz, at(qi, qj, kpl));
-    const T pxzpp = P(fxz, pxz, at(xp, qj, kpl)), pxzmp = P(fxz, pxz, at(xm, qj, kpl));
-    const T g3p = F(fxz, G3 + oz, at(qi, qj, kpl));
-    const T b11p = (gl<T>(G1, ox + at(xp, qj, kk)) + gl<T>(G1, oc + o)) / T(2);
-    const T b11m = (gl<T>(G1, ox + at(xm, qj, kk)) + gl<T>(G1, oc + o)) / T(2);
-    const T b22p = (gl<T>(G4, oy + at(qi, yp, kk)) + gl<T>(G4, oc + o)) / T(2);
-    const T b22m = (gl<T>(G4, oy + at(qi, ym, kk)) + gl<T>(G4, oc + o)) / T(2);
-    const T b33p = (g6p + gl<T>(G6, oc + o)) / T(2);
-    const T b33m = (g6m + gl<T>(G6, oc + o)) / T(2);
+    const T g6p = W(G6 + oz, kpl, o0), pzp = P(fz, pz, kpl, o0);
+    const T pyzpp = P(fyz, pyz, kpl, oyp), pyzmp = P(fyz, pyz, kpl, oym);
+    const T g5p = F(fyz, G5 + oz, kpl, o0);
+    const T pxzpp = P(fxz, pxz, kpl, oxp), pxzmp = P(fxz, pxz, kpl, oxm);
+    const T g3p = F(fxz, G3 + oz, kpl, o0);
+    const T b11p = (W(G1 + ox, kk, oxp) + W(G1 + oc, kk, o0)) / T(2);
+    const T b11m = (W(G1 + ox, kk, oxm) + W(G1 + oc, kk, o0)) / T(2);
+    const T b22p = (W(G4 + oy, kk, oyp) + W(G4 + oc, kk, o0)) / T(2);
+    const T b22m = (W(G4 + oy, kk, oym) + W(G4 + oc, kk, o0)) / T(2);
+    const T b33p = (g6p + W(G6 + oc, kk, o0)) / T(2);
+    const T b33m = (g6m + W(G6 + oc, kk, o0)) / T(2);
     const T tdiag = -(b11p + b11m + b22p + b22m + b33p + b33m);
-    const T pxp = P(fx, px, at(xp, qj, kk)), pxm = P(fx, px, at(xm, qj, kk));
-    const T pyp = P(fy, py, at(qi, yp, kk)), pym = P(fy, py, at(qi, ym, kk));
+    const T pxp = P(fx, px, kk, oxp), pxm = P(fx, px, kk, oxm);
+    const T pyp = P(fy, py, kk, oyp), pym = P(fy, py, kk, oym);
     T t1;
     if constexpr (DELTA) {
-      const T pc = phi[oc + o];
+      const T pc = V(pc_, kk, o0);
       t1 = (b11p * (pxp - pc) + b11m * (pxm - pc) + b22p * (pyp - pc) + b22m * (pym - pc) + b33p * (pzp - pc) +
             b33m * (pzm - pc));
     } else {
       t1 = (b11p * pxp + b11m * pxm + b22p * pyp + b22m * pym + b33p * pzp + b33m * pzm);
     }
-    const T pxypp = P(fxy, pxy, at(xp, yp, kk)), pxymp = P(fxy, pxy, at(xm, yp, kk));
-    const T pxypm = P(fxy, pxy, at(xp, ym, kk)), pxymm = P(fxy, pxy, at(xm, ym, kk));
-    const T t2 = ((pxypp - pxymp) * F(fxy, G2 + oy, at(qi, yp, kk))
-                - (pxypm - pxymm) * F(fxy, G2 + oy, at(qi, ym, kk))
-                + (pxypp - pxypm) * F(fxy, G2 + ox, at(xp, qj, kk))
-                - (pxymp - pxymm) * F(fxy, G2 + ox, at(xm, qj, kk))
+    const T pxypp = P(fxy, pxy, kk, opp), pxymp = P(fxy, pxy, kk, omp);
+    const T pxypm = P(fxy, pxy, kk, opm), pxymm = P(fxy, pxy, kk, omm);
+    const T t2 = ((pxypp - pxymp) * F(fxy, G2 + oy, kk, oyp)
+                - (pxypm - pxymm) * F(fxy, G2 + oy, kk, oym)
+                + (pxypp - pxypm) * F(fxy, G2 + ox, kk, oxp)
+                - (pxymp - pxymm) * F(fxy, G2 + ox, kk, oxm)
                 + (pyzpp - pyzmp) * g5p
                 - (pyzpm - pyzmm) * g5m
-                + (pyzpp - pyzpm) * F(fyz, G5 + oy, at(qi, yp, kk))
-                - (pyzmp - pyzmm) * F(fyz, G5 + oy, at(qi, ym, kk))
+                + (pyzpp - pyzpm) * F(fyz, G5 + oy, kk, oyp)
+                - (pyzmp - pyzmm) * F(fyz, G5 + oy, kk, oym)
                 + (pxzpp - pxzmp) * g3p
                 - (pxzpm - pxzmm) * g3m
-                + (pxzpp - pxzpm) * F(fxz, G3 + ox, at(xp, qj, kk))
-                - (pxzmp - pxzmm) * F(fxz, G3 + ox, at(xm, qj, kk))) / T(4);
-    if constexpr (DELTA) res[s] = phi[oc + o] + (rhs[oc + o] * dx2 - t1 - t2) / tdiag;
-    else res[s] = (rhs[oc + o] * dx2 - t1 - t2) / tdiag;
-    if constexpr (MASK) phi[oc + o] = res[s];
+                + (pxzpp - pxzpm) * F(fxz, G3 + ox, kk, oxp)
+                - (pxzmp - pxzmm) * F(fxz, G3 + ox, kk, oxm)) / T(4);
+    if constexpr (DELTA) res[s] = V(pc_, kk, o0) + (V(rc_, kk, o0) * dx2 - t1 - t2) / tdiag;
+    else res[s] = (V(rc_, kk, o0) * dx2 - t1 - t2) / tdiag;
+    if constexpr (MASK) *(T*)((char*)(pc_ + (i64)hh * kk) + (uint32_t)(o0 * (unsigned)sizeof(T))) = res[s];
     g6m = g6p; pzm = pzp; pyzpm = pyzpp; pyzmm = pyzmp; g5m = g5p; pxzpm = pxzpp; pxzmm = pxzmp; g3m = g3p;
   }
   // Stores after all loads: the compiler cannot tell that they never alias the other classes' phi,
@@ -487,8 +509,13 @@ __global__ __launch_bounds__(256) void k_colour_col(T* __restrict__ phi, const T
   // FP32 values and 16-bit coefficients too: 156 VGPRs and three times the time per launch).
   if constexpr (!MASK) {
 #pragma unroll
-    for (int s = 0; s < S; ++s) phi[oc + at(qi, qj, kk0 + s)] = res[s];
+    for (int s = 0; s < S; ++s) *(T*)((char*)(pc_ + (i64)hh * (kk0 + s)) + (uint32_t)(o0 * (unsigned)sizeof(T))) = res[s];
   }
+}
+// Workgroups of k_colour_col per segment of S class planes of an n^3 level.
+inline unsigned colour_col_blocks(int n) {
+  const int h = n >> 1;
+  return (unsigned)(h * h + 255) >> 8;
 }
 
 // Residual rhs - L[phi] at point (i, j, k) (calcresid, mmh/gauss1.f90:683-709).

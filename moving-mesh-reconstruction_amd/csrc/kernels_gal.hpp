@@ -218,6 +218,55 @@ __device__ __forceinline__ T gal_resid_point(const T* __restrict__ phi, const T*
   return rhs[p] - (gal_offsum<T, false, TG, SM>(phi, A, n, i, j, k, p) + gl<T>(A, 13 * n3 + p) * phi[p]);
 }
 
+// The launched kernels work on one sub-lattice class at a time (a colour launch's points, or one
+// class of a residual launch), so the classes of all 26 neighbours, and with them every array base
+// below, are uniform over the workgroup; a lane adds its 32-bit byte offset within a class array
+// (the load's scalar-base-plus-offset form, no 64-bit address arithmetic per load).  The products
+// are summed in the order of gal_offsum (offsets e = 0..26, the centre left out), so the results
+// are those of gal_colour_point and gal_resid_point, bit for bit.
+// Sum over the 26 off-centre offsets at the point (qi, qj, qk) of class c (in-class offset o0 =
+// qi + h qj + h^2 qk); Ac = A + c h^3 (the point's class in each coefficient field), h a power of two.
+template <class T, bool MASK, class TG>
+__device__ __forceinline__ T gal_offsum_cls(const T* __restrict__ phi, const TG* __restrict__ Ac, int h, int c, int qi,
+                                           int qj, int qk, unsigned o0, unsigned live) {
+  const int hm = h - 1;
+  const i64 h3 = (i64)h * h * h, n3 = 8 * h3;
+  const int io = c & 1, jo = (c >> 1) & 1, ko = c >> 2;
+  // in-class coordinates of the neighbours at offsets -1, 0, +1 along each axis (x + 2 qx + o
+  // lies in class x + o mod 2), scaled by the class array's strides
+  const unsigned X0 = io ? qi : (qi - 1) & hm, X2 = io ? (qi + 1) & hm : qi;
+  const unsigned Y0 = (unsigned)h * (jo ? qj : (qj - 1) & hm), Y1 = (unsigned)h * qj,
+                 Y2 = (unsigned)h * (jo ? (qj + 1) & hm : qj);
+  const unsigned hh = (unsigned)h * h;
+  const unsigned Z0 = hh * (ko ? qk : (qk - 1) & hm), Z1 = hh * qk, Z2 = hh * (ko ? (qk + 1) & hm : qk);
+  const unsigned X[3] = {X0, (unsigned)qi, X2}, Y[3] = {Y0, Y1, Y2}, Z[3] = {Z0, Z1, Z2};
+  const unsigned cc = 4u * ko + 2u * io + jo;  // colour index of the point's class
+  // All loads first, then the sum.  With MASK a left-out term still loads, from the point's own
+  // centre coefficient and value (read by the update anyway, so no extra memory traffic), and is
+  // skipped in the sum: loads behind a branch would each be waited for before the next.
+  T av[27], vv[27];
+  bool use[27];
+#pragma unroll
+  for (int e = 0; e < 27; ++e) {
+    if (e == 13) continue;
+    const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
+    use[e] = !MASK || ((live >> (cc ^ (4u * (ez != 0) + 2u * (ex != 0) + (ey != 0)))) & 1u);
+    const int cn = use[e] ? c ^ ((ex != 0) | ((ey != 0) << 1) | ((ez != 0) << 2)) : c;
+    const unsigned on = use[e] ? X[ex + 1] + Y[ey + 1] + Z[ez + 1] : o0;
+    const TG* ae = Ac + (use[e] ? (i64)e * n3 : 13 * n3);
+    av[e] = gl<T>((const TG*)((const char*)ae + (uint32_t)(o0 * (unsigned)sizeof(TG))), 0);
+    vv[e] = *(const T*)((const char*)(phi + (i64)cn * h3) + (uint32_t)(on * (unsigned)sizeof(T)));
+  }
+  T s = T(0);
+#pragma unroll
+  for (int e = 0; e < 27; ++e) {
+    if (e == 13) continue;
+    if (MASK && !use[e]) continue;
+    s = s + av[e] * vv[e];
+  }
+  return s;
+}
+
 // Batched launches: gridDim.y = B grids stored one after another (bstride = n^3 values; A holds
 // 27 n^3 per grid).  MASK: the first sweep from phi = 0 (live: classes updated earlier in it).
 template <class T, bool MASK = false, class TG = T>
@@ -225,17 +274,32 @@ __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T
                                                     int n, int kpar, int io, int jo, i64 bstride = 0,
                                                     unsigned live = 0xFFu, const int* __restrict__ act = nullptr) {
   if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
-  const int h = n >> 1;
+  const int h = n >> 1, lg = __builtin_ctz(h);
+  const i64 h3 = (i64)h * h * h, n3 = 8 * h3;
   phi += (i64)blockIdx.y * bstride;
   rhs += (i64)blockIdx.y * bstride;
   A += (i64)blockIdx.y * 27 * bstride;
   const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
-  const i64 q = (i64)blk * blockDim.x + threadIdx.x;
-  if (q >= (i64)h * h * h) return;
-  gal_colour_point<T, MASK, TG>(phi, rhs, A, n, kpar, io, jo, q, live);
+  const int q = (int)(blk * blockDim.x + threadIdx.x);  // in-class offset, h^3 <= 2^27
+  if (q >= h3) return;
+  const int c = io | (jo << 1) | (kpar << 2);
+  const int qi = q & (h - 1), qj = (q >> lg) & (h - 1), qk = q >> (2 * lg);
+  const TG* Ac = A + (i64)c * h3;
+  const uint32_t ob = (uint32_t)q * (uint32_t)sizeof(T);
+  T* pc = (T*)((char*)(phi + (i64)c * h3) + ob);
+  const T rv = *(const T*)((const char*)(rhs + (i64)c * h3) + ob);
+  const T s = gal_offsum_cls<T, MASK, TG>(phi, Ac, h, c, qi, qj, qk, (unsigned)q, live);
+  *pc = (rv - s) / gl<T>((const TG*)((const char*)(Ac + 13 * n3) + (uint32_t)q * (uint32_t)sizeof(TG)), 0);
 }
 
-// r = rhs - A phi at every point (one thread per point, sub-lattice order); r stored.
+// Workgroups of k_gal_resid for an n^3 level: gal_resid_blocks(n) / 8 per sub-lattice class.
+inline unsigned gal_resid_blocks(int n) {
+  const i64 h = n >> 1;
+  return 8u * (unsigned)((h * h * h + 255) >> 8);
+}
+
+// r = rhs - A phi at every point (one thread per point, sub-lattice order; the workgroups of one
+// class come together, gal_resid_blocks); r stored.
 template <class T, class TG = T>
 __global__ __launch_bounds__(256) void k_gal_resid(T* __restrict__ r, const T* __restrict__ phi, const T* __restrict__ rhs,
                                                    const TG* __restrict__ A, int n, i64 bstride = 0, const int* __restrict__ act = nullptr) {
@@ -244,11 +308,20 @@ __global__ __launch_bounds__(256) void k_gal_resid(T* __restrict__ r, const T* _
   phi += (i64)blockIdx.y * bstride;
   rhs += (i64)blockIdx.y * bstride;
   A += (i64)blockIdx.y * 27 * bstride;
-  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (i64)n * n * n) return;
-  int i, j, k;
-  point_of<1>(p, n, i, j, k);
-  r[p] = gal_resid_point<T, TG>(phi, rhs, A, n, i, j, k);
+  const int h = n >> 1, lg = __builtin_ctz(h);
+  const i64 h3 = (i64)h * h * h, n3 = 8 * h3;
+  const unsigned gx = (unsigned)((h3 + 255) >> 8);
+  const int c = (int)(blockIdx.x / gx);
+  const int q = (int)((blockIdx.x - (unsigned)c * gx) * blockDim.x + threadIdx.x);
+  if (q >= h3) return;
+  const int qi = q & (h - 1), qj = (q >> lg) & (h - 1), qk = q >> (2 * lg);
+  const TG* Ac = A + (i64)c * h3;
+  const uint32_t ob = (uint32_t)q * (uint32_t)sizeof(T);
+  const T s = gal_offsum_cls<T, false, TG>(phi, Ac, h, c, qi, qj, qk, (unsigned)q, 0xFFu);
+  const T a13 = gl<T>((const TG*)((const char*)(Ac + 13 * n3) + (uint32_t)q * (uint32_t)sizeof(TG)), 0);
+  const T pv = *(const T*)((const char*)(phi + (i64)c * h3) + ob);
+  const T rv = *(const T*)((const char*)(rhs + (i64)c * h3) + ob);
+  *(T*)((char*)(r + (i64)c * h3) + ob) = rv - (s + a13 * pv);
 }
 
 }  // namespace mmh

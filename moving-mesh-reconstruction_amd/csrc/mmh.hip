@@ -525,7 +525,7 @@ void colour_col_launch(mmh_ctx* c, int S, bool masked, bool delta, T* phi, const
                        int kp, int io, int jo, unsigned live) {
   const int h = n / 2;
   require(S > 1 && h % S == 0, "colour_col: points per thread must divide n/2");
-  const unsigned nb = nblk(cube(h) / S);
+  const unsigned nb = colour_col_blocks(n) * (unsigned)(h / S);
 #define COL_LAUNCH(SS)                                                                                           \
   if (masked) k_colour_col<T, false, true, SS, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo, live); \
   else if (delta) k_colour_col<T, true, false, SS, TG><<<nb, 256, 0, c->stream>>>(phi, rhs, G, n, dx2, kp, io, jo);   \
@@ -566,7 +566,7 @@ void sweeps(mmh_ctx* c, T* phi, const T* rhs, const TG* G, int n, T dx, int ns, 
     // Gauss-Seidel order from the colour-by-colour sweep (this is a preconditioner, not the
     // reference's relaxation); phi must hold its real values (0 for a sweep from zero).
     const int slabs = c->slab_now, per = h / S / slabs;
-    const unsigned nbs = nblk((i64)h * h * per);
+    const unsigned nbs = colour_col_blocks(n) * (unsigned)per;
     for (int s = 0; s < ns; ++s)
       for (int bi = 0; bi < slabs; ++bi) {
         const int b = reverse ? slabs - 1 - bi : bi;
@@ -634,7 +634,7 @@ void gal_sweeps(mmh_ctx* c, T* phi, const T* rhs, const TG* A, int n, int ns, bo
 template <class T, class TG = T>
 void gal_relax(mmh_ctx* c, T* phi, const T* rhs, const TG* A, T* r, int n, int nrelax, T* rc, bool from_zero = false) {
   gal_sweeps<T, TG>(c, phi, rhs, A, n, nrelax, false, from_zero);
-  k_gal_resid<T, TG><<<nblk(cube(n)), 256, 0, c->stream>>>(r, phi, rhs, A, n);
+  k_gal_resid<T, TG><<<gal_resid_blocks(n), 256, 0, c->stream>>>(r, phi, rhs, A, n);
   const int nrmax = std::min(4, std::max(1, n / 4));
   const int nrl = nrelax > nrmax ? (nrelax % nrmax ? nrelax % nrmax : nrmax) : nrelax;
   k_restrict_resid<T, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(rc, r, n, nrl % (n / 2));
