@@ -863,6 +863,52 @@ __device__ __forceinline__ void restrict_point(T* __restrict__ rc, const T* __re
   rc[lx<L, SM>(ci, cj, ck, h)] = acc;
 }
 
+// restrict_point of the sub-lattice layout for the launched kernel (n a power of two): the 27 fine
+// points lie in fixed classes (each index 2c, 2c + 1 or 2c - 1 has a known parity) at in-class
+// coordinates c or c - 1 per axis, so every read is a class base (uniform) plus a lane's 32-bit byte
+// offset.  The same terms in the same order as restrict_point.
+template <class T>
+__device__ __forceinline__ void restrict_point_cls(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0, int q) {
+  const int h = n >> 1, hm = h - 1, lg = __builtin_ctz(h);
+  const i64 h3 = (i64)h * h * h;
+  const int ci = q & hm, cj = (q >> lg) & hm, ck = q >> (2 * lg);
+  const unsigned hh = (unsigned)h * h;
+  const unsigned X0 = ci, X1 = (ci - 1) & hm, Y0 = (unsigned)h * cj, Y1 = (unsigned)h * ((cj - 1) & hm),
+                 Z0 = hh * ck, Z1 = hh * ((ck - 1) & hm);
+  // fine index 2c: class bit 0, coordinate c; 2c + 1: bit 1, c; 2c - 1: bit 1, c - 1
+#define IB 0, 0
+#define IBP 1, 0
+#define IBM 1, 1
+#define RR_(px, xs, py, ys, pz, zs)                                                          \
+  (*(const T*)((const char*)(r + (i64)((px) | ((py) << 1) | ((pz) << 2)) * h3) +           \
+               (uint32_t)(((xs) ? X1 : X0) + ((ys) ? Y1 : Y0) + ((zs) ? Z1 : Z0)) * (uint32_t)sizeof(T)))
+#define R_(...) RR_(__VA_ARGS__)
+  T acc = 0;
+  if (ck == ck0) {
+    acc = acc + R_(IB, IB, IB) / T(8)
+      + (R_(IBP, IB, IB) + R_(IBM, IB, IB) + R_(IB, IBP, IB) + R_(IB, IBM, IB) + R_(IB, IB, IBP)) / T(16)
+      + (R_(IBP, IBP, IB) + R_(IBP, IBM, IB) + R_(IBM, IBP, IB) + R_(IBM, IBM, IB) + R_(IBP, IB, IBP) + R_(IBM, IB, IBP) + R_(IB, IBP, IBP) + R_(IB, IBM, IBP)) / T(32)
+      + (R_(IBP, IBP, IBP) + R_(IBP, IBM, IBP) + R_(IBM, IBP, IBP) + R_(IBM, IBM, IBP)) / T(64);
+    acc = acc + R_(IB, IB, IBM) / T(16)
+      + (R_(IBP, IB, IBM) + R_(IBM, IB, IBM) + R_(IB, IBP, IBM) + R_(IB, IBM, IBM)) / T(32)
+      + (R_(IBP, IBP, IBM) + R_(IBP, IBM, IBM) + R_(IBM, IBP, IBM) + R_(IBM, IBM, IBM)) / T(64);
+  } else {
+    acc = acc + R_(IB, IB, IBM) / T(16)
+      + (R_(IBP, IB, IBM) + R_(IBM, IB, IBM) + R_(IB, IBP, IBM) + R_(IB, IBM, IBM)) / T(32)
+      + (R_(IBP, IBP, IBM) + R_(IBP, IBM, IBM) + R_(IBM, IBP, IBM) + R_(IBM, IBM, IBM)) / T(64);
+    acc = acc + R_(IB, IB, IB) / T(8)
+      + (R_(IBP, IB, IB) + R_(IBM, IB, IB) + R_(IB, IBP, IB) + R_(IB, IBM, IB) + R_(IB, IB, IBP)) / T(16)
+      + (R_(IBP, IBP, IB) + R_(IBP, IBM, IB) + R_(IBM, IBP, IB) + R_(IBM, IBM, IB) + R_(IBP, IB, IBP) + R_(IBM, IB, IBP) + R_(IB, IBP, IBP) + R_(IB, IBM, IBP)) / T(32)
+      + (R_(IBP, IBP, IBP) + R_(IBP, IBM, IBP) + R_(IBM, IBP, IBP) + R_(IBM, IBM, IBP)) / T(64);
+  }
+#undef R_
+#undef RR_
+#undef IB
+#undef IBP
+#undef IBM
+  *(T*)((char*)rc + (uint32_t)lx<1, true>(ci, cj, ck, h) * (uint32_t)sizeof(T)) = acc;
+}
+
 template <class T, int L>
 __global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, const T* __restrict__ r, int n, int ck0,
                                                         int batched = 0, const int* __restrict__ act = nullptr) {
@@ -874,7 +920,8 @@ __global__ __launch_bounds__(256) void k_restrict_resid(T* __restrict__ rc, cons
   }
   const i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (i64)h * h * h) return;
-  restrict_point<T, L>(rc, r, n, ck0, q);
+  if constexpr (L == 1) restrict_point_cls<T>(rc, r, n, ck0, (int)q);
+  else restrict_point<T, L>(rc, r, n, ck0, q);
 }
 
 // 27-point weights 1/8, 1/16, 1/32, 1/64 around fine point (i2,j2,k2): restrict/restrictu
@@ -941,6 +988,34 @@ __device__ __forceinline__ void prolong_cell(T* __restrict__ u, const T* __restr
   }
 }
 
+// prolong_cell for the launched kernel (n a power of two): fine point (2ci + a, 2cj + b, 2ck + e)
+// is element pp = ci + h (cj + h ck) of fine class a + 2b + 4e, a class base (uniform) plus the
+// lane's 32-bit byte offset; the coarse reads take 32-bit offsets too.  Same values and order.
+template <class T>
+__device__ __forceinline__ void prolong_cell_cls(T* __restrict__ u, const T* __restrict__ uh, int n, int add, int pp) {
+  const int h = n >> 1, hm = h - 1, lg = __builtin_ctz(h);
+  const i64 h3 = (i64)h * h * h;
+  const int ci = pp & hm, cj = (pp >> lg) & hm, ck = pp >> (2 * lg);
+  const int c1 = (ci + 1) & hm, d1 = (cj + 1) & hm, e1 = (ck + 1) & hm;
+  auto H = [&](int a, int b, int c) -> T {
+    return *(const T*)((const char*)uh + (uint32_t)lx<1, true>(a, b, c, h) * (uint32_t)sizeof(T));
+  };
+  T v[2][2][2];
+  v[0][0][0] = H(ci, cj, ck); v[0][0][1] = H(ci, cj, e1);
+  v[0][1][0] = H(ci, d1, ck); v[0][1][1] = H(ci, d1, e1);
+  v[1][0][0] = H(c1, cj, ck); v[1][0][1] = H(c1, cj, e1);
+  v[1][1][0] = H(c1, d1, ck); v[1][1][1] = H(c1, d1, e1);
+  const uint32_t ob = (uint32_t)pp * (uint32_t)sizeof(T);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int a = c & 1, b = (c >> 1) & 1, e = c >> 2;  // i = 2ci+a: low index ci, high index ci+a
+    const T s = (v[0][0][0] + v[0][0][e] + v[0][b][0] + v[0][b][e]
+                 + v[a][b][0] + v[a][b][e] + v[a][0][0] + v[a][0][e]) / T(8);
+    T* up = (T*)((char*)(u + (i64)c * h3) + ob);
+    *up = add ? *up + s : s;
+  }
+}
+
 template <class T, int L>
 __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __restrict__ uh, int n, int add,
                                                  int batched = 0, const int* __restrict__ act = nullptr) {
@@ -951,9 +1026,9 @@ __global__ __launch_bounds__(256) void k_prolong(T* __restrict__ u, const T* __r
     u += (i64)blockIdx.y * n * n * n;
     uh += (i64)blockIdx.y * h * h * h;
   }
-  if (L == 1) {
+  if constexpr (L == 1) {
     if (pp >= (i64)h * h * h) return;
-    prolong_cell<T>(u, uh, n, add, pp);
+    prolong_cell_cls<T>(u, uh, n, add, (int)pp);
     return;
   }
   if (pp >= (i64)n * n * n) return;
