@@ -129,7 +129,7 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 schedule 0), its residual norms and divergence guard included (bit-identical)
  *   "coarse_lds"  1 (default) / 0: the one-workgroup cycle holds its smallest levels (u, rhs, G)
  *                 in LDS while it runs (bit-identical)
- *   "sweep_slabs" 8 (default), 2, 4, 16, 32 or 0: the mixed solve's finest-level sweeps (n >= 512)
+ *   "sweep_slabs" 4 (default), 2, 8, 16, 32 or 0: the mixed solve's finest-level sweeps (n >= 512)
  *                 run all colours on one k-slab before the next (this many slabs; the reverse
  *                 sweep in exactly reverse order).  Another Gauss-Seidel order: the preconditioner
  *                 changes, the FP64 tolerance reached does not

@@ -51,7 +51,7 @@ struct mmh_ctx {
   int galerkin = 1;        // Krylov preconditioner: Galerkin coarse operators (option "galerkin")
   int nu_top = 0;          // Krylov preconditioner: sweeps on the finest level, 0 = nu_pre (option "nu_top")
   int coarse_fused = 16;   // Krylov preconditioner: levels <= this size in one workgroup (option "coarse_fused", 0 off)
-  int sweep_slabs = 8;  // mixed preconditioner's finest level (n >= 512): colour sweeps slab by slab (option "sweep_slabs", 0 off)
+  int sweep_slabs = 4;  // mixed preconditioner's finest level (n >= 512): colour sweeps slab by slab (option "sweep_slabs", 0 off)
   int slab_now = 0;     // set while such sweeps are issued
   int coarse_lds = 1;      // one-workgroup coarse cycle: its smallest levels held in LDS (option "coarse_lds")
   int coarse_fused_ref = 16;  // the reference V-cycle's levels <= this size in one workgroup (option "coarse_fused_ref")

@@ -248,7 +248,7 @@ def test_slab_ordered_sweeps_512(ctx):
             res[slabs] = (phi, st.cycles)
             print("  slabs %d: %d iterations" % (slabs, st.cycles), flush=True)
     finally:
-        ctx.set_option("sweep_slabs", 8)
+        ctx.set_option("sweep_slabs", 4)
     p0, c0 = res[0]
     for slabs in (2, 4, 8):
         p, c = res[slabs]
