@@ -641,14 +641,15 @@ template <class TZ> __device__ __forceinline__ TZ z_centred(const TZ* __restrict
   return (TZ)((double)z[p] - m);
 }
 
-// Per-block sums of z*r and z*ro in FP64 (z widened exactly from its storage type, its mean removed
-// on the fly when zsum is given): the two dot products of one flexible-CG iteration in one pass
-// over z.  Blocking of reduce(): block b covers [b*RB*RPT, (b+1)*RB*RPT), thread t adds elements t,
-// t+RB, ... of it in order, then the halving tree.  Batched (gridDim.y = B): grid y reads at
-// y*bstride and writes its block sums at y*gridDim.x (zsum applies to unbatched launches only).
-template <class TZ>
+// Per-block sums of z*r and z*ro in FP64 (z, r and ro widened exactly from their storage types, the
+// mean of z removed on the fly when zsum is given): the two dot products of one flexible-CG
+// iteration in one pass over z.  Blocking of reduce(): block b covers [b*RB*RPT, (b+1)*RB*RPT),
+// thread t adds elements t, t+RB, ... of it in order, then the halving tree.  Batched (gridDim.y =
+// B): grid y reads at y*bstride and writes its block sums at y*gridDim.x (zsum applies to
+// unbatched launches only).
+template <class TZ, class TR = double>
 __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __restrict__ b2, const TZ* __restrict__ z,
-                                             const double* __restrict__ r, const double* __restrict__ ro, i64 n,
+                                             const TR* __restrict__ r, const TR* __restrict__ ro, i64 n,
                                              const double* __restrict__ zsum = nullptr, i64 bstride = 0, const int* __restrict__ act = nullptr) {
   if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   __shared__ double s1[RB], s2[RB];
@@ -663,8 +664,8 @@ __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __
     const i64 p = base + (i64)m * RB + t;
     if (p < n) {
       const double zz = (double)z_centred<TZ>(z, p, zsum, n);
-      a1 = a1 + zz * r[p];
-      a2 = a2 + zz * ro[p];
+      a1 = a1 + zz * (double)r[p];
+      a2 = a2 + zz * (double)ro[p];
     }
   }
   a1 = tree_reduce<RB, 0>(a1, s1);
@@ -675,17 +676,19 @@ __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __
   }
 }
 
-// r -= alpha q (the previous r kept in rold); per-block max|r|.  With r32, the new r is also
-// stored rounded to FP32 (the right-hand side of an FP32 preconditioner).  x is advanced by the
-// direction update (k_cg_dir), one pass later.
+// r -= alpha q (the previous r kept in rold, or rounded to FP32 in rold32); per-block max|r|.  With
+// r32, the new r is also stored rounded to FP32 (the right-hand side of an FP32 preconditioner).
+// x is advanced by the direction update (k_cg_dir), one pass later.
 __global__ __launch_bounds__(RB) void k_cg_update(double* __restrict__ r, double* __restrict__ rold,
                                                   const double* __restrict__ q, double alpha, double* __restrict__ bmax,
-                                                  i64 n, float* __restrict__ r32 = nullptr) {
+                                                  i64 n, float* __restrict__ r32 = nullptr,
+                                                  float* __restrict__ rold32 = nullptr) {
   const i64 i = (i64)blockIdx.x * RB + threadIdx.x;
   double a = 0.0;
   if (i < n) {
     const double ro = r[i];
-    rold[i] = ro;
+    if (rold) rold[i] = ro;
+    if (rold32) rold32[i] = (float)ro;
     const double rn = ro - alpha * q[i];
     r[i] = rn;
     if (r32) r32[i] = (float)rn;

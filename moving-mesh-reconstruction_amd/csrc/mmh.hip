@@ -54,6 +54,7 @@ struct mmh_ctx {
   int sweep_slabs = 4;  // mixed preconditioner's finest level (n >= 512): colour sweeps slab by slab (option "sweep_slabs", 0 off)
   int slab_now = 0;     // set while such sweeps are issued
   int coarse_lds = 1;      // one-workgroup coarse cycle: its smallest levels held in LDS (option "coarse_lds")
+  int dots32 = 1;          // mixed Krylov solve: z.r and z.r_old from the FP32 roundings of r (option "dots32")
   int coarse_fused_ref = 16;  // the reference V-cycle's levels <= this size in one workgroup (option "coarse_fused_ref")
   int zero_start = 1;      // first sweep from u = 0 reads only the colour classes already updated (option "zero_start")
   int colour_col = 8;      // colour kernels of levels n >= colour_col_min_n: points per thread along k (option "colour_col": 1 .. 16)
@@ -365,10 +366,12 @@ void build_hier(mmh_ctx* c, Hier<T>& H, const std::string& tag, T* u, T* rhs, co
 // rest of the Krylov solve's workspace (its vectors, the FP32 hierarchy) and 4 GiB to spare, beside
 // the memory already held; otherwise the rebuilt coarse operators are used.
 // The FP32 copies counted are those build_fp32 makes (bf: 16-bit coefficients on levels >= 32^3).
-bool galerkin_fits(mmh_ctx* c, const Hier<double>& H, const std::string& tag, bool mixed, bool bf) {
+// d32: the mixed Krylov solve's previous residual is kept in FP32 (option "dots32").
+bool galerkin_fits(mmh_ctx* c, const Hier<double>& H, const std::string& tag, bool mixed, bool bf, bool d32) {
   size_t need = 0;
   const size_t n3 = (size_t)cube(H.n[0]);
-  for (const char* k : {"cg_r:8", "cg_ro:8", "cg_p:8", "cg_q:8", "det:8"}) need += ws_need(c, k, n3 * sizeof(double));
+  for (const char* k : {"cg_r:8", "cg_p:8", "cg_q:8", "det:8"}) need += ws_need(c, k, n3 * sizeof(double));
+  need += d32 ? ws_need(c, "cg_ro32:4", n3 * sizeof(float)) : ws_need(c, "cg_ro:8", n3 * sizeof(double));
   if (!mixed) need += ws_need(c, "cg_z:8", n3 * sizeof(double));
   for (int l = 0; l < H.nlev; ++l) {
     const size_t m3 = (size_t)cube(H.n[l]);
@@ -1355,8 +1358,14 @@ double dot_dev(mmh_ctx* c, const double* a, const double* b, i64 n, double* dst)
 void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int n, const Sched& S,
          const mmh_solve_opts* o, mmh_solve_stats* st, double rden, double* sums) {
   const i64 n3 = cube(n);
+  // With the FP32 preconditioner (option "dots32"), the dot products z.r and z.r_old that set alpha
+  // and beta read r and r_old rounded to FP32, as the preconditioner itself receives r: r_old is
+  // kept in FP32 only (the r update writes it), 12 B per point less per iteration.  z comes from an
+  // FP32 V-cycle, so the products carry rounding of the same order either way; the residual
+  // recurrence and the stopping test stay FP64.
+  const bool d32 = F && c->dots32;
   double* r = wsT<double>(c, "cg_r", n3);
-  double* ro = wsT<double>(c, "cg_ro", n3);
+  double* ro = d32 ? nullptr : wsT<double>(c, "cg_ro", n3);
   double* p = wsT<double>(c, "cg_p", n3);
   double* q = wsT<double>(c, "cg_q", n3);
   // z = M r: FP64, or with the FP32 preconditioner the FP32 V-cycle's own solution array, read
@@ -1365,6 +1374,7 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   double* z = F ? nullptr : wsT<double>(c, "cg_z", n3);
   float* z32 = F ? F->u[0] : nullptr;
   float* r32 = F ? F->rhs[0] : nullptr;
+  float* ro32 = d32 ? wsT<float>(c, "cg_ro32", n3) : nullptr;
   const unsigned nb = nblk(n3, RB);
   double* bm = (double*)ws_get(c, "cg_bm", sizeof(double) * nb);
   double* bs = (double*)ws_get(c, "cg_bs", sizeof(double) * nb);
@@ -1406,7 +1416,8 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   // z.r and z.ro of the current z (deterministic block sums, one read-back for both)
   auto dots = [&](const double* rr, const double* rro, double& zr, double& zro) {
     const unsigned nd = nblk(n3, RB * RPT);
-    if (F) k_dot2<float><<<nd, RB, 0, c->stream>>>(bs, bs2, z32, rr, rro, n3, zsum);
+    if (d32) k_dot2<float, float><<<nd, RB, 0, c->stream>>>(bs, bs2, z32, r32, rr == rro ? r32 : ro32, n3, zsum);
+    else if (F) k_dot2<float><<<nd, RB, 0, c->stream>>>(bs, bs2, z32, rr, rro, n3, zsum);
     else k_dot2<double><<<nd, RB, 0, c->stream>>>(bs, bs2, z, rr, rro, n3, zsum);
     LAUNCHCHK();
     reduce<double>(c, bs, nd, 0, sums + 8);
@@ -1444,7 +1455,7 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
       if (st) { st->cycles = it; st->nonneg_pLp = nonneg_pq; st->nonneg_zr = nonneg_zr; }
       throw MmhError(MMH_ENOCONV, "solve: breakdown, p.Lp = " + std::to_string(pq));
     }
-    k_cg_update<<<nb, RB, 0, c->stream>>>(r, ro, q, alpha, bm, n3, r32);
+    k_cg_update<<<nb, RB, 0, c->stream>>>(r, ro, q, alpha, bm, n3, r32, ro32);
     LAUNCHCHK();
     reduce<double>(c, bm, nb, 1, sums + 6);
     const double rr = read_scalar<double>(c, sums + 6) / rden;
@@ -1580,9 +1591,11 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
   build_hier(c, H, "sd", us, rs, defpot, ubig, nu, 1.0, n, 0);
   // The mixed Krylov solve keeps no FP64 preconditioned residual: an FP64 solve's buffer for it
   // (N^3 words, 8 GiB at 1024^3) goes before the Galerkin operators are sized.
+  const bool d32 = o->krylov && o->mixed == 1 && c->dots32;
   if (o->krylov && o->mixed) ws_free(c, "cg_z:8");
+  if (d32) ws_free(c, "cg_ro:8");  // its FP32 rounding replaces it
   const bool gal = o->krylov && c->galerkin && !S.delta &&
-                   galerkin_fits(c, H, "sd", o->mixed != 0, o->mixed == 1 && c->precond_bf16);
+                   galerkin_fits(c, H, "sd", o->mixed != 0, o->mixed == 1 && c->precond_bf16, d32);
   if (gal) {
     try {
       build_galerkin(c, H, "sd");
@@ -1626,7 +1639,8 @@ void solve_dev(mmh_ctx* c, double* u, double* rhs, const double* defpot, const d
       size_t freeb = 0, total = 0;
       HIPCHK(hipMemGetInfo(&freeb, &total));
       size_t vec = 0;
-      for (const char* k : {"cg_r:8", "cg_ro:8", "cg_p:8", "cg_q:8"}) vec += ws_need(c, k, (size_t)n3 * sizeof(double));
+      for (const char* k : {"cg_r:8", "cg_p:8", "cg_q:8"}) vec += ws_need(c, k, (size_t)n3 * sizeof(double));
+      vec += d32 ? ws_need(c, "cg_ro32:4", (size_t)n3 * sizeof(float)) : ws_need(c, "cg_ro:8", (size_t)n3 * sizeof(double));
       if (vec > 0 && freeb < vec + ((size_t)1 << 30))
         for (int l = 1; l + 1 < H.nlev; ++l)
           if (H.A[l]) {
@@ -2077,6 +2091,8 @@ int mmh_set_option(mmh_ctx* c, const char* key, int value) {
       c->sweep_slabs = value;
     } else if (k == "coarse_lds") {
       c->coarse_lds = value ? 1 : 0;
+    } else if (k == "dots32") {
+      c->dots32 = value ? 1 : 0;
     } else if (k == "batch_coarse_fused") {
       require(value == 0 || value == 4 || value == 8 || value == 16 || value == 32,
               "set_option: batch_coarse_fused is 0 (off) or the largest level size run one workgroup per grid, 4 to 32");
@@ -2115,7 +2131,8 @@ int mmh_get_option(mmh_ctx* c, const char* key, int* value) {
         {"batch_graph", c->batch_graph}, {"batch_autotune", c->batch_autotune}, {"mesh_solve", c->mesh_solve},
         {"mesh_tol", c->mesh_tol}, {"mesh_cycles", c->mesh_cycles}, {"precond_bf16", c->precond_bf16},
         {"batch_persistent", c->batch_persistent}, {"batch_coarse_fused", c->batch_coarse_fused},
-        {"coarse_lds", c->coarse_lds}, {"sweep_slabs", c->sweep_slabs}};
+        {"coarse_lds", c->coarse_lds}, {"sweep_slabs", c->sweep_slabs},
+        {"dots32", c->dots32}};
     auto it = v.find(k);
     if (it == v.end()) throw MmhError(MMH_EINVAL, std::string("get_option: unknown key ") + key);
     *value = it->second;
