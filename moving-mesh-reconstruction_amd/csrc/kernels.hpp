@@ -204,6 +204,56 @@ __global__ void k_relayout(T* __restrict__ dst, const T* __restrict__ src, int n
 // adj(A) adj(A)^T / det (lsquare, 1173-1181); A = I + Hess(def)/dx^2; det <= 1e-5 -> identity.
 // def and u are column-major; G is 6 fields (G11, G12, G13, G22, G23, G33) in layout L.
 // Batched (gridDim.y = B, batched = 1): grid y at def + y n^3, u + y nu n^3, G + y 6 n^3.
+// Metric at point (i, j, k) from its def and u neighbourhoods: dp(di, dj, dk) reads def, up(di, dj,
+// dk) reads u (component 0 of nu), st(f, v) stores field f; the expressions of cbajraw.
+template <class T, class DP, class UP, class ST>
+__device__ __forceinline__ void metric_point(DP dp, UP up, ST st, T dx, int lsquare) {
+  const T di2 = T(1) / (dx * dx);
+  const T d0 = dp(0, 0, 0);
+  T phixx = (dp(1, 0, 0) - T(2) * d0 + dp(-1, 0, 0)) * di2;
+  T phiyy = (dp(0, 1, 0) - T(2) * d0 + dp(0, -1, 0)) * di2;
+  T phizz = (dp(0, 0, 1) - T(2) * d0 + dp(0, 0, -1)) * di2;
+  T phixy = (dp(1, 1, 0) - dp(-1, 1, 0) - dp(1, -1, 0) + dp(-1, -1, 0)) * di2 / T(4);
+  T phiyz = (dp(0, 1, 1) - dp(0, 1, -1) - dp(0, -1, 1) + dp(0, -1, -1)) * di2 / T(4);
+  T phixz = (dp(1, 0, 1) - dp(-1, 0, 1) - dp(1, 0, -1) + dp(-1, 0, -1)) * di2 / T(4);
+  T a11 = T(1) + phixx, a12 = phixy, a13 = phixz, a22 = T(1) + phiyy, a23 = phiyz, a33 = T(1) + phizz;
+  T det = a11 * a22 * a33 + T(2) * a12 * a13 * a23 - (a13 * a13) * a22 - a11 * (a23 * a23) - (a12 * a12) * a33;
+  T b11 = a22 * a33 - a23 * a23;
+  T b12 = a13 * a23 - a12 * a33;
+  T b13 = a12 * a23 - a13 * a22;
+  T b22 = a11 * a33 - a13 * a13;
+  T b23 = a12 * a13 - a11 * a23;
+  T b33 = a11 * a22 - a12 * a12;
+  if (det <= T(1.e-5)) { b11 = 1; b12 = 0; b13 = 0; b22 = 1; b23 = 0; b33 = 1; det = 1; }
+  if (lsquare) {
+    st(0, (b11 * b11 + b12 * b12 + b13 * b13) / det);
+    st(1, (b11 * b12 + b12 * b22 + b13 * b23) / det);
+    st(2, (b11 * b13 + b12 * b23 + b13 * b33) / det);
+    st(3, (b12 * b12 + b22 * b22 + b23 * b23) / det);
+    st(4, (b12 * b13 + b22 * b23 + b23 * b33) / det);
+    st(5, (b13 * b13 + b23 * b23 + b33 * b33) / det);
+    return;
+  }
+  T urg = up(-1, -1, -1);
+  bool first = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (first) { first = false; continue; }
+        urg = urg + up(a - 1, b - 1, c - 1);
+      }
+  urg = urg / T(27);
+  st(0, urg * b11 / det);
+  st(1, urg * b12 / det);
+  st(2, urg * b13 / det);
+  st(3, urg * b22 / det);
+  st(4, urg * b23 / det);
+  st(5, urg * b33 / det);
+}
+
 template <class T, int L>
 __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __restrict__ def, const T* __restrict__ u,
                                                 int nu, int n, T dx, int lsquare, int batched = 0) {
@@ -217,55 +267,45 @@ __global__ __launch_bounds__(256) void k_metric(T* __restrict__ G, const T* __re
   const i64 pp = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (pp >= n3) return;
   // threads walk the column-major inputs in order; the store goes to layout L
+  if ((n & (n - 1)) == 0 && (n * n) % 256 == 0 && n <= 1024) {
+    // A workgroup's 256 points lie in one plane k (n a power of two, n^2 a multiple of 256): the
+    // plane bases of def, u and G are scalars and a lane adds its 32-bit byte offset in the plane.
+    const int lg = __builtin_ctz(n), m = n - 1;
+    const int k = (int)(((i64)blockIdx.x * blockDim.x) >> (2 * lg));
+    const int r = (int)(pp - ((i64)k << (2 * lg)));
+    const int i = r & m, j = r >> lg;
+    const int im = (i - 1) & m, ip = (i + 1) & m, jm = (j - 1) & m, jp = (j + 1) & m;
+    const int is[3] = {im, i, ip}, js[3] = {jm, j, jp};
+    auto plane = [&](int dk) { return (i64)((k + dk) & m) << (2 * lg); };
+    auto dp = [&](int di, int dj, int dk) -> T {
+      const uint32_t o = (uint32_t)(is[di + 1] + (js[dj + 1] << lg)) * (uint32_t)sizeof(T);
+      return *(const T*)((const char*)(def + plane(dk)) + o);
+    };
+    auto up = [&](int di, int dj, int dk) -> T {
+      const uint32_t o = (uint32_t)nu * (uint32_t)(is[di + 1] + (js[dj + 1] << lg)) * (uint32_t)sizeof(T);
+      return *(const T*)((const char*)(u + (i64)nu * plane(dk)) + o);
+    };
+    // store offset: layout 1 splits the plane into classes by (i & 1, j & 1) (class bit 2 = k & 1 and
+    // the class plane k >> 1 are uniform); layout 0 is the plane itself
+    const int h = n >> 1;
+    const i64 h3 = (i64)h * h * h;
+    const T* gbase = G + (L == 1 ? (i64)(4 * (k & 1)) * h3 + (i64)h * h * (k >> 1) : ((i64)k << (2 * lg)));
+    const uint32_t so = L == 1 ? (uint32_t)(((i & 1) + 2 * (j & 1)) * (uint32_t)h3 + (i >> 1) + (uint32_t)h * (j >> 1)) *
+                                     (uint32_t)sizeof(T)
+                               : (uint32_t)r * (uint32_t)sizeof(T);
+    auto st = [&](int f, T v) { *(T*)((char*)(gbase + f * n3) + so) = v; };
+    metric_point<T>(dp, up, st, dx, lsquare);
+    return;
+  }
   int i, j, k;
   point_of<0>(pp, n, i, j, k);
   const i64 p = lx<L>(i, j, k, n);
   const int ip = wrapi(i + 1, n), im = wrapi(i - 1, n), jp = wrapi(j + 1, n), jm = wrapi(j - 1, n), kp = wrapi(k + 1, n), km = wrapi(k - 1, n);
-  const T di2 = T(1) / (dx * dx);
-  const T d0 = def[ix3(i, j, k, n)];
-  T phixx = (def[ix3(ip, j, k, n)] - T(2) * d0 + def[ix3(im, j, k, n)]) * di2;
-  T phiyy = (def[ix3(i, jp, k, n)] - T(2) * d0 + def[ix3(i, jm, k, n)]) * di2;
-  T phizz = (def[ix3(i, j, kp, n)] - T(2) * d0 + def[ix3(i, j, km, n)]) * di2;
-  T phixy = (def[ix3(ip, jp, k, n)] - def[ix3(im, jp, k, n)] - def[ix3(ip, jm, k, n)] + def[ix3(im, jm, k, n)]) * di2 / T(4);
-  T phiyz = (def[ix3(i, jp, kp, n)] - def[ix3(i, jp, km, n)] - def[ix3(i, jm, kp, n)] + def[ix3(i, jm, km, n)]) * di2 / T(4);
-  T phixz = (def[ix3(ip, j, kp, n)] - def[ix3(im, j, kp, n)] - def[ix3(ip, j, km, n)] + def[ix3(im, j, km, n)]) * di2 / T(4);
-  T a11 = T(1) + phixx, a12 = phixy, a13 = phixz, a22 = T(1) + phiyy, a23 = phiyz, a33 = T(1) + phizz;
-  T det = a11 * a22 * a33 + T(2) * a12 * a13 * a23 - (a13 * a13) * a22 - a11 * (a23 * a23) - (a12 * a12) * a33;
-  T b11 = a22 * a33 - a23 * a23;
-  T b12 = a13 * a23 - a12 * a33;
-  T b13 = a12 * a23 - a13 * a22;
-  T b22 = a11 * a33 - a13 * a13;
-  T b23 = a12 * a13 - a11 * a23;
-  T b33 = a11 * a22 - a12 * a12;
-  if (det <= T(1.e-5)) { b11 = 1; b12 = 0; b13 = 0; b22 = 1; b23 = 0; b33 = 1; det = 1; }
-  if (lsquare) {
-    G[0 * n3 + p] = (b11 * b11 + b12 * b12 + b13 * b13) / det;
-    G[1 * n3 + p] = (b11 * b12 + b12 * b22 + b13 * b23) / det;
-    G[2 * n3 + p] = (b11 * b13 + b12 * b23 + b13 * b33) / det;
-    G[3 * n3 + p] = (b12 * b12 + b22 * b22 + b23 * b23) / det;
-    G[4 * n3 + p] = (b12 * b13 + b22 * b23 + b23 * b33) / det;
-    G[5 * n3 + p] = (b13 * b13 + b23 * b23 + b33 * b33) / det;
-    return;
-  }
   const int is[3] = {im, i, ip}, js[3] = {jm, j, jp}, ks[3] = {km, k, kp};
-  T urg = u[(i64)nu * ix3(im, jm, km, n)];
-  bool first = true;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (first) { first = false; continue; }
-        urg = urg + u[(i64)nu * ix3(is[a], js[b], ks[c], n)];
-      }
-  urg = urg / T(27);
-  G[0 * n3 + p] = urg * b11 / det;
-  G[1 * n3 + p] = urg * b12 / det;
-  G[2 * n3 + p] = urg * b13 / det;
-  G[3 * n3 + p] = urg * b22 / det;
-  G[4 * n3 + p] = urg * b23 / det;
-  G[5 * n3 + p] = urg * b33 / det;
+  auto dp = [&](int di, int dj, int dk) -> T { return def[ix3(is[di + 1], js[dj + 1], ks[dk + 1], n)]; };
+  auto up = [&](int di, int dj, int dk) -> T { return u[(i64)nu * ix3(is[di + 1], js[dj + 1], ks[dk + 1], n)]; };
+  auto st = [&](int f, T v) { G[f * n3 + p] = v; };
+  metric_point<T>(dp, up, st, dx, lsquare);
 }
 
 // The 19-point symmetric curvilinear operator at one point (mmh/gauss1.f90:954-978), on arrays in
