@@ -22,12 +22,13 @@ namespace mmh {
 // Coefficients of the fine 19-point operator L phi = (t1 + t2 + tdiag phi) * di2 (stencil19,
 // mmh/gauss1.f90:954-978) at point (i, j, k), read off the expression: faces from the face
 // averages of G11/G22/G33, edges from the cross terms of G12/G23/G13.
-template <class T>
+// SM: n a power of two with n^3 < 2^31, indices in 32-bit shifts and masks (the same values).
+template <class T, bool SM = false>
 __device__ __forceinline__ void coef19(const T* __restrict__ G, int n, int i, int j, int k, double di2, double a[19]) {
   const i64 n3 = (i64)n * n * n;
   const int m = n - 1;
   auto g = [&](int comp, int di, int dj, int dk) -> double {
-    return (double)G[(i64)comp * n3 + lx<1>((i + di) & m, (j + dj) & m, (k + dk) & m, n)];
+    return (double)G[(i64)comp * n3 + lx<1, SM>((i + di) & m, (j + dj) & m, (k + dk) & m, n)];
   };
   const double g1 = g(0, 0, 0, 0), g4 = g(3, 0, 0, 0), g6 = g(5, 0, 0, 0);
   const double b11p = (g(0, 1, 0, 0) + g1) / 2, b11m = (g(0, -1, 0, 0) + g1) / 2;
@@ -100,14 +101,15 @@ __device__ __forceinline__ void rap19_points(double (&acc)[27], const T* __restr
     constexpr double wr = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0) / 8.0;
     const int m = n - 1;
     double a[19];
-    coef19<T>(G, n, (2 * ci + dx) & m, (2 * cj + dy) & m, (2 * ck + dz) & m, di2, a);
+    coef19<T, true>(G, n, (2 * ci + dx) & m, (2 * cj + dy) & m, (2 * ck + dz) & m, di2, a);
     rap19_terms<FD, 0>(acc, a, wr);
     rap19_points<T, FD + 1>(acc, G, n, ci, cj, ck, di2);
   }
 }
 
 // Level 1 from the fine 19-point operator: one thread per coarse point q (column-major coarse
-// order), A1 (27 fields of (n/2)^3, sub-lattice layout of the coarse level).
+// order), A1 (27 fields of (n/2)^3, sub-lattice layout of the coarse level).  n a power of two,
+// n <= 1024 (32-bit indices).
 template <class T>
 __global__ __launch_bounds__(256) void k_rap19(double* __restrict__ A1, const T* __restrict__ G, int n, double di2,
                                                int batched = 0) {
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(256) void k_rap19(double* __restrict__ A1, const T*
   }
   if (q >= h3) return;
   int ci, cj, ck;
-  point_of<0>(q, h, ci, cj, ck);
+  point_of<0, true>(q, h, ci, cj, ck);
   double acc[27];
 #pragma unroll
   for (int e = 0; e < 27; ++e) acc[e] = 0.0;
