@@ -19,6 +19,12 @@ typedef int64_t i64;
 struct bf16 {
   unsigned short b;
 };
+// Two bfloat16 coefficients in one word: field 2f in the low half, field 2f + 1 in the high half.
+struct bf16x2 {
+  unsigned w;
+};
+__device__ __forceinline__ float bf16_lo(unsigned w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(unsigned w) { return __uint_as_float(w & 0xFFFF0000u); }
 template <class T, class TG> __device__ __forceinline__ T gl(const TG* __restrict__ p, i64 o) { return p[o]; }
 template <> __device__ __forceinline__ float gl<float, bf16>(const bf16* __restrict__ p, i64 o) {
   return __uint_as_float((unsigned)p[o].b << 16);

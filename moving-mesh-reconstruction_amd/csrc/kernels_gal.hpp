@@ -228,9 +228,11 @@ __device__ __forceinline__ T gal_resid_point(const T* __restrict__ phi, const T*
 // are those of gal_colour_point and gal_resid_point, bit for bit.
 // Sum over the 26 off-centre offsets at the point (qi, qj, qk) of class c (in-class offset o0 =
 // qi + h qj + h^2 qk); Ac = A + c h^3 (the point's class in each coefficient field), h a power of two.
+// TG = bf16x2: the 27 fields in 14 words (fields 2f, 2f + 1 in word f), read as words.  The centre
+// coefficient comes back in a13.
 template <class T, bool MASK, class TG>
 __device__ __forceinline__ T gal_offsum_cls(const T* __restrict__ phi, const TG* __restrict__ Ac, int h, int c, int qi,
-                                           int qj, int qk, unsigned o0, unsigned live) {
+                                           int qj, int qk, unsigned o0, unsigned live, T& a13) {
   const int hm = h - 1;
   const i64 h3 = (i64)h * h * h, n3 = 8 * h3;
   const int io = c & 1, jo = (c >> 1) & 1, ko = c >> 2;
@@ -248,6 +250,7 @@ __device__ __forceinline__ T gal_offsum_cls(const T* __restrict__ phi, const TG*
   // skipped in the sum: loads behind a branch would each be waited for before the next.
   T av[27], vv[27];
   bool use[27];
+  use[13] = false;
 #pragma unroll
   for (int e = 0; e < 27; ++e) {
     if (e == 13) continue;
@@ -255,9 +258,29 @@ __device__ __forceinline__ T gal_offsum_cls(const T* __restrict__ phi, const TG*
     use[e] = !MASK || ((live >> (cc ^ (4u * (ez != 0) + 2u * (ex != 0) + (ey != 0)))) & 1u);
     const int cn = use[e] ? c ^ ((ex != 0) | ((ey != 0) << 1) | ((ez != 0) << 2)) : c;
     const unsigned on = use[e] ? X[ex + 1] + Y[ey + 1] + Z[ez + 1] : o0;
-    const TG* ae = Ac + (use[e] ? (i64)e * n3 : 13 * n3);
-    av[e] = gl<T>((const TG*)((const char*)ae + (uint32_t)(o0 * (unsigned)sizeof(TG))), 0);
     vv[e] = *(const T*)((const char*)(phi + (i64)cn * h3) + (uint32_t)(on * (unsigned)sizeof(T)));
+  }
+  if constexpr (std::is_same<TG, bf16x2>::value) {
+    // word f unless neither of its fields is used (then the centre word 6, read anyway)
+    unsigned w[14];
+#pragma unroll
+    for (int f = 0; f < 14; ++f) {
+      const bool need = !MASK || use[2 * f] || (2 * f + 1 < 27 && use[2 * f + 1]);
+      const TG* af = Ac + (need ? (i64)f * n3 : 6 * n3);
+      w[f] = *(const unsigned*)((const char*)af + (uint32_t)(o0 * 4u));
+    }
+#pragma unroll
+    for (int e = 0; e < 27; ++e)
+      if (e != 13) av[e] = (e & 1) ? bf16_hi(w[e >> 1]) : bf16_lo(w[e >> 1]);
+    a13 = bf16_hi(w[6]);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 27; ++e) {
+      if (e == 13) continue;
+      const TG* ae = Ac + (use[e] ? (i64)e * n3 : 13 * n3);
+      av[e] = gl<T>((const TG*)((const char*)ae + (uint32_t)(o0 * (unsigned)sizeof(TG))), 0);
+    }
+    a13 = gl<T>((const TG*)((const char*)(Ac + 13 * n3) + (uint32_t)(o0 * (unsigned)sizeof(TG))), 0);
   }
   T s = T(0);
 #pragma unroll
@@ -277,10 +300,11 @@ __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T
                                                     unsigned live = 0xFFu, const int* __restrict__ act = nullptr) {
   if (act && !act[blockIdx.y]) return;  // a grid of a batch that has converged
   const int h = n >> 1, lg = __builtin_ctz(h);
-  const i64 h3 = (i64)h * h * h, n3 = 8 * h3;
+  const i64 h3 = (i64)h * h * h;
+  constexpr int NF = std::is_same<TG, bf16x2>::value ? 14 : 27;  // coefficient arrays per grid
   phi += (i64)blockIdx.y * bstride;
   rhs += (i64)blockIdx.y * bstride;
-  A += (i64)blockIdx.y * 27 * bstride;
+  A += (i64)blockIdx.y * NF * bstride;
   const unsigned blk = xcd_remap(blockIdx.x, gridDim.x);
   const int q = (int)(blk * blockDim.x + threadIdx.x);  // in-class offset, h^3 <= 2^27
   if (q >= h3) return;
@@ -290,8 +314,9 @@ __global__ __launch_bounds__(256) void k_gal_colour(T* __restrict__ phi, const T
   const uint32_t ob = (uint32_t)q * (uint32_t)sizeof(T);
   T* pc = (T*)((char*)(phi + (i64)c * h3) + ob);
   const T rv = *(const T*)((const char*)(rhs + (i64)c * h3) + ob);
-  const T s = gal_offsum_cls<T, MASK, TG>(phi, Ac, h, c, qi, qj, qk, (unsigned)q, live);
-  *pc = (rv - s) / gl<T>((const TG*)((const char*)(Ac + 13 * n3) + (uint32_t)q * (uint32_t)sizeof(TG)), 0);
+  T a13;
+  const T s = gal_offsum_cls<T, MASK, TG>(phi, Ac, h, c, qi, qj, qk, (unsigned)q, live, a13);
+  *pc = (rv - s) / a13;
 }
 
 // Workgroups of k_gal_resid for an n^3 level: gal_resid_blocks(n) / 8 per sub-lattice class.
@@ -309,9 +334,9 @@ __global__ __launch_bounds__(256) void k_gal_resid(T* __restrict__ r, const T* _
   r += (i64)blockIdx.y * bstride;
   phi += (i64)blockIdx.y * bstride;
   rhs += (i64)blockIdx.y * bstride;
-  A += (i64)blockIdx.y * 27 * bstride;
+  A += (i64)blockIdx.y * (std::is_same<TG, bf16x2>::value ? 14 : 27) * bstride;
   const int h = n >> 1, lg = __builtin_ctz(h);
-  const i64 h3 = (i64)h * h * h, n3 = 8 * h3;
+  const i64 h3 = (i64)h * h * h;
   const unsigned gx = (unsigned)((h3 + 255) >> 8);
   const int c = (int)(blockIdx.x / gx);
   const int q = (int)((blockIdx.x - (unsigned)c * gx) * blockDim.x + threadIdx.x);
@@ -319,8 +344,8 @@ __global__ __launch_bounds__(256) void k_gal_resid(T* __restrict__ r, const T* _
   const int qi = q & (h - 1), qj = (q >> lg) & (h - 1), qk = q >> (2 * lg);
   const TG* Ac = A + (i64)c * h3;
   const uint32_t ob = (uint32_t)q * (uint32_t)sizeof(T);
-  const T s = gal_offsum_cls<T, false, TG>(phi, Ac, h, c, qi, qj, qk, (unsigned)q, 0xFFu);
-  const T a13 = gl<T>((const TG*)((const char*)(Ac + 13 * n3) + (uint32_t)q * (uint32_t)sizeof(TG)), 0);
+  T a13;
+  const T s = gal_offsum_cls<T, false, TG>(phi, Ac, h, c, qi, qj, qk, (unsigned)q, 0xFFu, a13);
   const T pv = *(const T*)((const char*)(phi + (i64)c * h3) + ob);
   const T rv = *(const T*)((const char*)(rhs + (i64)c * h3) + ob);
   *(T*)((char*)(r + (i64)c * h3) + ob) = rv - (s + a13 * pv);

@@ -296,7 +296,7 @@ template <class T> struct Hier {
   // 16-bit copies of G / A that the FP32 preconditioner reads instead, where set (option
   // "precond_bf16"); the arithmetic stays FP32
   const bf16* Gb[MMH_MAXLEV] = {};
-  const bf16* Ab[MMH_MAXLEV] = {};
+  const bf16x2* Ab[MMH_MAXLEV] = {};  // 16-bit Galerkin coefficients, two per word
   double* scal;  // per level: [0] r1, [1] relaxerr, [2] sum; then [3*MAXLEV] rhs sums
   int* guard;    // per level, plus entry flag at [MAXLEV]
 };
@@ -379,7 +379,7 @@ bool galerkin_fits(mmh_ctx* c, const Hier<double>& H, const std::string& tag, bo
     if (l > 0) need += ws_need(c, tag + std::to_string(l) + "A:8", 27 * m3 * sizeof(double));
     if (!mixed) continue;
     const bool b16 = bf && H.n[l] >= 32;
-    if (l > 0) need += b16 ? ws_need(c, t + "Ab", 27 * m3 * sizeof(bf16)) : ws_need(c, t + "A:4", 27 * m3 * sizeof(float));
+    if (l > 0) need += b16 ? ws_need(c, t + "Ab2", 14 * m3 * sizeof(bf16x2)) : ws_need(c, t + "A:4", 27 * m3 * sizeof(float));
     if (b16 && l == 0) need += ws_need(c, t + "Gb", 6 * m3 * sizeof(bf16));
     else if (!b16) need += ws_need(c, t + "G:4", 6 * m3 * sizeof(float));
     need += ws_need(c, t + "u:4", m3 * sizeof(float)) + ws_need(c, t + "rhs:4", m3 * sizeof(float)) +
@@ -1257,7 +1257,7 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   const i64 n3 = cube(n);
   double* sc = Hf.scal + 3 * l;
   const bool zero = c->zero_start && (l > 0 || S.top_zero);
-  if (Hf.Ab[l]) gal_relax<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Ab[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1], zero);
+  if (Hf.Ab[l]) gal_relax<float, bf16x2>(c, Hf.u[l], Hf.rhs[l], Hf.Ab[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1], zero);
   else if (Hf.A[l]) gal_relax<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], Hf.r[l], n, S.pre(n), Hf.rhs[l + 1], zero);
   else if (Hf.Gb[l]) {
     c->slab_now = (l == 0 && S.top_zero) ? c->sweep_slabs : 0;
@@ -1271,7 +1271,7 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   vcycle_mixed(c, Hf, Hd, l + 1, ng_top, S);
   k_prolong<float, 1><<<nblk(cube(n / 2)), 256, 0, c->stream>>>(Hf.u[l], Hf.u[l + 1], n, 1);
   LAUNCHCHK();
-  if (Hf.Ab[l]) gal_sweeps<float, bf16>(c, Hf.u[l], Hf.rhs[l], Hf.Ab[l], n, S.post(n), S.symmetric);
+  if (Hf.Ab[l]) gal_sweeps<float, bf16x2>(c, Hf.u[l], Hf.rhs[l], Hf.Ab[l], n, S.post(n), S.symmetric);
   else if (Hf.A[l]) gal_sweeps<float>(c, Hf.u[l], Hf.rhs[l], Hf.A[l], n, S.post(n), S.symmetric);
   else if (Hf.Gb[l]) {
     c->slab_now = (l == 0 && S.top_zero) ? c->sweep_slabs : 0;
@@ -1292,6 +1292,20 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
 __global__ void k_to_bf16(bf16* __restrict__ dst, const double* __restrict__ src, i64 n) {
   const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) dst[p] = to_bf16(src[p]);
+}
+// The 27 coefficient fields of m3 points (FP64) into 14 words per point: word f of point p holds
+// fields 2f (low half) and 2f + 1 (high half; 0 for f = 13), each rounded as k_to_bf16 rounds.
+__global__ void k_to_bf16x2(bf16x2* __restrict__ dst, const double* __restrict__ src, i64 m3, int f) {
+  const i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= m3) return;
+  const unsigned lo = to_bf16(src[(i64)(2 * f) * m3 + p]).b;
+  const unsigned hi = 2 * f + 1 < 27 ? to_bf16(src[(i64)(2 * f + 1) * m3 + p]).b : 0u;
+  dst[(i64)f * m3 + p].w = lo | (hi << 16);
+}
+void convert_bf16x2(mmh_ctx* c, bf16x2* dst, const double* src, i64 m3) {
+  require(m3 <= ((i64)1 << 30), "convert_bf16x2: level too large for one launch per word");
+  for (int f = 0; f < 14; ++f) k_to_bf16x2<<<nblk(m3), 256, 0, c->stream>>>(dst, src, m3, f);
+  LAUNCHCHK();
 }
 void convert_bf16(mmh_ctx* c, bf16* dst, const double* src, i64 n) {
   constexpr i64 CH = (i64)1 << 30;  // one launch per 2^30 values (32-bit dispatch sizes)
@@ -1322,8 +1336,8 @@ void build_fp32(mmh_ctx* c, Hier<float>& F, Hier<double>& H, bool bf = false) {
     const bool b16 = bf && m >= 32;
     if (H.A[l]) {
       if (b16) {
-        bf16* a = (bf16*)ws_get(c, t + "Ab", sizeof(bf16) * 27 * cube(m));
-        convert_bf16(c, a, H.A[l], 27 * cube(m));
+        bf16x2* a = (bf16x2*)ws_get(c, t + "Ab2", sizeof(bf16x2) * 14 * cube(m));
+        convert_bf16x2(c, a, H.A[l], cube(m));
         F.Ab[l] = a;
       } else {
         F.A[l] = wsT<float>(c, t + "A", 27 * cube(m));
