@@ -129,7 +129,8 @@ int mmh_release_workspace(mmh_ctx* ctx);            /* free cached device buffer
  *                 schedule 0), its residual norms and divergence guard included (bit-identical)
  *   "dots32"      1 (default) / 0: the mixed Krylov solve (mixed = 1) forms z.r and z.r_old from
  *                 r and r_old rounded to FP32 (as its FP32 preconditioner receives r) and keeps
- *                 r_old in FP32 only; the residual recurrence and the stopping test stay FP64
+ *                 r_old in FP32 only, and the same pass forms the sum of z that removes its mean;
+ *                 the residual recurrence and the stopping test stay FP64
  *   "coarse_lds"  1 (default) / 0: the one-workgroup cycle holds its smallest levels (u, rhs, G)
  *                 in LDS while it runs (bit-identical)
  *   "sweep_slabs" 4 (default), 2, 8, 16, 32 or 0: the mixed solve's finest-level sweeps (n >= 512)

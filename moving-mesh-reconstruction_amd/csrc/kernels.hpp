@@ -722,6 +722,45 @@ __global__ __launch_bounds__(RB) void k_dot2(double* __restrict__ b1, double* __
   }
 }
 
+// Per-block sums of z, z*r, z*ro, r and ro (FP64, the inputs widened exactly; blocking of k_dot2):
+// one pass that gives the dot products of z with its mean removed, z.r - (sum z / n) sum r, and that
+// mean itself, where z's sum would otherwise take a pass of its own.
+template <class TZ, class TR>
+__global__ __launch_bounds__(RB) void k_dot5(double* __restrict__ bz, double* __restrict__ b1, double* __restrict__ b2,
+                                             double* __restrict__ br, double* __restrict__ bro, const TZ* __restrict__ z,
+                                             const TR* __restrict__ r, const TR* __restrict__ ro, i64 n) {
+  __shared__ double s[RB];
+  const int t = threadIdx.x;
+  const i64 base = (i64)blockIdx.x * (RB * RPT);
+  double az = 0.0, a1 = 0.0, a2 = 0.0, ar = 0.0, aro = 0.0;
+#pragma unroll
+  for (int m = 0; m < RPT; ++m) {
+    const i64 p = base + (i64)m * RB + t;
+    if (p < n) {
+      const double zz = (double)z[p], rr = (double)r[p], rro = (double)ro[p];
+      az = az + zz;
+      a1 = a1 + zz * rr;
+      a2 = a2 + zz * rro;
+      ar = ar + rr;
+      aro = aro + rro;
+    }
+  }
+  az = tree_reduce<RB, 0>(az, s);
+  if (t == 0) bz[blockIdx.x] = az;
+  __syncthreads();
+  a1 = tree_reduce<RB, 0>(a1, s);
+  if (t == 0) b1[blockIdx.x] = a1;
+  __syncthreads();
+  a2 = tree_reduce<RB, 0>(a2, s);
+  if (t == 0) b2[blockIdx.x] = a2;
+  __syncthreads();
+  ar = tree_reduce<RB, 0>(ar, s);
+  if (t == 0) br[blockIdx.x] = ar;
+  __syncthreads();
+  aro = tree_reduce<RB, 0>(aro, s);
+  if (t == 0) bro[blockIdx.x] = aro;
+}
+
 // r -= alpha q (the previous r kept in rold, or rounded to FP32 in rold32); per-block max|r|.  With
 // r32, the new r is also stored rounded to FP32 (the right-hand side of an FP32 preconditioner).
 // x is advanced by the direction update (k_cg_dir), one pass later.

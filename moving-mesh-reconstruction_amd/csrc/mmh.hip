@@ -309,6 +309,7 @@ struct Sched {
   bool symmetric = false;  // post-smoothing in reverse colour order (symmetric V-cycle)
   bool delta = false;      // difference form of the FP32 stencil (mixed preconditioner only)
   bool defer_top_mean = false;  // finest level: sum only, its consumers remove the mean on the fly
+  bool sum_by_dots = false;     // with defer_top_mean: not even the sum (the dot-product pass forms it)
   int nu_top = 0;  // kind 1: sweeps each way on the finest level when > 0 (option "nu_top")
   bool top_zero = false;  // the finest level's u is 0 on entry (a preconditioner's z); coarser ones always are
   int pre(int n) const { return kind == 0 ? 4 * ng / n : (nu_top > 0 && n == ng ? nu_top : nu_pre); }
@@ -1281,8 +1282,11 @@ void vcycle_mixed(mmh_ctx* c, Hier<float>& Hf, Hier<double>& Hd, int l, int ng_t
   }
   else relax_dev(c, Hf.u[l], Hf.rhs[l], Hf.G[l], Hf.r[l], n, Hf.dx[l], S.post(n), S.guard, (float*)nullptr, sc + 1,
                  S.symmetric, S.delta);
+  if (S.defer_top_mean && l == 0) {
+    if (!S.sum_by_dots) reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
+    return;
+  }
   reduce<float>(c, Hf.u[l], n3, 0, sc + 2);
-  if (S.defer_top_mean && l == 0) return;
   k_zerosum_guard<float><<<nblk(n3), 256, 0, c->stream>>>(Hf.u[l], sc + 2, sc, n3, (n == ng_top || !S.guard) ? 0 : 1,
                                                           Hf.guard + l);
   LAUNCHCHK();
@@ -1410,6 +1414,16 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   Sz.top_zero = true;  // z is cleared before each V-cycle
   Sz.defer_top_mean = n > 4 && !(F ? coarse_cycle_applies(c, *F, 0, S) : coarse_cycle_applies(c, H, 0, S));
   const double* zsum = Sz.defer_top_mean ? (F ? F->scal + 2 : H.scal + 2) : nullptr;
+  // With dots32 the mixed solve's dot-product pass also forms z's sum (k_dot5): z.r with the mean
+  // removed is z.r - (sum z / n) sum r, r and r_old being mean-free up to rounding, and the V-cycle
+  // skips its own pass over z for that sum.
+  const bool sum5 = d32 && Sz.defer_top_mean;
+  Sz.sum_by_dots = sum5;
+  const unsigned nd5 = nblk(n3, RB * RPT);
+  double* d5 = sum5 ? (double*)ws_get(c, "cg_d5", 5 * sizeof(double)) : nullptr;
+  double* bs3 = sum5 ? (double*)ws_get(c, "cg_bs3", sizeof(double) * nd5) : nullptr;
+  double* bs4 = sum5 ? (double*)ws_get(c, "cg_bs4", sizeof(double) * nd5) : nullptr;
+  double* bs5 = sum5 ? (double*)ws_get(c, "cg_bs5", sizeof(double) * nd5) : nullptr;
   auto precond_work = [&] {
     if (!F) {
       fill<double>(c, z, 0.0, n3);
@@ -1430,6 +1444,23 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
   // z.r and z.ro of the current z (deterministic block sums, one read-back for both)
   auto dots = [&](const double* rr, const double* rro, double& zr, double& zro) {
     const unsigned nd = nblk(n3, RB * RPT);
+    if (sum5) {
+      k_dot5<float, float><<<nd, RB, 0, c->stream>>>(bs3, bs, bs2, bs4, bs5, z32, r32, rr == rro ? r32 : ro32, n3);
+      LAUNCHCHK();
+      reduce<double>(c, bs3, nd, 0, d5 + 0);
+      reduce<double>(c, bs, nd, 0, d5 + 1);
+      reduce<double>(c, bs2, nd, 0, d5 + 2);
+      reduce<double>(c, bs4, nd, 0, d5 + 3);
+      reduce<double>(c, bs5, nd, 0, d5 + 4);
+      reduce<double>(c, bs3, nd, 0, F->scal + 2);  // z's sum where the direction update reads it
+      double h[5];
+      HIPCHK(hipMemcpyAsync(h, d5, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      const double m = h[0] / (double)n3;
+      zr = h[1] - m * h[3];
+      zro = h[2] - m * h[4];
+      return;
+    }
     if (d32) k_dot2<float, float><<<nd, RB, 0, c->stream>>>(bs, bs2, z32, r32, rr == rro ? r32 : ro32, n3, zsum);
     else if (F) k_dot2<float><<<nd, RB, 0, c->stream>>>(bs, bs2, z32, rr, rro, n3, zsum);
     else k_dot2<double><<<nd, RB, 0, c->stream>>>(bs, bs2, z, rr, rro, n3, zsum);
