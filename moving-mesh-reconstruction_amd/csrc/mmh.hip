@@ -1473,7 +1473,14 @@ void pcg(mmh_ctx* c, Hier<double>& H, Hier<float>* F, double* x, double* b, int 
     zr = h[0];
     zro = h[1];
   };
-  op_resid<double>(c, r, bm, x, b, H.G[0], n, 1.0);
+  // r = b - L x.  From x = 0 (a cold start) that is b itself (every product with a zero neighbour
+  // is zero; only the sign of a zero entry could differ): a max-norm of x, a quarter of the
+  // operator's bytes, decides.
+  reduce<double>(c, x, n3, 1, sums + 7);
+  if (read_scalar<double>(c, sums + 7) == 0.0)
+    HIPCHK(hipMemcpyAsync(r, b, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+  else
+    op_resid<double>(c, r, bm, x, b, H.G[0], n, 1.0);
   if (F) k_convert<double, float><<<nblk(n3), 256, 0, c->stream>>>(r32, r, n3);
   precond();
   double rz, unused;
