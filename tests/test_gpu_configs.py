@@ -256,3 +256,26 @@ def test_slab_ordered_sweeps_512(ctx):
         assert float((p - p0).abs().max() / p0.abs().max()) <= 1e-7
     ctx.release_workspace()
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_warm_start_256(ctx, mixed):
+    """The Krylov solve's first residual from a non-zero iterate (the operator pass; from x = 0 the
+    right-hand side is copied instead): from the converged solution the solve stops after one
+    iteration, and from 1.5 times it the solve returns to the same solution.  A first residual
+    taken as b would add the solution once more (2x or 2.5x it) and fail both."""
+    n = 256
+    print("case warm start n=%d mixed=%d" % (n, mixed), flush=True)
+    rho, d, rhs0 = workload(ctx, n, 2.0, 42)
+    x0 = torch.zeros(n ** 3, **F64)
+    st, ok = bench_solve(ctx, x0, rhs0.clone(), d, rho, mixed)
+    assert ok and st.relres[st.cycles - 1] <= 1e-8
+    scale = float(x0.abs().max())
+    for start, max_cycles in ((1.0, 2), (1.5, st.cycles + 3)):
+        phi = x0 * start
+        s1, ok1 = bench_solve(ctx, phi, rhs0.clone(), d, rho, mixed)
+        print("  start %.1f x: %d iterations, relres %.3e" % (start, s1.cycles, s1.relres[s1.cycles - 1]), flush=True)
+        assert ok1 and s1.relres[s1.cycles - 1] <= 1e-8 and s1.cycles <= max_cycles
+        assert float((phi - x0).abs().max()) / scale <= 1e-5
+    ctx.release_workspace()
+    torch.cuda.empty_cache()
