@@ -279,3 +279,29 @@ def test_warm_start_256(ctx, mixed):
         assert float((phi - x0).abs().max()) / scale <= 1e-5
     ctx.release_workspace()
     torch.cuda.empty_cache()
+
+
+def test_dots32_matches_fp64_dots_256(ctx):
+    """The mixed Krylov solve's dot products from the FP32 roundings of r and r_old, with z's sum
+    formed in the same pass (option dots32, k_dot5), against the FP64 residuals and the V-cycle's own
+    sum of z (k_dot2): both reach 1e-8 in the same number of iterations (within one) and agree
+    within 1e-7."""
+    n = 256
+    print("case dots32 n=%d" % n, flush=True)
+    rho, d, rhs0 = workload(ctx, n, 2.0, 42)
+    out = {}
+    try:
+        for v in (0, 1):
+            ctx.set_option("dots32", v)
+            phi = torch.zeros(n ** 3, **F64)
+            st, ok = bench_solve(ctx, phi, rhs0.clone(), d, rho, True)
+            assert ok and st.relres[st.cycles - 1] <= 1e-8
+            out[v] = (phi, st.cycles)
+            print("  dots32 %d: %d iterations" % (v, st.cycles), flush=True)
+    finally:
+        ctx.set_option("dots32", 1)
+    (p0, c0), (p1, c1) = out[0], out[1]
+    assert abs(c0 - c1) <= 1
+    assert float((p1 - p0).abs().max() / p0.abs().max()) <= 1e-7
+    ctx.release_workspace()
+    torch.cuda.empty_cache()
